@@ -1,0 +1,225 @@
+/*
+ * voila_hip.h -- C ABI of the MI355X (gfx950) backend for VOILA's data-parallel
+ * operator path: selection, hash-table build/probe with bucket chaining,
+ * gather/scatter and hash group-by aggregation.
+ *
+ * Conventions (stricter than the reference, which aborts on ASSERT):
+ *   - every entry point returns an int status, VH_OK == 0;
+ *   - device buffers are caller owned unless stated; scratch is library owned
+ *     and released by the matching *_free / *_reset;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *   - one host thread per handle.
+ *
+ * Each declaration cites the reference interface it replaces (file:line in
+ * t1mm3/vldb_voila).
+ */
+#ifndef VOILA_HIP_H
+#define VOILA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types                                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Type codes in the order of the reference's TypeCode (runtime.hpp:270-275). */
+enum vh_type {
+	VH_INVALID = 0,
+	VH_U8, VH_I8, VH_U16, VH_I16, VH_U32, VH_I32, VH_U64, VH_I64, VH_I128
+};
+
+enum vh_status {
+	VH_OK = 0,
+	VH_ERR_INVALID = 1,      /* bad argument / descriptor                  */
+	VH_ERR_UNSUPPORTED = 2,  /* shape the backend does not implement       */
+	VH_ERR_NOMEM = 3,        /* device allocation failed                   */
+	VH_ERR_HIP = 4,          /* HIP runtime error (see vh_last_error)      */
+	VH_ERR_CAPACITY = 5,     /* a table or buffer limit was exceeded        */
+	VH_ERR_STATE = 6,        /* call out of lifecycle order                */
+	VH_ERR_NODEVICE = 7      /* no usable GPU                              */
+};
+
+/* Comparison primitives vec_{lt,le,gt,ge,eq,ne} (kernel_generator.py:415-454). */
+enum vh_cmp { VH_LT = 0, VH_LE, VH_GT, VH_GE, VH_EQ, VH_NE };
+
+/* Arithmetic primitives vec_{add,sub,mul} (kernel_generator.py:439-445). */
+enum vh_expr_op { VH_E_COL = 0, VH_E_CONST, VH_E_ADD, VH_E_SUB, VH_E_MUL };
+
+/* aggr_sum / aggr_count (kernel_generator.py:289-312). */
+enum vh_aggr_fn { VH_A_SUM = 0, VH_A_COUNT };
+
+/* relalg::HashJoin::Variant (relalg.hpp:137-140). */
+enum vh_join_variant { VH_JOIN01 = 0, VH_JOINN };
+
+#define VH_MAX_COLS     8
+#define VH_MAX_PREDS    4
+#define VH_MAX_JOINS    4
+#define VH_MAX_PAYLOAD  2
+#define VH_MAX_KEYS     4
+#define VH_MAX_AGGRS    8
+#define VH_MAX_EXPR     32
+
+/* A base column: runtime::Attribute {data, type, minmax} (db-engine-paradigms
+ * include/common/runtime/Database.hpp:28-75, Types.hpp:11-35).  min/max drive
+ * the typing of every derived value, as in typing_pass.cpp:309-326. */
+typedef struct vh_column {
+	int32_t type;          /* enum vh_type of the stored values            */
+	int32_t reserved;
+	const void* data;      /* device pointer (GPU entry points)            */
+	int64_t min;
+	int64_t max;
+} vh_column;
+
+/* Conjunctive column-vs-constant predicate: `select(cmp(col, const))`
+ * (relalg_translator.cpp:343-378, kernel_generator.py:448-454). */
+typedef struct vh_pred {
+	int32_t cmp;           /* enum vh_cmp                                  */
+	int32_t col;           /* column index in the owning table             */
+	int64_t value;
+} vh_pred;
+
+/* Expression node.  Nodes form a DAG stored children-first:
+ *   VH_E_COL   : a = virtual column index
+ *   VH_E_CONST : value
+ *   VH_E_ADD/SUB/MUL : a, b = child node indices (< this node's index)   */
+typedef struct vh_expr {
+	int32_t op;
+	int32_t a;
+	int32_t b;
+	int32_t reserved;
+	int64_t value;
+} vh_expr;
+
+/* A scanned base table with an optional conjunctive filter. */
+typedef struct vh_table_desc {
+	int64_t rows;
+	int32_t ncols;
+	int32_t npreds;
+	vh_column cols[VH_MAX_COLS];
+	vh_pred preds[VH_MAX_PREDS];
+} vh_table_desc;
+
+/* HashJoin of the running probe stream with a build table
+ * (relalg.hpp:135-165, relalg_translator.cpp:812-1115).  The build side is
+ * materialised (write), indexed (bucket_build) and probed with
+ * bucket_lookup / check / gather / bucket_next.  The payload columns become
+ * new virtual columns of the probe stream, in join order. */
+typedef struct vh_join_desc {
+	vh_table_desc build;
+	int32_t build_key;                  /* build column index (single key)   */
+	int32_t probe_key;                  /* virtual column of the probe stream */
+	int32_t variant;                    /* enum vh_join_variant              */
+	int32_t npayload;
+	int32_t payload[VH_MAX_PAYLOAD];    /* build column indices              */
+} vh_join_desc;
+
+typedef struct vh_aggr_desc {
+	int32_t fn;            /* enum vh_aggr_fn                              */
+	int32_t expr;          /* root expression node for VH_A_SUM            */
+} vh_aggr_desc;
+
+/*
+ * One pipeline: scan(probe) -> select -> join* -> HashAggr(keys, aggregates).
+ * Virtual columns: [0, probe.ncols) are the probe columns, followed by the
+ * payload columns of joins[0], joins[1], ... in order.
+ * nkeys == 0 is HashAggr::Global (relalg.hpp:120-123).
+ * Output rows: keys, then aggregates, as relalg_translator.cpp:736-747 emits
+ * them; a group exists iff its implicit count > 0 (:698-701).
+ */
+typedef struct vh_pipeline_desc {
+	vh_table_desc probe;
+	int32_t njoins;
+	int32_t nkeys;
+	int32_t naggrs;
+	int32_t nexpr;
+	vh_join_desc joins[VH_MAX_JOINS];
+	int32_t keys[VH_MAX_KEYS];
+	vh_aggr_desc aggrs[VH_MAX_AGGRS];
+	vh_expr expr[VH_MAX_EXPR];
+} vh_pipeline_desc;
+
+
+/* ------------------------------------------------------------------------ */
+/* Variant descriptor: the knobs the autotuner searches.  The reference's    */
+/* analog is BlendConfig{computation_type, concurrent_fsms, prefetch}        */
+/* (blend_space_point.hpp:10-45).  -1 / 0 select the built-in heuristics.   */
+/* ------------------------------------------------------------------------ */
+
+enum vh_group_mode {
+	VH_GROUP_AUTO = -1,
+	VH_GROUP_REG = 0,      /* <= 8 dense groups in per-lane registers         */
+	VH_GROUP_LDS = 1,      /* dense groups in a per-block LDS table           */
+	VH_GROUP_GLOBAL = 2,   /* dense groups, global atomics                    */
+	VH_GROUP_HASH = 3      /* packed key, global CAS open addressing          */
+};
+
+typedef struct vh_config {
+	int32_t group_mode;    /* enum vh_group_mode                               */
+	int32_t grid;          /* probe-kernel workgroups, 0 = auto                */
+	int32_t table_shift;   /* join table size vs calc_num_buckets: <<n / >>-n  */
+	int32_t reserved[5];
+} vh_config;
+
+/* ------------------------------------------------------------------------ */
+/* Runtime: device, memory, errors (plumbing for callers without a HIP       */
+/* runtime of their own).                                                     */
+/* ------------------------------------------------------------------------ */
+
+const char* vh_last_error(void);
+int vh_init(int device);
+int vh_malloc(void** ptr, size_t bytes);
+int vh_free(void* ptr);
+int vh_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int vh_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int vh_synchronize(void);
+int vh_stream_create(void** stream);
+int vh_stream_destroy(void* stream);
+int vh_event_create(void** event);
+int vh_event_destroy(void* event);
+int vh_event_record(void* event, void* stream);
+int vh_event_elapsed_ms(float* ms, void* start, void* stop);
+int vh_stream_synchronize(void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Query lifecycle.  Mirrors the generated library's ABI                     */
+/*   voila_query_new / voila_query_run / voila_query_reset / voila_query_free */
+/* (emitted at codegen.cpp:371-392, bound by dlsym at compiler.cpp:475-491). */
+/* The query owns its hash tables and scratch; the caller owns the columns   */
+/* (device pointers in the descriptor, borrowed for the query's lifetime).   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vh_query vh_query;
+
+int vh_query_new(vh_query** q, const vh_pipeline_desc* desc, const vh_config* cfg);
+/* Enqueue one full run (clear tables, build every join, probe + aggregate)
+ * on `stream`; asynchronous, capturable in a hipGraph. */
+int vh_query_run(vh_query* q, void* stream);
+/* One phase of vh_query_run, in order: 0 = clear tables and flags,
+ * 1 = build every join table (pipelines A+B of relalg_translator.cpp:828-928),
+ * 2 = fused probe + aggregate (pipeline C, :933-1111, and the group-by).
+ * Lets a caller time the probe kernel alone; the result is the same. */
+int vh_query_run_phase(vh_query* q, void* stream, int phase);
+/* Synchronise `stream` and render QueryResult text: one row per group,
+ * '|'-separated, numbers as voila_cast<T>::to_cstr prints them
+ * (runtime_framework.cpp:56-111, runtime.hpp:348-387).  *len = bytes
+ * needed (excluding NUL); VH_ERR_CAPACITY if `cap` is too small. */
+int vh_query_result(vh_query* q, void* stream, char* buf, size_t cap, size_t* len,
+	int64_t* nrows);
+int vh_query_reset(vh_query* q);
+int vh_query_free(vh_query* q);
+/* Mode the lowering chose (enum vh_group_mode) and the probe-kernel grid. */
+int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t* groups);
+
+/* Type the reference's TypingPass gives [min, max] (typing_pass.cpp:9-22). */
+int vh_type_from_minmax(double min, double max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOILA_HIP_H */

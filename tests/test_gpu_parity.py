@@ -1,0 +1,206 @@
+"""GPU parity: the fused gfx950 path (through the C ABI) against the CPU
+oracle, bit-exact on sorted QueryResult rows."""
+import numpy as np
+import pytest
+
+import oracle
+from vldb_voila_amd import abi, datagen, runtime, workloads
+from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
+
+pytestmark = pytest.mark.gpu
+
+MODES = [abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH]
+
+
+def gpu_rows(builder_fn, mode=abi.VH_GROUP_AUTO, **cfg):
+    """builder_fn(addr_map) -> Pipeline over device copies."""
+    return oracle.sorted_rows(runtime.run_pipeline(builder_fn(), runtime.make_config(group_mode=mode, **cfg)))
+
+
+def on_gpu(arrays_list, make):
+    devs = [runtime.upload(a) for a in arrays_list]
+    return make(*[runtime.addrs(d) for d in devs]), devs
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    print("gpu parity: init device 0", flush=True)
+    runtime.init(0)
+
+
+@pytest.mark.parametrize("mode", [abi.VH_GROUP_AUTO, abi.VH_GROUP_REG] + MODES[1:])
+def test_filter_groupby_small(mode):
+    print(f"filter_groupby 300k mode={mode}", flush=True)
+    t = datagen.gen_filter_groupby(300_000, seed=11)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.filter_groupby_pipeline(t).desc()))
+    p, devs = on_gpu([t], lambda a: workloads.filter_groupby_pipeline(t, a))
+    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+    assert got == want
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_join_aggregate_small(mode):
+    print(f"join_aggregate 50k/200k mode={mode}", flush=True)
+    b, pr = datagen.gen_join_aggregate(50_000, 200_000, seed=13)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+    assert got == want
+
+
+def test_filter_groupby_full_size():
+    print("filter_groupby 6M rows seed 42", flush=True)
+    t = datagen.gen_filter_groupby(6_000_000, seed=42)
+    d = workloads.filter_groupby_pipeline(t).desc()
+    want = oracle.sorted_rows(oracle.run_pipeline(d, threads=8)[0])
+    p, devs = on_gpu([t], lambda a: workloads.filter_groupby_pipeline(t, a))
+    q = runtime.Query(p.desc())
+    for rep in range(3):     # repeated runs reuse the tables and the merge counter
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+    # size-independent properties: counts add up to the selected rows
+    m = t["date"] <= workloads.FILTER_GROUPBY_CUTOFF
+    rows = [r.split("|") for r in want]
+    assert sum(int(r[5]) for r in rows) == int(m.sum())
+    assert len(rows) == 6
+
+
+def test_join_aggregate_full_size():
+    print("join_aggregate 1.5M/6M seed 42", flush=True)
+    b, pr = datagen.gen_join_aggregate(1_500_000, 6_000_000, seed=42)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    q = runtime.Query(p.desc())
+    for rep in range(3):
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+    m = pr["date"] <= workloads.JOIN_DATE_CUTOFF
+    assert sum(int(r.split("|")[1]) for r in want) == int(pr["price"][m].sum())
+
+
+def test_joinn_duplicate_keys_last_join():
+    print("JoinN duplicates", flush=True)
+    rng = np.random.default_rng(5)
+    build = {"key": rng.integers(0, 2000, 6000).astype(np.int32),
+             "pay": rng.integers(-7, 9, 6000).astype(np.int32)}
+    probe = {"fk": rng.integers(-10, 2100, 50000).astype(np.int32),
+             "v": rng.integers(-10**9, 10**9, 50000).astype(np.int64)}
+
+    def make(bb, pp):
+        p = Pipeline(table_from_arrays("s", probe, pp))
+        p.join(table_from_arrays("r", build, bb), "key", "fk", payload=["pay"],
+               variant=abi.VH_JOINN, names=["pay"])
+        p.groupby(["pay"], [("sum", col("v") * 3), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None, None).desc()))
+    p, devs = on_gpu([build, probe], make)
+    for mode in MODES:
+        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+        assert got == want, mode
+
+
+def test_join01_first_match_unique_keys():
+    print("Join01 unique keys", flush=True)
+    b, pr = datagen.gen_join_aggregate(10_000, 80_000, seed=3)
+    want = oracle.sorted_rows(oracle.bruteforce(
+        workloads.join_aggregate_pipeline(b, pr, variant=abi.VH_JOIN01).desc()))
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y, variant=abi.VH_JOIN01))
+    assert oracle.sorted_rows(runtime.run_pipeline(p)) == want
+
+
+def test_wide_sums_signed_i64_into_i128():
+    print("wide signed sums", flush=True)
+    rng = np.random.default_rng(9)
+    n = 400_000
+    t = {"g": rng.integers(0, 5, n).astype(np.int32),
+         "a": rng.integers(-2**62, 2**62, n).astype(np.int64),
+         "b": rng.integers(-2**40, 2**40, n).astype(np.int64)}
+
+    def make(addr):
+        p = Pipeline(table_from_arrays("t", t, addr))
+        p.groupby(["g"], [("sum", col("a")), ("sum", col("a") - col("b")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
+    p, devs = on_gpu([t], make)
+    for mode in [abi.VH_GROUP_AUTO] + MODES:
+        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+        assert got == want, mode
+
+
+def test_many_groups_hash_and_global():
+    print("many groups", flush=True)
+    rng = np.random.default_rng(21)
+    n = 500_000
+    t = {"k1": rng.integers(0, 700, n).astype(np.int32),
+         "k2": rng.integers(-50, 50, n).astype(np.int32),
+         "v": rng.integers(0, 10**6, n).astype(np.int64)}
+
+    def make(addr):
+        p = Pipeline(table_from_arrays("t", t, addr))
+        p.select("v", ">=", 1000)
+        p.groupby(["k1", "k2"], [("sum", col("v")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
+    p, devs = on_gpu([t], make)
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH):
+        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+        assert got == want, mode
+
+
+def test_global_aggregate_and_empty_inputs():
+    print("global aggregate / empty", flush=True)
+    t = datagen.gen_filter_groupby(100_000, seed=2)
+
+    def make(addr, cutoff):
+        p = Pipeline(table_from_arrays("t", t, addr))
+        p.select("date", "<=", cutoff)
+        p.groupby([], [("sum", col("price") * col("qty")), ("count",)])
+        return p
+    for cutoff in (workloads.FILTER_GROUPBY_CUTOFF, 0):
+        want = oracle.sorted_rows(oracle.bruteforce(make(None, cutoff).desc()))
+        p, devs = on_gpu([t], lambda a: make(a, cutoff))
+        assert oracle.sorted_rows(runtime.run_pipeline(p)) == want
+    # empty probe and empty build
+    e = {"k": np.zeros(0, dtype=np.int32), "v": np.zeros(0, dtype=np.int64)}
+    p, devs = on_gpu([e], lambda a: Pipeline(table_from_arrays("e", e, a)).groupby(["k"], [("sum", col("v"))]))
+    assert runtime.run_pipeline(p) == ""
+    b, pr = datagen.gen_join_aggregate(1000, 5000, seed=1)
+    b0 = {k: v[:0] for k, v in b.items()}
+    p, devs = on_gpu([b0, pr], lambda x, y: workloads.join_aggregate_pipeline(b0, pr, x, y))
+    assert runtime.run_pipeline(p) == ""
+
+
+def test_build_side_filter_two_joins():
+    print("two joins with build filters", flush=True)
+    rng = np.random.default_rng(17)
+    cust = {"ck": np.arange(1, 30001, dtype=np.int32), "region": rng.integers(0, 5, 30000).astype(np.int32),
+            "nation": rng.integers(0, 25, 30000).astype(np.int32)}
+    supp = {"sk": np.arange(1, 2001, dtype=np.int32), "region": rng.integers(0, 5, 2000).astype(np.int32),
+            "nation": rng.integers(0, 25, 2000).astype(np.int32)}
+    fact = {"c": rng.integers(1, 30001, 300_000).astype(np.int32),
+            "s": rng.integers(1, 2001, 300_000).astype(np.int32),
+            "rev": rng.integers(100000, 10**7, 300_000).astype(np.int64),
+            "cost": rng.integers(0, 10**6, 300_000).astype(np.int64)}
+
+    def make(fa, ca, sa):
+        p = Pipeline(table_from_arrays("f", fact, fa))
+        p.join(table_from_arrays("c", cust, ca), "ck", "c", payload=["nation"], build_preds=[("region", "==", 2)],
+               names=["c_nation"])
+        p.join(table_from_arrays("s", supp, sa), "sk", "s", payload=["nation"], build_preds=[("region", "==", 2)],
+               names=["s_nation"])
+        p.groupby(["c_nation", "s_nation"], [("sum", col("rev") - col("cost")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None, None, None).desc()))
+    p, devs = on_gpu([fact, cust, supp], make)
+    for mode in MODES:
+        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+        assert got == want, mode
+
+
+def test_key_outside_declared_range_is_an_error():
+    print("bad metadata", flush=True)
+    t = {"k": np.array([1, 2, 3, 9], dtype=np.int32), "v": np.array([1, 2, 3, 4], dtype=np.int64)}
+    p, devs = on_gpu([t], lambda a: Pipeline(table_from_arrays("t", t, a)).groupby(["k"], [("sum", col("v"))]))
+    p.probe.columns[0].vmax = 3   # lie about max
+    with pytest.raises(runtime.VoilaHipError):
+        runtime.run_pipeline(p)

@@ -1,0 +1,694 @@
+// Host side of the gfx950 backend: typing, lowering of a vh_pipeline_desc
+// into the fused-kernel plan, device buffers, the query lifecycle and the
+// QueryResult text.  Everything here is plain host C++ over the HIP runtime.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/voila_hip.h"
+#include "vh_engine.hpp"
+#include "vh_kernels.hpp"
+
+namespace vh {
+
+// ------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_last_error = buf;
+	return code;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+#define HIP_TRY(x)                                                                    \
+	do {                                                                              \
+		hipError_t _e = (x);                                                          \
+		if (_e != hipSuccess)                                                         \
+			return fail(VH_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_e));     \
+	} while (0)
+
+// ------------------------------------------------------------------------
+// typing: restates typing_pass.cpp for the values a pipeline produces
+// ------------------------------------------------------------------------
+
+static void type_limits(int t, double& lo, double& hi)
+{
+	switch (t) {
+	case VH_U8: lo = 0; hi = 255.0; break;
+	case VH_I8: lo = -128.0; hi = 127.0; break;
+	case VH_U16: lo = 0; hi = 65535.0; break;
+	case VH_I16: lo = -32768.0; hi = 32767.0; break;
+	case VH_U32: lo = 0; hi = 4294967295.0; break;
+	case VH_I32: lo = -2147483648.0; hi = 2147483647.0; break;
+	case VH_U64: lo = 0; hi = (double)UINT64_MAX; break;
+	case VH_I64: lo = (double)INT64_MIN; hi = (double)INT64_MAX; break;
+	default: lo = -1.7e38; hi = 1.7e38; break;
+	}
+}
+
+// type_from_minmax (typing_pass.cpp:9-22)
+int type_from_minmax(double mn, double mx)
+{
+	static const int order[8] = {VH_U8, VH_I8, VH_U16, VH_I16, VH_U32, VH_I32, VH_U64, VH_I64};
+	for (int t : order) {
+		double lo, hi;
+		type_limits(t, lo, hi);
+		if (mn >= lo && mx <= hi)
+			return t;
+	}
+	return VH_I128;
+}
+
+// scan type: signed -> unsigned when min >= 0 (typing_pass.cpp:309-326)
+static int scan_type(int t, int64_t mn)
+{
+	if (mn < 0)
+		return t;
+	switch (t) {
+	case VH_I8: return VH_U8;
+	case VH_I16: return VH_U16;
+	case VH_I32: return VH_U32;
+	default: return t;
+	}
+}
+
+static bool type_signed(int t)
+{
+	return t == VH_I8 || t == VH_I16 || t == VH_I32 || t == VH_I64 || t == VH_I128;
+}
+
+static const double kMaxCard = 1099511627776.0;   // QueryConfig::max_card = 2^40
+
+int type_pipeline(const vh_pipeline_desc* d, Typing& ty)
+{
+	ty.vcols.clear();
+	for (int c = 0; c < d->probe.ncols; c++) {
+		const vh_column& col = d->probe.cols[c];
+		ty.vcols.push_back({scan_type(col.type, col.min), (double)col.min, (double)col.max,
+			(double)col.min, (double)col.max});
+	}
+	for (int j = 0; j < d->njoins; j++) {
+		const vh_join_desc& jd = d->joins[j];
+		if (jd.probe_key < 0 || jd.probe_key >= (int)ty.vcols.size())
+			return fail(VH_ERR_INVALID, "join %d: probe_key %d out of range", j, jd.probe_key);
+		if (jd.npayload < 0 || jd.npayload > VH_MAX_PAYLOAD)
+			return fail(VH_ERR_INVALID, "join %d: npayload %d", j, jd.npayload);
+		for (int p = 0; p < jd.npayload; p++) {
+			if (jd.payload[p] < 0 || jd.payload[p] >= jd.build.ncols)
+				return fail(VH_ERR_INVALID, "join %d: payload column %d", j, jd.payload[p]);
+			const vh_column& col = jd.build.cols[jd.payload[p]];
+			ty.vcols.push_back({scan_type(col.type, col.min), (double)col.min, (double)col.max,
+				(double)col.min, (double)col.max});
+		}
+	}
+	ty.nodes.assign(d->nexpr, TypeProps{});
+	for (int n = 0; n < d->nexpr; n++) {
+		const vh_expr& e = d->expr[n];
+		TypeProps t{};
+		switch (e.op) {
+		case VH_E_COL:
+			if (e.a < 0 || e.a >= (int)ty.vcols.size())
+				return fail(VH_ERR_INVALID, "expr %d: column %d", n, e.a);
+			t = ty.vcols[e.a];
+			break;
+		case VH_E_CONST:
+			t.dmin = t.dmax = t.emin = t.emax = (double)e.value;
+			t.type = type_from_minmax(t.dmin, t.dmax);
+			break;
+		case VH_E_ADD:
+		case VH_E_SUB:
+		case VH_E_MUL: {
+			if (e.a < 0 || e.a >= n || e.b < 0 || e.b >= n)
+				return fail(VH_ERR_INVALID, "expr %d: children must precede the node", n);
+			const TypeProps& A = ty.nodes[e.a];
+			const TypeProps& B = ty.nodes[e.b];
+			double x, y;
+			// ARITH2_INFIX range rule (typing_pass.cpp:164-176)
+			if (e.op == VH_E_ADD) { x = A.dmin + B.dmin; y = A.dmax + B.dmax; }
+			else if (e.op == VH_E_SUB) { x = A.dmin - B.dmax; y = A.dmax - B.dmin; }
+			else { x = A.dmin * B.dmin; y = A.dmax * B.dmax; }
+			t.dmin = std::min(x, y);
+			t.dmax = std::max(x, y);
+			t.type = type_from_minmax(t.dmin, t.dmax);
+			// exact interval of the true value
+			if (e.op == VH_E_ADD) { t.emin = A.emin + B.emin; t.emax = A.emax + B.emax; }
+			else if (e.op == VH_E_SUB) { t.emin = A.emin - B.emax; t.emax = A.emax - B.emin; }
+			else {
+				double c[4] = {A.emin * B.emin, A.emin * B.emax, A.emax * B.emin, A.emax * B.emax};
+				t.emin = *std::min_element(c, c + 4);
+				t.emax = *std::max_element(c, c + 4);
+			}
+			break;
+		}
+		default:
+			return fail(VH_ERR_INVALID, "expr %d: op %d", n, e.op);
+		}
+		ty.nodes[n] = t;
+	}
+	ty.aggrs.assign(d->naggrs, TypeProps{});
+	ty.final_aggrs.assign(d->naggrs, TypeProps{});
+	for (int a = 0; a < d->naggrs; a++) {
+		const vh_aggr_desc& ad = d->aggrs[a];
+		TypeProps t{};
+		if (ad.fn == VH_A_SUM) {
+			if (ad.expr < 0 || ad.expr >= d->nexpr)
+				return fail(VH_ERR_INVALID, "aggregate %d: expr %d", a, ad.expr);
+			t.dmin = kMaxCard * ty.nodes[ad.expr].dmin;
+			t.dmax = kMaxCard * ty.nodes[ad.expr].dmax;
+		} else if (ad.fn == VH_A_COUNT) {
+			t.dmin = 0;
+			t.dmax = kMaxCard;
+		} else {
+			return fail(VH_ERR_INVALID, "aggregate %d: fn %d", a, ad.fn);
+		}
+		t.type = type_from_minmax(t.dmin, t.dmax);
+		ty.aggrs[a] = t;
+		// re-aggregation (count -> sum) of the flushed partitions
+		TypeProps f{};
+		f.dmin = kMaxCard * t.dmin;
+		f.dmax = kMaxCard * t.dmax;
+		f.type = type_from_minmax(f.dmin, f.dmax);
+		ty.final_aggrs[a] = f;
+	}
+	for (int k = 0; k < d->nkeys; k++)
+		if (d->keys[k] < 0 || d->keys[k] >= (int)ty.vcols.size())
+			return fail(VH_ERR_INVALID, "key %d: column %d", k, d->keys[k]);
+	return VH_OK;
+}
+
+// ------------------------------------------------------------------------
+// lowering
+// ------------------------------------------------------------------------
+
+// calc_num_buckets (runtime_struct.cpp:790-814)
+static uint64_t calc_num_buckets(uint64_t count)
+{
+	uint64_t n = std::max<uint64_t>({count, 2048, 2 * 1024});
+	int exp = 64 - __builtin_clzll(n);
+	if ((double)(1ull << exp) < (double)n / 0.7)
+		exp++;
+	return 1ull << exp;
+}
+
+static int role_of_vcol(const vh_pipeline_desc* d, int v, Role& r)
+{
+	memset(&r, 0, sizeof(r));
+	if (v < d->probe.ncols) {
+		const vh_column& c = d->probe.cols[v];
+		r.kind = ROLE_COL;
+		r.ptr = c.data;
+		r.type = c.type;
+		return VH_OK;
+	}
+	int base = d->probe.ncols;
+	for (int j = 0; j < d->njoins; j++) {
+		if (v < base + d->joins[j].npayload) {
+			r.kind = ROLE_PAY;
+			r.slot = j;
+			r.type = d->joins[j].build.cols[d->joins[j].payload[v - base]].type;
+			return VH_OK;
+		}
+		base += d->joins[j].npayload;
+	}
+	return fail(VH_ERR_INVALID, "virtual column %d", v);
+}
+
+// Flatten an ADD/SUB tree over <= 2 columns and constants into sa*a + sb*b + c.
+static bool linear_term(const vh_pipeline_desc* d, int n, int sign, Term& T, int& ncols)
+{
+	const vh_expr& e = d->expr[n];
+	switch (e.op) {
+	case VH_E_CONST:
+		T.c += sign * e.value;
+		return true;
+	case VH_E_COL: {
+		Role r;
+		if (role_of_vcol(d, e.a, r) != VH_OK)
+			return false;
+		if (ncols == 0) { T.a = r; T.sa = sign; }
+		else if (ncols == 1) { T.b = r; T.sb = sign; }
+		else return false;
+		ncols++;
+		return true;
+	}
+	case VH_E_ADD:
+		return linear_term(d, e.a, sign, T, ncols) && linear_term(d, e.b, sign, T, ncols);
+	case VH_E_SUB:
+		return linear_term(d, e.a, sign, T, ncols) && linear_term(d, e.b, -sign, T, ncols);
+	default:
+		return false;
+	}
+}
+
+static bool product_terms(const vh_pipeline_desc* d, int n, SumArg& s)
+{
+	const vh_expr& e = d->expr[n];
+	if (e.op == VH_E_MUL)
+		return product_terms(d, e.a, s) && product_terms(d, e.b, s);
+	if (s.nterms >= 3)
+		return false;
+	Term& T = s.t[s.nterms];
+	memset(&T, 0, sizeof(T));
+	int ncols = 0;
+	if (!linear_term(d, n, 1, T, ncols))
+		return false;
+	s.nterms++;
+	return true;
+}
+
+static bool fits_type(int t, double lo, double hi)
+{
+	double tl, th;
+	type_limits(t, tl, th);
+	return lo >= tl && hi <= th && t != VH_I128;
+}
+
+int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
+{
+	memset(&L.plan, 0, sizeof(L.plan));
+	int rc = type_pipeline(d, L.ty);
+	if (rc)
+		return rc;
+	Plan& P = L.plan;
+	P.rows = d->probe.rows;
+	if (P.rows < 0)
+		return fail(VH_ERR_INVALID, "negative row count");
+	if (d->probe.ncols < 0 || d->probe.ncols > VH_MAX_COLS || d->probe.npreds < 0 ||
+		d->probe.npreds > VH_MAX_PREDS || d->njoins < 0 || d->njoins > VH_MAX_JOINS ||
+		d->nkeys < 0 || d->nkeys > VH_MAX_KEYS || d->naggrs < 0 || d->naggrs > VH_MAX_AGGRS ||
+		d->nexpr < 0 || d->nexpr > VH_MAX_EXPR)
+		return fail(VH_ERR_INVALID, "descriptor counts out of range");
+	for (int c = 0; c < d->probe.ncols; c++)
+		if (d->probe.rows > 0 && !d->probe.cols[c].data)
+			return fail(VH_ERR_INVALID, "probe column %d has no data", c);
+
+	// selection
+	P.npreds = d->probe.npreds;
+	for (int p = 0; p < P.npreds; p++) {
+		const vh_pred& pr = d->probe.preds[p];
+		if (pr.col < 0 || pr.col >= d->probe.ncols || pr.cmp < VH_LT || pr.cmp > VH_NE)
+			return fail(VH_ERR_INVALID, "predicate %d", p);
+		role_of_vcol(d, pr.col, P.preds[p].r);
+		P.preds[p].cmp = pr.cmp;
+		P.preds[p].value = pr.value;
+	}
+
+	// joins
+	P.njoins = d->njoins;
+	L.builds.assign(d->njoins, BuildPlan{});
+	L.capacity.assign(d->njoins, 0);
+	for (int j = 0; j < d->njoins; j++) {
+		const vh_join_desc& jd = d->joins[j];
+		BuildPlan& B = L.builds[j];
+		memset(&B, 0, sizeof(B));
+		if (jd.build_key < 0 || jd.build_key >= jd.build.ncols)
+			return fail(VH_ERR_INVALID, "join %d: build_key %d", j, jd.build_key);
+		if (jd.npayload > 1)
+			return fail(VH_ERR_UNSUPPORTED, "join %d: more than one payload column", j);
+		if (jd.variant != VH_JOIN01 && jd.variant != VH_JOINN)
+			return fail(VH_ERR_INVALID, "join %d: variant %d", j, jd.variant);
+		const vh_column& kc = jd.build.cols[jd.build_key];
+		if ((double)kc.max - (double)kc.min >= 4294967295.0)
+			return fail(VH_ERR_UNSUPPORTED, "join %d: key range exceeds 32 bits", j);
+		B.rows = jd.build.rows;
+		B.npreds = jd.build.npreds;
+		if (B.npreds < 0 || B.npreds > VH_MAX_PREDS)
+			return fail(VH_ERR_INVALID, "join %d: npreds", j);
+		for (int p = 0; p < B.npreds; p++) {
+			const vh_pred& pr = jd.build.preds[p];
+			if (pr.col < 0 || pr.col >= jd.build.ncols || pr.cmp < VH_LT || pr.cmp > VH_NE)
+				return fail(VH_ERR_INVALID, "join %d: predicate %d", j, p);
+			Role& r = B.preds[p].r;
+			memset(&r, 0, sizeof(r));
+			r.kind = ROLE_COL;
+			r.ptr = jd.build.cols[pr.col].data;
+			r.type = jd.build.cols[pr.col].type;
+			B.preds[p].cmp = pr.cmp;
+			B.preds[p].value = pr.value;
+		}
+		B.key = kc.data;
+		B.key_type = kc.type;
+		B.kmin = kc.min;
+		B.kmax = kc.max;
+		B.npay = jd.npayload;
+		if (jd.npayload) {
+			const vh_column& pc = jd.build.cols[jd.payload[0]];
+			if ((double)pc.max - (double)pc.min >= 4294967295.0)
+				return fail(VH_ERR_UNSUPPORTED, "join %d: payload range exceeds 32 bits", j);
+			B.pay = pc.data;
+			B.pay_type = pc.type;
+			B.pmin = pc.min;
+		}
+		uint64_t cap = calc_num_buckets((uint64_t)std::max<int64_t>(jd.build.rows, 0));
+		if (cfg && cfg->table_shift > 0)
+			cap <<= cfg->table_shift;
+		if (cfg && cfg->table_shift < 0)
+			cap = std::max<uint64_t>(cap >> (-cfg->table_shift), 2048);
+		while ((double)cap < (double)jd.build.rows / 0.7)   // never above 70% load
+			cap <<= 1;
+		L.capacity[j] = cap;
+		B.mask = cap - 1;
+
+		JoinT& J = P.joins[j];
+		J.mask = cap - 1;
+		if ((rc = role_of_vcol(d, jd.probe_key, J.key)))
+			return rc;
+		J.kmin = kc.min;
+		J.kmax = kc.max;
+		J.pmin = B.pmin;
+		J.npay = jd.npayload;
+		J.variant = jd.variant;
+		J.last = j == d->njoins - 1;
+	}
+
+	// group keys
+	P.nkeys = d->nkeys;
+	double domain = 1;
+	int bits = 0;
+	for (int k = 0; k < d->nkeys; k++) {
+		const TypeProps& t = L.ty.vcols[d->keys[k]];
+		if ((rc = role_of_vcol(d, d->keys[k], P.keys[k].r)))
+			return rc;
+		P.keys[k].min = (int64_t)t.dmin;
+		double range = t.dmax - t.dmin + 1;
+		domain *= range;
+		int b = 0;
+		while (b < 63 && (double)(1ull << b) < range)
+			b++;
+		P.keys[k].shift = bits;
+		P.keys[k].range = (u64)(t.dmax - t.dmin);
+		bits += b;
+		L.key_range[k] = (uint64_t)range;
+		L.key_bits[k] = b;
+	}
+	{
+		uint64_t mult = 1;
+		for (int k = d->nkeys - 1; k >= 0; k--) {
+			P.keys[k].mult = mult;
+			if (domain <= 1e18)
+				mult *= L.key_range[k];
+		}
+	}
+
+	// sums
+	P.nsums = 0;
+	L.sum_of_aggr.assign(d->naggrs, -1);
+	for (int a = 0; a < d->naggrs; a++) {
+		if (d->aggrs[a].fn != VH_A_SUM)
+			continue;
+		SumArg& s = P.sums[P.nsums];
+		memset(&s, 0, sizeof(s));
+		const int root = d->aggrs[a].expr;
+		// every node's true value must fit its typed width, as 64-bit
+		// evaluation is then exact (no wrap in the reference either)
+		for (int n = 0; n <= root; n++) {
+			const TypeProps& t = L.ty.nodes[n];
+			if (!fits_type(t.type, t.emin, t.emax))
+				return fail(VH_ERR_UNSUPPORTED, "aggregate %d: node %d may wrap its type", a, n);
+		}
+		if (!product_terms(d, root, s))
+			return fail(VH_ERR_UNSUPPORTED, "aggregate %d: not a product of linear terms", a);
+		const int at = L.ty.nodes[root].type;
+		s.arg_signed = type_signed(at);
+		s.res128 = L.ty.aggrs[a].type == VH_I128;
+		s.wide = s.res128 && (at == VH_U64 || at == VH_I64);
+		L.sum_of_aggr[a] = P.nsums;
+		P.nsums++;
+	}
+
+	// group-by mode
+	int mode = cfg ? cfg->group_mode : -1;
+	const size_t lds_budget = 64 * 1024;
+	const double lds_need = domain * P.nsums * 12 + domain * 4 + 16;
+	if (mode < 0) {
+		if (domain <= kRegSlots && P.nsums <= kRegSums)
+			mode = GB_REG;
+		else if (lds_need <= lds_budget)
+			mode = GB_LDS;
+		else if (domain <= (double)(1 << 24))
+			mode = GB_GLOBAL;
+		else
+			mode = GB_HASH;
+	}
+	if (mode == GB_REG && !(domain <= kRegSlots && P.nsums <= kRegSums))
+		return fail(VH_ERR_UNSUPPORTED, "register group-by needs <= %d groups and <= %d sums",
+			kRegSlots, kRegSums);
+	if (mode == GB_LDS && lds_need > lds_budget)
+		return fail(VH_ERR_UNSUPPORTED, "LDS group-by needs %.0f bytes", lds_need);
+	if (mode == GB_GLOBAL && domain > (double)(1 << 26))
+		return fail(VH_ERR_UNSUPPORTED, "dense global group-by domain %.0f too large", domain);
+	if (mode == GB_HASH && bits > 63)
+		return fail(VH_ERR_UNSUPPORTED, "packed group key needs %d bits", bits);
+	if (mode < GB_REG || mode > GB_HASH)
+		return fail(VH_ERR_INVALID, "group mode %d", mode);
+	P.mode = mode;
+	if (mode == GB_HASH) {
+		// groups <= min(domain, tuples); tuples <= probe rows unless a JoinN fans out
+		double bound = std::min(domain, (double)std::max<int64_t>(P.rows, 1));
+		uint64_t cap = 1024;
+		while ((double)cap < 2 * bound && cap < (1ull << 30))
+			cap <<= 1;
+		P.hmask = cap - 1;
+		P.domain = 0;
+		L.groups = cap;
+	} else {
+		P.domain = (uint64_t)domain;
+		L.groups = P.domain;
+	}
+	L.wide = false;
+	for (int s = 0; s < P.nsums; s++) {
+		if (mode == GB_REG || mode == GB_LDS)
+			L.wide |= P.sums[s].wide != 0;
+		else
+			L.wide |= P.sums[s].res128 != 0;
+	}
+
+	// launch shape: enough blocks for 256 CUs, capped so partials stay small
+	int grid = cfg && cfg->grid > 0 ? cfg->grid : 0;
+	if (!grid) {
+		int64_t want = (P.rows + 1023) / 1024;
+		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 2048));
+	}
+	L.grid = grid;
+	L.build_grid.assign(d->njoins, 0);
+	for (int j = 0; j < d->njoins; j++)
+		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 255) / 256, 4096));
+	return VH_OK;
+}
+
+// ------------------------------------------------------------------------
+// query: device state, run, result
+// ------------------------------------------------------------------------
+
+Query::~Query()
+{
+	for (void* p : allocs)
+		if (p)
+			(void)hipFree(p);
+}
+
+int Query::alloc(void** p, size_t bytes)
+{
+	*p = nullptr;
+	if (!bytes)
+		bytes = 16;
+	hipError_t e = hipMalloc(p, bytes);
+	if (e != hipSuccess)
+		return fail(VH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+	allocs.push_back(*p);
+	return VH_OK;
+}
+
+int query_create(Query* q)
+{
+	Lowered& L = q->L;
+	Plan& P = L.plan;
+	int rc;
+	// flags: [0] error, [1 + j] dups of join j, [8] completion counter
+	if ((rc = q->alloc((void**)&q->flags, 16 * sizeof(u32))))
+		return rc;
+	HIP_TRY(hipMemset(q->flags, 0, 16 * sizeof(u32)));
+	P.error = q->flags;
+	P.done = q->flags + 8;
+	for (int j = 0; j < P.njoins; j++) {
+		if ((rc = q->alloc((void**)&q->slots[j], L.capacity[j] * sizeof(u64))))
+			return rc;
+		P.joins[j].slots = q->slots[j];
+		P.joins[j].dups = q->flags + 1 + j;
+		L.builds[j].slots = q->slots[j];
+		L.builds[j].dups = q->flags + 1 + j;
+		L.builds[j].error = q->flags;
+	}
+	const u64 groups = L.groups;
+	const int nsums = std::max(P.nsums, 1);
+	if ((rc = q->alloc((void**)&P.out_lo, groups * nsums * sizeof(u64))))
+		return rc;
+	if ((rc = q->alloc((void**)&P.out_hi, groups * nsums * sizeof(i64))))
+		return rc;
+	if ((rc = q->alloc((void**)&P.out_cnt, groups * sizeof(u64))))
+		return rc;
+	if (P.mode == GB_REG || P.mode == GB_LDS) {
+		const u64 W = 2 * P.domain * P.nsums + P.domain;
+		if ((rc = q->alloc((void**)&P.partials, (u64)L.grid * W * sizeof(u64))))
+			return rc;
+	}
+	if (P.mode == GB_HASH) {
+		if ((rc = q->alloc((void**)&P.hkeys, groups * sizeof(u64))))
+			return rc;
+	}
+	// device copies of the plans
+	if ((rc = q->alloc((void**)&q->dplan, sizeof(Plan))))
+		return rc;
+	HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
+	for (int j = 0; j < P.njoins; j++) {
+		if ((rc = q->alloc((void**)&q->dbuild[j], sizeof(BuildPlan))))
+			return rc;
+		HIP_TRY(hipMemcpy(q->dbuild[j], &L.builds[j], sizeof(BuildPlan), hipMemcpyHostToDevice));
+	}
+	return VH_OK;
+}
+
+int query_run(Query* q, hipStream_t s, int phase)
+{
+	Lowered& L = q->L;
+	Plan& P = L.plan;
+	if (phase < -1 || phase > 2)
+		return fail(VH_ERR_INVALID, "phase %d", phase);
+	if (phase == -1 || phase == 0) {
+	// clear: error/dup flags, join slots, atomically accumulated outputs
+	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
+	for (int j = 0; j < P.njoins; j++)
+		HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
+	if (P.mode == GB_GLOBAL || P.mode == GB_HASH) {
+		const u64 groups = L.groups;
+		const int nsums = std::max(P.nsums, 1);
+		HIP_TRY(launch_fill(P.out_lo, groups * nsums, 0, s));
+		HIP_TRY(launch_fill((u64*)P.out_hi, groups * nsums, 0, s));
+		HIP_TRY(launch_fill(P.out_cnt, groups, 0, s));
+		if (P.mode == GB_HASH)
+			HIP_TRY(launch_fill(P.hkeys, groups, ~0ull, s));
+	}
+	}
+	if (phase == -1 || phase == 1) {
+		for (int j = 0; j < P.njoins; j++)
+			if (L.builds[j].rows > 0)
+				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
+	}
+	if (phase == -1 || phase == 2) {
+		HIP_TRY(launch_pipeline(P, q->dplan, L.grid, L.wide, s));
+		q->ran = true;
+	}
+	return VH_OK;
+}
+
+// voila_cast<T>::to_cstr (runtime.hpp:348-387): magnitude digits, then '-'.
+static void append_i128(std::string& out, __int128 v)
+{
+	char buf[41];
+	unsigned __int128 m = v < 0 ? (unsigned __int128)(-v) : (unsigned __int128)v;
+	char* d = buf + sizeof(buf);
+	do {
+		*--d = (char)('0' + (int)(m % 10));
+		m /= 10;
+	} while (m);
+	if (v < 0)
+		*--d = '-';
+	out.append(d, (size_t)(buf + sizeof(buf) - d));
+}
+
+static __int128 typed_value(int t, u64 lo, i64 hi)
+{
+	switch (t) {
+	case VH_U64: return (__int128)lo;
+	case VH_I64: return (__int128)(i64)lo;
+	case VH_I128: return (__int128)(((unsigned __int128)(u64)hi << 64) | lo);
+	default: return type_signed(t) ? (__int128)(i64)lo : (__int128)lo;
+	}
+}
+
+int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
+{
+	if (!q->ran)
+		return fail(VH_ERR_STATE, "query has not run");
+	Lowered& L = q->L;
+	Plan& P = L.plan;
+	const vh_pipeline_desc& d = q->desc;
+	HIP_TRY(hipStreamSynchronize(s));
+	u32 flags[16];
+	HIP_TRY(hipMemcpy(flags, q->flags, sizeof(flags), hipMemcpyDeviceToHost));
+	if (flags[0] & 4u)
+		return fail(VH_ERR_INVALID, "a build key lies outside its column's [min, max]");
+	if (flags[0] & 1u)
+		return fail(VH_ERR_CAPACITY, "group hash table full");
+	if (flags[0] & 8u)
+		return fail(VH_ERR_INVALID, "a group key lies outside its column's [min, max]");
+	for (int j = 0; j < P.njoins - 1; j++)
+		if (d.joins[j].variant == VH_JOINN && flags[1 + j])
+			return fail(VH_ERR_UNSUPPORTED,
+				"join %d: JoinN with duplicate build keys before the last join", j);
+	const u64 groups = L.groups;
+	const int nsums = std::max(P.nsums, 1);
+	std::vector<u64> lo(groups * nsums), cnt(groups), hk;
+	std::vector<i64> hi(groups * nsums);
+	HIP_TRY(hipMemcpy(lo.data(), P.out_lo, lo.size() * 8, hipMemcpyDeviceToHost));
+	HIP_TRY(hipMemcpy(hi.data(), P.out_hi, hi.size() * 8, hipMemcpyDeviceToHost));
+	HIP_TRY(hipMemcpy(cnt.data(), P.out_cnt, cnt.size() * 8, hipMemcpyDeviceToHost));
+	if (P.mode == GB_HASH) {
+		hk.resize(groups);
+		HIP_TRY(hipMemcpy(hk.data(), P.hkeys, hk.size() * 8, hipMemcpyDeviceToHost));
+	}
+	text.clear();
+	nrows = 0;
+	for (u64 g = 0; g < groups; g++) {
+		if (!cnt[g])
+			continue;   // a group exists iff its count > 0 (relalg_translator.cpp:698-701)
+		bool first = true;
+		for (int k = 0; k < d.nkeys; k++) {
+			int64_t v;
+			if (P.mode == GB_HASH) {
+				const u64 m = L.key_bits[k] >= 64 ? ~0ull : ((1ull << L.key_bits[k]) - 1);
+				v = P.keys[k].min + (int64_t)((hk[g] >> P.keys[k].shift) & m);
+			} else {
+				v = P.keys[k].min + (int64_t)((g / P.keys[k].mult) % L.key_range[k]);
+			}
+			if (!first)
+				text.push_back('|');
+			first = false;
+			append_i128(text, (__int128)v);
+		}
+		for (int a = 0; a < d.naggrs; a++) {
+			__int128 v;
+			if (d.aggrs[a].fn == VH_A_COUNT) {
+				v = (__int128)cnt[g];
+			} else {
+				const int s2 = L.sum_of_aggr[a];
+				v = typed_value(L.ty.aggrs[a].type, lo[s2 * groups + g], hi[s2 * groups + g]);
+			}
+			if (!first)
+				text.push_back('|');
+			first = false;
+			append_i128(text, v);
+		}
+		text.push_back('\n');
+		nrows++;
+	}
+	return VH_OK;
+}
+
+} // namespace vh

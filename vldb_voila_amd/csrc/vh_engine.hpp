@@ -1,0 +1,65 @@
+// Host-side engine state shared by the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/voila_hip.h"
+#include "vh_plan.hpp"
+
+namespace vh {
+
+struct TypeProps {     // ScalarTypeProps (typing_pass.cpp) + exact value interval
+	int type = VH_INVALID;
+	double dmin = 0, dmax = 0;   // range the typing rules derive
+	double emin = 0, emax = 0;   // exact interval of the value
+};
+
+struct Typing {
+	std::vector<TypeProps> vcols;
+	std::vector<TypeProps> nodes;
+	std::vector<TypeProps> aggrs;        // first-level aggregate columns
+	std::vector<TypeProps> final_aggrs;  // after re-aggregation (count -> sum)
+};
+
+struct Lowered {
+	Typing ty;
+	Plan plan;
+	std::vector<BuildPlan> builds;
+	std::vector<uint64_t> capacity;      // join table slots
+	std::vector<int> build_grid;
+	std::vector<int> sum_of_aggr;        // aggregate -> sum index (-1: count)
+	uint64_t key_range[VH_MAX_KEYS] = {};
+	int key_bits[VH_MAX_KEYS] = {};
+	uint64_t groups = 0;                 // output slots (domain or hash capacity)
+	bool wide = false;
+	int grid = 0;
+};
+
+struct Query {
+	vh_pipeline_desc desc;
+	vh_config cfg;
+	Lowered L;
+	u32* flags = nullptr;
+	u64* slots[VH_MAX_JOINS] = {};
+	Plan* dplan = nullptr;
+	BuildPlan* dbuild[VH_MAX_JOINS] = {};
+	std::vector<void*> allocs;
+	bool ran = false;
+	std::string result;
+	~Query();
+	int alloc(void** p, size_t bytes);
+};
+
+int fail(int code, const char* fmt, ...);
+const char* last_error();
+int type_from_minmax(double mn, double mx);
+int type_pipeline(const vh_pipeline_desc* d, Typing& ty);
+int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
+int query_create(Query* q);
+int query_run(Query* q, hipStream_t s, int phase);
+int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows);
+
+} // namespace vh

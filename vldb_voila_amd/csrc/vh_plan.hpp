@@ -1,0 +1,129 @@
+// Lowered form of a vh_pipeline_desc, passed by value to the fused gfx950
+// kernels.  Every field is uniform across the grid, so the kernels branch on
+// it with scalar instructions only.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/voila_hip.h"
+
+namespace vh {
+
+typedef unsigned long long u64;
+typedef long long i64;
+typedef unsigned int u32;
+
+// Where a value of the probe stream comes from.
+enum RoleKind { ROLE_NONE = 0, ROLE_COL = 1, ROLE_PAY = 2, ROLE_CONST = 3 };
+
+struct Role {
+	const void* ptr;     // ROLE_COL: probe column base address
+	i64 c;               // ROLE_CONST: value
+	int kind;
+	int type;            // ROLE_COL: enum vh_type of the column
+	int slot;            // ROLE_PAY: join index (one payload per join)
+	int pad;
+};
+
+// A linear term sa*a + sb*b + c with sa, sb in {-1, 0, 1}.  A sum argument is
+// the product of up to three terms, e.g. price * (100 - disc).
+struct Term {
+	Role a;
+	Role b;
+	i64 c;
+	int sa;
+	int sb;
+};
+
+struct SumArg {
+	Term t[3];
+	int nterms;
+	int wide;            // result i128 and argument wider than 32 bits: lanes
+	                     // track a carry word; narrower arguments cannot
+	                     // overflow a 64-bit lane sum
+	int arg_signed;      // argument type signed (sign extends into i128)
+	int res128;          // result typed i128 (global modes accumulate exactly)
+};
+
+struct Pred {
+	Role r;
+	i64 value;
+	int cmp;
+	int pad;
+};
+
+// Open-addressing join table: 64-bit slots, low word = key - kmin, high word
+// = payload - pmin (both < 2^32 - 1), all-ones = empty.  Linear probing.
+struct JoinT {
+	u64* slots;
+	u64 mask;
+	Role key;            // probe-side key
+	i64 kmin;
+	i64 kmax;
+	i64 pmin;
+	int npay;
+	int variant;         // enum vh_join_variant
+	int last;            // last join of the pipeline: JoinN may match many
+	int pad;
+	const u32* dups;     // set by the build when a key occurs twice
+};
+
+struct KeyT {
+	Role r;
+	i64 min;
+	u64 mult;            // dense index multiplier (direct modes)
+	u64 range;           // max - min
+	int shift;           // bit offset of (v - min) in the packed key (hash mode)
+	int pad;
+};
+
+enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3 };
+
+static constexpr int kRegSlots = 8;    // dense groups held in registers
+static constexpr int kRegSums = 4;     // sums held in registers
+
+struct Plan {
+	i64 rows;
+	int npreds;
+	int njoins;
+	int nkeys;
+	int nsums;
+	int vec4;            // all probe columns 16-byte aligned
+	int mode;            // enum GroupMode
+	Pred preds[VH_MAX_PREDS];
+	JoinT joins[VH_MAX_JOINS];
+	KeyT keys[VH_MAX_KEYS];
+	SumArg sums[VH_MAX_AGGRS];
+	u64 domain;          // dense group domain (direct modes)
+	// outputs: lo[s * groups + g], hi[...], cnt[g]; groups = domain or hash capacity
+	u64* out_lo;
+	i64* out_hi;
+	u64* out_cnt;
+	// per-block partials and completion counter (GB_REG / GB_LDS)
+	u64* partials;
+	u32* done;
+	// hash mode: packed keys, EMPTY = ~0
+	u64* hkeys;
+	u64 hmask;
+	u32* error;          // bit 0: hash table full, bit 3: group key outside [min, max]
+};
+
+struct BuildPlan {
+	i64 rows;
+	int npreds;
+	int key_type;
+	Pred preds[VH_MAX_PREDS];
+	const void* key;
+	const void* pay;
+	int pay_type;
+	int npay;
+	i64 kmin;
+	i64 kmax;
+	i64 pmin;
+	u64* slots;
+	u64 mask;
+	u32* dups;
+	u32* error;          // bit 2: key outside [kmin, kmax]
+};
+
+} // namespace vh

@@ -1,0 +1,113 @@
+"""Seeded synthetic inputs for the benchmark workloads.
+
+PRNG (counter based, so every column is reproducible in any order and in C):
+
+    mix64(z)      = splitmix64 finaliser:
+                    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9
+                    z = (z ^ (z >> 27)) * 0x94D049BB133111EB
+                    z ^ (z >> 31)                              (all mod 2^64)
+    base(s, c)    = mix64(s * 0x9E3779B97F4A7C15 + c)          seed s, column id c
+    r(s, c, i)    = mix64(base(s, c) + (i + 1) * 0x9E3779B97F4A7C15)
+    uniform_int(lo, hi) [lo, hi) = lo + r % (hi - lo)
+    unit double   = (r >> 11) * 2^-53
+    permutation   = stable argsort of r over [0, n)
+    zipf(theta,n) = rank k in [1, n] by inverse CDF of k^-theta (theta 0 = uniform)
+
+The same arrays feed the GPU path and the CPU oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+M64 = (1 << 64) - 1
+GOLDEN = 0x9E3779B97F4A7C15
+SPAN_START = 2448623          # Julian day number of 1992-01-01 (types::Date)
+SPAN_DAYS = 2526
+
+
+def mix64_int(z: int) -> int:
+    z &= M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def rand_u64(seed: int, column: int, n: int, chunk: int = 1 << 24) -> np.ndarray:
+    """r(seed, column, i) for i in [0, n)."""
+    base = mix64_int((seed * GOLDEN + column) & M64)
+    out = np.empty(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        for s in range(0, n, chunk):
+            e = min(n, s + chunk)
+            i = np.arange(s + 1, e + 1, dtype=np.uint64)
+            out[s:e] = _mix64(np.uint64(base) + i * np.uint64(GOLDEN))
+    return out
+
+
+def uniform_int(seed, column, n, lo, hi, dtype):
+    r = rand_u64(seed, column, n)
+    return (np.int64(lo) + (r % np.uint64(hi - lo)).astype(np.int64)).astype(dtype)
+
+
+def unit_double(seed, column, n):
+    return (rand_u64(seed, column, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def permutation(seed, column, n):
+    return np.argsort(rand_u64(seed, column, n), kind="stable")
+
+
+def zipf_ranks(seed, column, n, items, theta):
+    """Ranks in [1, items]; theta == 0 is uniform."""
+    if theta == 0:
+        return uniform_int(seed, column, n, 1, items + 1, np.int64)
+    w = np.arange(1, items + 1, dtype=np.float64) ** (-float(theta))
+    cdf = np.cumsum(w)
+    cdf /= cdf[-1]
+    u = unit_double(seed, column, n)
+    return np.minimum(np.searchsorted(cdf, u, side="right"), items - 1).astype(np.int64) + 1
+
+
+def _col(arr, lo=None, hi=None):
+    """(array, min, max) with min/max taken from the data (Import.cpp:175-186)."""
+    if arr.size == 0:
+        return arr, 0, 0
+    return arr, int(arr.min()), int(arr.max())
+
+
+# --------------------------------------------------------------------------
+# Workload inputs
+# --------------------------------------------------------------------------
+
+def gen_filter_groupby(rows=6_000_000, seed=42):
+    """Filter + low-cardinality group-by (TPC-H Q1 shape)."""
+    t = {
+        "flag": uniform_int(seed, 1, rows, 0, 3, np.int32),
+        "status": uniform_int(seed, 2, rows, 0, 2, np.int32),
+        "qty": (uniform_int(seed, 3, rows, 1, 51, np.int64) * 100).astype(np.int64),
+        "price": uniform_int(seed, 4, rows, 90000, 10500000, np.int64),
+        "disc": uniform_int(seed, 5, rows, 0, 11, np.int64),
+        "date": uniform_int(seed, 6, rows, SPAN_START, SPAN_START + SPAN_DAYS, np.int32),
+    }
+    return t
+
+
+def gen_join_aggregate(build_rows=1_500_000, probe_rows=6_000_000, seed=42):
+    """Single hash join + 5-group aggregate."""
+    build = {
+        "key": (permutation(seed, 11, build_rows) + 1).astype(np.int32),
+        "nation": uniform_int(seed, 12, build_rows, 0, 25, np.int32),
+        "segment": uniform_int(seed, 13, build_rows, 0, 5, np.int32),
+    }
+    probe = {
+        "fk": uniform_int(seed, 14, probe_rows, 1, build_rows + 1, np.int32),
+        "date": uniform_int(seed, 15, probe_rows, SPAN_START, SPAN_START + SPAN_DAYS, np.int32),
+        "price": uniform_int(seed, 16, probe_rows, 90000, 10500000, np.int64),
+    }
+    return build, probe

@@ -1,0 +1,219 @@
+"""ctypes binding of libvoila_hip.so.  Loading fails loudly when the library
+is missing: there is no CPU fallback on the product path."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvoila_hip.so")
+_LIB = None
+
+
+class VoilaHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i32, i64 = C.c_void_p, C.c_size_t, C.c_int32, C.c_int64
+        L.vh_last_error.restype = C.c_char_p
+        L.vh_init.argtypes = [C.c_int]
+        L.vh_malloc.argtypes = [C.POINTER(vp), sz]
+        L.vh_free.argtypes = [vp]
+        L.vh_memcpy_h2d.argtypes = [vp, vp, sz]
+        L.vh_memcpy_d2h.argtypes = [vp, vp, sz]
+        L.vh_stream_create.argtypes = [C.POINTER(vp)]
+        L.vh_stream_destroy.argtypes = [vp]
+        L.vh_stream_synchronize.argtypes = [vp]
+        L.vh_event_create.argtypes = [C.POINTER(vp)]
+        L.vh_event_destroy.argtypes = [vp]
+        L.vh_event_record.argtypes = [vp, vp]
+        L.vh_event_elapsed_ms.argtypes = [C.POINTER(C.c_float), vp, vp]
+        L.vh_query_new.argtypes = [C.POINTER(vp), vp, vp]
+        L.vh_query_run.argtypes = [vp, vp]
+        L.vh_query_run_phase.argtypes = [vp, vp, C.c_int]
+        L.vh_query_result.argtypes = [vp, vp, C.c_char_p, sz, C.POINTER(sz), C.POINTER(i64)]
+        L.vh_query_reset.argtypes = [vp]
+        L.vh_query_free.argtypes = [vp]
+        L.vh_query_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+        L.vh_type_from_minmax.argtypes = [C.c_double, C.c_double]
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise VoilaHipError(rc, lib().vh_last_error().decode())
+    return rc
+
+
+_INIT = False
+
+
+def init(device=0):
+    global _INIT
+    if not _INIT:
+        check(lib().vh_init(device))
+        _INIT = True
+
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        init()
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().vh_malloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        if arr.nbytes:
+            check(lib().vh_memcpy_h2d(b.ptr, arr.ctypes.data, arr.nbytes))
+        return b
+
+    def to_numpy(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().vh_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().vh_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def upload(arrays: dict) -> dict:
+    """Device copies of a dict of numpy columns; returns name -> DeviceBuffer."""
+    return {k: DeviceBuffer.from_numpy(v) for k, v in arrays.items()}
+
+
+def addrs(bufs: dict) -> dict:
+    return {k: v.ptr for k, v in bufs.items()}
+
+
+class Stream:
+    def __init__(self):
+        init()
+        p = C.c_void_p()
+        check(lib().vh_stream_create(C.byref(p)))
+        self.ptr = p.value
+
+    def synchronize(self):
+        check(lib().vh_stream_synchronize(self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().vh_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        init()
+        p = C.c_void_p()
+        check(lib().vh_event_create(C.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        check(lib().vh_event_record(self.ptr, stream.ptr if stream else None))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib().vh_event_elapsed_ms(C.byref(ms), self.ptr, end.ptr))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().vh_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def make_config(group_mode=-1, grid=0, table_shift=0):
+    c = abi.vh_config()
+    c.group_mode = group_mode
+    c.grid = grid
+    c.table_shift = table_shift
+    return c
+
+
+class Query:
+    """voila_query_new / run / reset / free over a vh_pipeline_desc whose
+    column pointers are device addresses."""
+
+    def __init__(self, desc: abi.vh_pipeline_desc, config: abi.vh_config | None = None):
+        init()
+        self.desc = desc
+        self.config = config
+        p = C.c_void_p()
+        check(lib().vh_query_new(C.byref(p), C.byref(desc),
+                                 C.byref(config) if config is not None else None))
+        self.ptr = p.value
+
+    def run(self, stream: Stream | None = None):
+        check(lib().vh_query_run(self.ptr, stream.ptr if stream else None))
+
+    def run_phase(self, phase: int, stream: Stream | None = None):
+        check(lib().vh_query_run_phase(self.ptr, stream.ptr if stream else None, phase))
+
+    def result(self, stream: Stream | None = None) -> str:
+        n = C.c_size_t(0)
+        rows = C.c_int64(0)
+        s = stream.ptr if stream else None
+        check(lib().vh_query_result(self.ptr, s, None, 0, C.byref(n), C.byref(rows)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().vh_query_result(self.ptr, s, buf, n.value + 1, C.byref(n), C.byref(rows)))
+        return buf.value.decode()
+
+    def info(self):
+        m, g, n = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().vh_query_info(self.ptr, C.byref(m), C.byref(g), C.byref(n)))
+        return {"group_mode": m.value, "grid": g.value, "groups": n.value}
+
+    def reset(self):
+        check(lib().vh_query_reset(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            lib().vh_query_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def run_pipeline(pipeline_builder, config=None) -> str:
+    """Convenience: build desc from a Pipeline whose columns are device
+    addresses, run once, return the result text."""
+    q = Query(pipeline_builder.desc(), config)
+    try:
+        q.run()
+        return q.result()
+    finally:
+        q.free()

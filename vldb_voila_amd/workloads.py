@@ -1,0 +1,36 @@
+"""The benchmark pipelines, as vh_pipeline_desc over given column addresses."""
+from __future__ import annotations
+
+from . import abi
+from .datagen import SPAN_START
+from .pipeline import Pipeline, col, table_from_arrays
+
+FILTER_GROUPBY_CUTOFF = SPAN_START + 2435   # keeps 2436 of 2526 days (~96.4%)
+JOIN_DATE_CUTOFF = SPAN_START + 1262        # keeps 1263 of 2526 days (50%)
+
+
+def filter_groupby_pipeline(t: dict, addrs=None) -> Pipeline:
+    """select date <= cutoff; group by (flag, status):
+    sum(qty), sum(price), sum(price * (100 - disc)), count."""
+    lineitem = table_from_arrays("lineitem", t, addrs)
+    p = Pipeline(lineitem)
+    p.select("date", "<=", FILTER_GROUPBY_CUTOFF)
+    p.groupby(["flag", "status"], [
+        ("sum", col("qty")),
+        ("sum", col("price")),
+        ("sum", col("price") * (100 - col("disc"))),
+        ("count",),
+    ])
+    return p
+
+
+def join_aggregate_pipeline(build: dict, probe: dict, build_addrs=None, probe_addrs=None,
+                            variant=abi.VH_JOINN) -> Pipeline:
+    """select date <= cutoff; join fk = key; group by segment: sum(price)."""
+    r = table_from_arrays("build", build, build_addrs)
+    s = table_from_arrays("probe", probe, probe_addrs)
+    p = Pipeline(s)
+    p.select("date", "<=", JOIN_DATE_CUTOFF)
+    p.join(r, "key", "fk", payload=["segment"], variant=variant, names=["segment"])
+    p.groupby(["segment"], [("sum", col("price"))])
+    return p
