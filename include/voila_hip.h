@@ -155,7 +155,9 @@ enum vh_group_mode {
 	VH_GROUP_REG = 0,      /* <= 8 dense groups in per-lane registers         */
 	VH_GROUP_LDS = 1,      /* dense groups in a per-block LDS table           */
 	VH_GROUP_GLOBAL = 2,   /* dense groups, global atomics                    */
-	VH_GROUP_HASH = 3      /* packed key, global CAS open addressing          */
+	VH_GROUP_HASH = 3,     /* packed key, global CAS open addressing          */
+	VH_GROUP_LDSHASH = 4   /* per-block LDS hash table over the dense index,
+	                          global atomics when it is full                  */
 };
 
 typedef struct vh_config {
@@ -217,6 +219,67 @@ int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t
 
 /* Type the reference's TypingPass gives [min, max] (typing_pass.cpp:9-22). */
 int vh_type_from_minmax(double min, double max);
+
+/* ------------------------------------------------------------------------ */
+/* Statement level: the vec_* primitive family (kernel_generator.py:178-660) */
+/* over whole device columns.  Calling convention of the generated           */
+/* primitives (kernel_generator.py:42-43): sel == NULL means the dense range */
+/* [0, inum), else inum ascending int32 positions; maps write res[i] at the  */
+/* same position i; selections write a compacted list and its length into   */
+/* the device int32 *d_onum.  Row handles are row + 1, 0 = empty, as the     */
+/* reference's row pointers (0 = NULL).                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Device chained hash table: IHashTable (runtime_struct.hpp:453-503) with
+ * NSM rows stored as columns of 64-bit words (128-bit for `wide` columns,
+ * which hold sums and counts).  Sized once for max_rows; its bucket array
+ * is calc_num_buckets(max_rows) (runtime_struct.cpp:790-814). */
+typedef struct vh_table vh_table;
+int vh_table_new(vh_table** t, int32_t ncols, const int32_t* wide, int64_t max_rows);
+int vh_table_free(vh_table* t);
+int vh_table_reset(vh_table* t, void* stream);               /* ITable::reset   */
+int vh_table_rows(vh_table* t, void* stream, int64_t* rows);  /* synchronises   */
+int vh_table_column(vh_table* t, int32_t col, void** dev_ptr);
+
+int vh_vec_seltrue(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
+	int32_t* d_onum, void* stream);                           /* vec_seltrue  */
+int vh_vec_selfalse(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
+	int32_t* d_onum, void* stream);                           /* vec_selfalse */
+/* vec_<cmp>: res[i] = col1[i] cmp (col2 ? col2[i] : constant), both typed `type` */
+int vh_vec_cmp(int32_t cmp, int32_t type, const int32_t* sel, int64_t inum, uint8_t* res,
+	const void* col1, const void* col2, int64_t constant, void* stream);
+/* vec_add/sub/mul: both operands cast to res_type first (kernel_generator.py:439-445) */
+int vh_vec_arith(int32_t op, int32_t res_type, const int32_t* sel, int64_t inum, void* res,
+	int32_t t1, const void* col1, int32_t t2, const void* col2, int64_t constant, void* stream);
+int vh_vec_hash(int32_t type, const int32_t* sel, int64_t inum, uint64_t* res, const void* col,
+	void* stream);                                           /* vec_hash     */
+int vh_vec_rehash(int32_t type, const int32_t* sel, int64_t inum, uint64_t* res,
+	const uint64_t* hash, const void* col, void* stream);    /* vec_rehash   */
+int vh_vec_bucket_lookup(vh_table* t, const int32_t* sel, int64_t inum, uint64_t* res,
+	const uint64_t* hash, void* stream);                     /* kernel_generator.py:525-541 */
+int vh_vec_bucket_next(vh_table* t, const int32_t* sel, int64_t inum, uint64_t* res,
+	const uint64_t* bucket, void* stream);                   /* :543-555 */
+/* bucket_insert (:470-519): a new row per bucket and call; a tuple whose head
+ * already points at a row of this call gets 0 and retries. */
+int vh_vec_bucket_insert(vh_table* t, const int32_t* sel, int64_t inum, uint64_t* res,
+	const uint64_t* hash, void* stream);
+int vh_vec_check(vh_table* t, int32_t col, const int32_t* sel, int64_t inum, uint8_t* res,
+	const uint64_t* bucket, int32_t key_type, const void* key, void* stream);   /* :219-244 */
+int vh_vec_gather(vh_table* t, int32_t col, const int32_t* sel, int64_t inum, void* res,
+	int32_t res_type, const uint64_t* bucket, void* stream);                     /* :322-326 */
+int vh_vec_scatter(vh_table* t, int32_t col, const int32_t* sel, int64_t inum,
+	const uint64_t* bucket, int32_t val_type, const void* val, void* stream);     /* :346-357 */
+int vh_vec_aggr_sum(vh_table* t, int32_t col, const int32_t* sel, int64_t inum,
+	const uint64_t* bucket, int32_t val_type, const void* val, void* stream);     /* :289-312 */
+int vh_vec_aggr_count(vh_table* t, int32_t col, const int32_t* sel, int64_t inum,
+	const uint64_t* bucket, void* stream);
+/* write_pos + write (relalg_translator.cpp:828-907): append inum rows;
+ * col == -1 writes the hash column. */
+int vh_vec_write_pos(vh_table* t, int64_t inum, int64_t* base);
+int vh_vec_write(vh_table* t, int32_t col, int64_t base, const int32_t* sel, int64_t inum,
+	int32_t val_type, const void* val, void* stream);
+/* bucket_build (ITable::build_index, runtime_struct.cpp:1283-1324) */
+int vh_vec_bucket_build(vh_table* t, void* stream);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,8 @@ from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
 
 pytestmark = pytest.mark.gpu
 
-MODES = [abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH]
+MODES = [abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH,
+         abi.VH_GROUP_LDSHASH]
 
 
 def gpu_rows(builder_fn, mode=abi.VH_GROUP_AUTO, **cfg):
@@ -142,7 +143,7 @@ def test_many_groups_hash_and_global():
         return p
     want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
     p, devs = on_gpu([t], make)
-    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH):
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
         got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
         assert got == want, mode
 

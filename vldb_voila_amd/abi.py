@@ -78,7 +78,7 @@ class vh_pipeline_desc(C.Structure):
                 ("aggrs", vh_aggr_desc * VH_MAX_AGGRS), ("expr", vh_expr * VH_MAX_EXPR)]
 
 
-VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH = -1, 0, 1, 2, 3
+VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH, VH_GROUP_LDSHASH = -1, 0, 1, 2, 3, 4
 
 
 class vh_config(C.Structure):

@@ -437,15 +437,20 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	const size_t lds_budget = 64 * 1024;
 	const double lds_need = domain * P.nsums * 12 + domain * 4 + 16;
 	if (mode < 0) {
+		// built-in heuristic; the autotuner and the learned selector override it
 		if (domain <= kRegSlots && P.nsums <= kRegSums)
 			mode = GB_REG;
-		else if (lds_need <= lds_budget)
+		else if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
+		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
+			mode = GB_LDSHASH;
 		else if (domain <= (double)(1 << 24))
 			mode = GB_GLOBAL;
 		else
 			mode = GB_HASH;
 	}
+	if (mode == GB_LDSHASH && !(domain <= (double)(1 << 26) && P.nsums <= 4))
+		return fail(VH_ERR_UNSUPPORTED, "LDS hash group-by needs a dense domain <= 2^26 and <= 4 sums");
 	if (mode == GB_REG && !(domain <= kRegSlots && P.nsums <= kRegSums))
 		return fail(VH_ERR_UNSUPPORTED, "register group-by needs <= %d groups and <= %d sums",
 			kRegSlots, kRegSums);
@@ -455,7 +460,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		return fail(VH_ERR_UNSUPPORTED, "dense global group-by domain %.0f too large", domain);
 	if (mode == GB_HASH && bits > 63)
 		return fail(VH_ERR_UNSUPPORTED, "packed group key needs %d bits", bits);
-	if (mode < GB_REG || mode > GB_HASH)
+	if (mode < GB_REG || mode > GB_LDSHASH)
 		return fail(VH_ERR_INVALID, "group mode %d", mode);
 	P.mode = mode;
 	if (mode == GB_HASH) {
@@ -476,7 +481,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		if (mode == GB_REG || mode == GB_LDS)
 			L.wide |= P.sums[s].wide != 0;
 		else
-			L.wide |= P.sums[s].res128 != 0;
+			L.wide |= P.sums[s].res128 != 0;   // GLOBAL, HASH, LDSHASH: exact atomics
 	}
 
 	// launch shape: enough blocks for 256 CUs, capped so partials stay small
@@ -575,7 +580,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
 	for (int j = 0; j < P.njoins; j++)
 		HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
-	if (P.mode == GB_GLOBAL || P.mode == GB_HASH) {
+	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH) {
 		const u64 groups = L.groups;
 		const int nsums = std::max(P.nsums, 1);
 		HIP_TRY(launch_fill(P.out_lo, groups * nsums, 0, s));

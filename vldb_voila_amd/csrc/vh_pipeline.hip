@@ -108,7 +108,7 @@ struct Acc {
 
 template <int MODE, bool WIDE>
 __device__ __forceinline__ void consume(const Plan& P, i64 row, const PayRegs& pay,
-	Acc<MODE, WIDE>& A, u64* lds_lo, int* lds_hi, u32* lds_cnt)
+	Acc<MODE, WIDE>& A, u64* lds_lo, int* lds_hi, u32* lds_cnt, u32* lds_keys)
 {
 	if (MODE == GB_HASH) {
 		u64 packed = 0;
@@ -219,6 +219,41 @@ __device__ __forceinline__ void consume(const Plan& P, i64 row, const PayRegs& p
 		return;
 	}
 
+	if (MODE == GB_LDSHASH) {
+		const u32 key = (u32)g;
+		u32 slot = (u32)(voila_hash((u64)key)) & (kLdsHashSlots - 1);
+		int found = -1;
+		for (int n = 0; n < kLdsHashProbes; n++) {
+			const u32 k = lds_keys[slot];
+			if (k == key) { found = (int)slot; break; }
+			if (k == 0xFFFFFFFFu) {
+				const u32 old = atomicCAS(&lds_keys[slot], 0xFFFFFFFFu, key);
+				if (old == 0xFFFFFFFFu || old == key) { found = (int)slot; break; }
+			}
+			slot = (slot + 1) & (kLdsHashSlots - 1);
+		}
+		if (found >= 0) {
+			atomicAdd(&lds_cnt[found], 1u);
+#pragma unroll
+			for (int s = 0; s < VH_MAX_AGGRS; s++) {
+				if (s < P.nsums) {
+					const u64 v = eval_sum(P.sums[s], row, pay);
+					const int idx = s * kLdsHashSlots + found;
+					if (WIDE && P.sums[s].wide) {
+						const u64 o = atomicAdd(&lds_lo[idx], v);
+						const int h = (int)((o + v) < o) - (int)(P.sums[s].arg_signed && (i64)v < 0);
+						if (h)
+							atomicAdd(&lds_hi[idx], h);
+					} else {
+						atomicAdd(&lds_lo[idx], v);
+					}
+				}
+			}
+			return;
+		}
+		// LDS table full: fall through to the global arrays
+	}
+
 	// GB_GLOBAL
 	atomicAdd(&P.out_cnt[g], 1ull);
 #pragma unroll
@@ -237,7 +272,7 @@ __device__ __forceinline__ void consume(const Plan& P, i64 row, const PayRegs& p
 
 template <int MODE, bool WIDE>
 __device__ __forceinline__ void process_row(const Plan& P, i64 row, Acc<MODE, WIDE>& A,
-	u64* lds_lo, int* lds_hi, u32* lds_cnt)
+	u64* lds_lo, int* lds_hi, u32* lds_cnt, u32* lds_keys)
 {
 	if (!preds_pass(P.preds, P.npreds, row))
 		return;
@@ -254,7 +289,7 @@ __device__ __forceinline__ void process_row(const Plan& P, i64 row, Acc<MODE, WI
 		}
 	}
 	if (P.njoins == 0) {
-		consume<MODE, WIDE>(P, row, pay, A, lds_lo, lds_hi, lds_cnt);
+		consume<MODE, WIDE>(P, row, pay, A, lds_lo, lds_hi, lds_cnt, lds_keys);
 		return;
 	}
 	// last join: every match continues (JoinN) or the first one (Join01)
@@ -276,7 +311,7 @@ __device__ __forceinline__ void process_row(const Plan& P, i64 row, Acc<MODE, WI
 			case 2: pay.p[2] = p; break;
 			default: pay.p[3] = p; break;
 			}
-			consume<MODE, WIDE>(P, row, pay, A, lds_lo, lds_hi, lds_cnt);
+			consume<MODE, WIDE>(P, row, pay, A, lds_lo, lds_hi, lds_cnt, lds_keys);
 			if (J.variant == VH_JOIN01)
 				return;
 		}
@@ -350,6 +385,7 @@ __global__ __launch_bounds__(256) void pipeline_kernel(const Plan* __restrict__ 
 	u64* lds_lo = nullptr;
 	int* lds_hi = nullptr;
 	u32* lds_cnt = nullptr;
+	u32* lds_keys = nullptr;
 
 	if (MODE == GB_REG) {
 #pragma unroll
@@ -376,6 +412,23 @@ __global__ __launch_bounds__(256) void pipeline_kernel(const Plan* __restrict__ 
 		__syncthreads();
 	}
 
+	if (MODE == GB_LDSHASH) {
+		const int n = P.nsums * kLdsHashSlots;
+		lds_lo = lds;
+		lds_hi = (int*)(lds + n);
+		lds_cnt = (u32*)(lds_hi + n);
+		lds_keys = lds_cnt + kLdsHashSlots;
+		for (int i = threadIdx.x; i < n; i += blockDim.x) {
+			lds_lo[i] = 0;
+			lds_hi[i] = 0;
+		}
+		for (int i = threadIdx.x; i < kLdsHashSlots; i += blockDim.x) {
+			lds_cnt[i] = 0;
+			lds_keys[i] = 0xFFFFFFFFu;
+		}
+		__syncthreads();
+	}
+
 	const i64 stride = (i64)gridDim.x * blockDim.x;
 	constexpr int U = 4;
 	for (i64 base = (i64)blockIdx.x * blockDim.x + threadIdx.x; base < P.rows; base += stride * U) {
@@ -383,8 +436,31 @@ __global__ __launch_bounds__(256) void pipeline_kernel(const Plan* __restrict__ 
 		for (int u = 0; u < U; u++) {
 			const i64 row = base + u * stride;
 			if (row < P.rows)
-				process_row<MODE, WIDE>(P, row, A, lds_lo, lds_hi, lds_cnt);
+				process_row<MODE, WIDE>(P, row, A, lds_lo, lds_hi, lds_cnt, lds_keys);
 		}
+	}
+
+	if (MODE == GB_LDSHASH) {
+		// flush this block's LDS groups into the global arrays
+		__syncthreads();
+		for (int i = threadIdx.x; i < kLdsHashSlots; i += blockDim.x) {
+			const u32 key = lds_keys[i];
+			if (key == 0xFFFFFFFFu)
+				continue;
+			atomicAdd(&P.out_cnt[key], (u64)lds_cnt[i]);
+			for (int s = 0; s < P.nsums; s++) {
+				const SumArg& S = P.sums[s];
+				const u64 lo = lds_lo[s * kLdsHashSlots + i];
+				const i64 hi = (WIDE && S.wide) ? (i64)lds_hi[s * kLdsHashSlots + i]
+					: (S.arg_signed ? (i64)lo >> 63 : 0);
+				const u64 idx = (u64)s * P.domain + key;
+				if (S.res128)
+					atomic_add_i128(&P.out_lo[idx], &P.out_hi[idx], lo, hi);
+				else
+					atomicAdd(&P.out_lo[idx], lo);
+			}
+		}
+		return;
 	}
 
 	if (MODE != GB_REG && MODE != GB_LDS)
@@ -535,6 +611,8 @@ hipError_t launch_build(const BuildPlan* B, int grid, hipStream_t s)
 
 size_t pipeline_lds_bytes(const Plan& P)
 {
+	if (P.mode == GB_LDSHASH)
+		return (size_t)P.nsums * kLdsHashSlots * 12 + kLdsHashSlots * 8 + 16;
 	if (P.mode == GB_LDS)
 		return P.domain * P.nsums * 12 + P.domain * 4 + 16;
 	if (P.mode == GB_REG)
@@ -550,6 +628,7 @@ hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, bool wide, h
 	case GB_REG: if (wide) VH_LAUNCH(GB_REG, true); else VH_LAUNCH(GB_REG, false); break;
 	case GB_LDS: if (wide) VH_LAUNCH(GB_LDS, true); else VH_LAUNCH(GB_LDS, false); break;
 	case GB_GLOBAL: if (wide) VH_LAUNCH(GB_GLOBAL, true); else VH_LAUNCH(GB_GLOBAL, false); break;
+	case GB_LDSHASH: if (wide) VH_LAUNCH(GB_LDSHASH, true); else VH_LAUNCH(GB_LDSHASH, false); break;
 	default: if (wide) VH_LAUNCH(GB_HASH, true); else VH_LAUNCH(GB_HASH, false); break;
 	}
 #undef VH_LAUNCH

@@ -77,7 +77,13 @@ struct KeyT {
 	int pad;
 };
 
-enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3 };
+enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH = 4 };
+
+// GB_LDSHASH: per-block LDS open-addressing table over the dense group index,
+// kLdsHashSlots entries, at most kLdsHashProbes probes; a tuple that finds no
+// slot goes straight to the global arrays (same layout as GB_GLOBAL).
+static constexpr int kLdsHashSlots = 1024;
+static constexpr int kLdsHashProbes = 32;
 
 static constexpr int kRegSlots = 8;    // dense groups held in registers
 static constexpr int kRegSums = 4;     // sums held in registers

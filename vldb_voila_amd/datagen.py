@@ -111,3 +111,47 @@ def gen_join_aggregate(build_rows=1_500_000, probe_rows=6_000_000, seed=42):
         "price": uniform_int(seed, 16, probe_rows, 90000, 10500000, np.int64),
     }
     return build, probe
+
+
+def _years_of_days(days: np.ndarray) -> np.ndarray:
+    """Calendar year of Julian day numbers (proleptic Gregorian)."""
+    a = days.astype(np.int64) + 32044
+    b = (4 * a + 3) // 146097
+    c = a - 146097 * b // 4
+    d = (4 * c + 3) // 1461
+    e = c - 1461 * d // 4
+    m = (5 * e + 2) // 153
+    month = m + 3 - 12 * (m // 10)
+    return (100 * b + d - 4800 + m // 10).astype(np.int32) + 0 * month
+
+
+def gen_star(fact_rows=60_000_000, zipf_part=0.0, seed=42, dims_scale=1.0):
+    """Star schema: fact with four FKs; date (2,556 days from 1992-01-01,
+    year 1992..1998), customer / supplier (region U5, nation = 5 * region +
+    U5), part (category U25, brand U1000)."""
+    n_date = 2556
+    n_cust = max(1, int(300_000 * dims_scale))
+    n_supp = max(1, int(20_000 * dims_scale))
+    n_part = max(1, int(1_400_000 * dims_scale))
+    dkey = np.arange(SPAN_START, SPAN_START + n_date, dtype=np.int32)
+    date = {"d_key": dkey, "d_year": _years_of_days(dkey)}
+    c_region = uniform_int(seed, 21, n_cust, 0, 5, np.int32)
+    customer = {"c_key": (permutation(seed, 22, n_cust) + 1).astype(np.int32),
+                "c_region": c_region,
+                "c_nation": (c_region * 5 + uniform_int(seed, 23, n_cust, 0, 5, np.int32)).astype(np.int32)}
+    s_region = uniform_int(seed, 24, n_supp, 0, 5, np.int32)
+    supplier = {"s_key": (permutation(seed, 25, n_supp) + 1).astype(np.int32),
+                "s_region": s_region,
+                "s_nation": (s_region * 5 + uniform_int(seed, 26, n_supp, 0, 5, np.int32)).astype(np.int32)}
+    part = {"p_key": (permutation(seed, 27, n_part) + 1).astype(np.int32),
+            "p_category": uniform_int(seed, 28, n_part, 0, 25, np.int32),
+            "p_brand": uniform_int(seed, 29, n_part, 0, 1000, np.int32)}
+    fact = {
+        "f_date": uniform_int(seed, 31, fact_rows, SPAN_START, SPAN_START + n_date, np.int32),
+        "f_cust": uniform_int(seed, 32, fact_rows, 1, n_cust + 1, np.int32),
+        "f_supp": uniform_int(seed, 33, fact_rows, 1, n_supp + 1, np.int32),
+        "f_part": zipf_ranks(seed, 34, fact_rows, n_part, zipf_part).astype(np.int32),
+        "f_revenue": uniform_int(seed, 35, fact_rows, 100_000, 10_000_000, np.int64),
+        "f_cost": uniform_int(seed, 36, fact_rows, 10_000, 1_000_000, np.int64),
+    }
+    return fact, date, customer, supplier, part

@@ -34,3 +34,26 @@ def join_aggregate_pipeline(build: dict, probe: dict, build_addrs=None, probe_ad
     p.join(r, "key", "fk", payload=["segment"], variant=variant, names=["segment"])
     p.groupby(["segment"], [("sum", col("price"))])
     return p
+
+
+STAR_REGION = 2
+
+
+def star_pipeline(fact, date, customer, supplier, part, addrs=None, region=STAR_REGION) -> Pipeline:
+    """Star join: fact x customer (region = r) x supplier (region = r) x date
+    x part; group by (c_nation, s_nation, year): sum(revenue - cost).
+    The filtered dimensions are joined first; part (no filter, no payload)
+    last, so only surviving tuples probe its 1.4M-key table."""
+    a = addrs or {}
+    f = table_from_arrays("fact", fact, a.get("fact"))
+    p = Pipeline(f)
+    p.join(table_from_arrays("customer", customer, a.get("customer")), "c_key", "f_cust",
+           payload=["c_nation"], build_preds=[("c_region", "==", region)], names=["c_nation"])
+    p.join(table_from_arrays("supplier", supplier, a.get("supplier")), "s_key", "f_supp",
+           payload=["s_nation"], build_preds=[("s_region", "==", region)], names=["s_nation"])
+    p.join(table_from_arrays("date", date, a.get("date")), "d_key", "f_date",
+           payload=["d_year"], names=["d_year"])
+    p.join(table_from_arrays("part", part, a.get("part")), "p_key", "f_part", payload=[],
+           names=[])
+    p.groupby(["c_nation", "s_nation", "d_year"], [("sum", col("f_revenue") - col("f_cost")), ("count",)])
+    return p
