@@ -164,7 +164,8 @@ typedef struct vh_config {
 	int32_t group_mode;    /* enum vh_group_mode                               */
 	int32_t grid;          /* probe-kernel workgroups, 0 = auto                */
 	int32_t table_shift;   /* join table size vs calc_num_buckets: <<n / >>-n  */
-	int32_t reserved[5];
+	int32_t tile_shift;    /* batched work item = 65536 << n rows (n < 0: >>)  */
+	int32_t reserved[4];
 } vh_config;
 
 /* ------------------------------------------------------------------------ */
@@ -280,6 +281,25 @@ int vh_vec_write(vh_table* t, int32_t col, int64_t base, const int32_t* sel, int
 	int32_t val_type, const void* val, void* stream);
 /* bucket_build (ITable::build_index, runtime_struct.cpp:1283-1324) */
 int vh_vec_bucket_build(vh_table* t, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched pipelines: many independent pipelines in one persistent launch.   */
+/* The reference runs pipelines one at a time, each with its own flavor     */
+/* (runtime_framework.cpp:172-198, QueryConfig::pipeline_default_blend at   */
+/* runtime_framework.hpp:78); cfgs[i] is pipeline i's variant.              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vh_batch vh_batch;
+int vh_batch_new(vh_batch** b, int32_t n, const vh_pipeline_desc* descs, const vh_config* cfgs,
+	int32_t grid);
+/* Enqueue: one multi-buffer clear + one persistent kernel (asynchronous). */
+int vh_batch_run(vh_batch* b, void* stream);
+/* Same work, captured once into a hipGraph and replayed (stream != NULL). */
+int vh_batch_run_graph(vh_batch* b, void* stream);
+int vh_batch_result(vh_batch* b, int32_t i, void* stream, char* buf, size_t cap, size_t* len,
+	int64_t* nrows);
+int vh_batch_info(const vh_batch* b, int32_t* items, int32_t* grid, int64_t* lds_bytes);
+int vh_batch_free(vh_batch* b);
 
 #ifdef __cplusplus
 }

@@ -83,4 +83,4 @@ VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH, VH_GR
 
 class vh_config(C.Structure):
     _fields_ = [("group_mode", C.c_int32), ("grid", C.c_int32), ("table_shift", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("tile_shift", C.c_int32), ("reserved", C.c_int32 * 4)]

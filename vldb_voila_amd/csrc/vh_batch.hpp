@@ -1,0 +1,29 @@
+// Work items and launchers of the persistent batched kernel (vh_batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vh_plan.hpp"
+
+namespace vh {
+
+struct BatchItem {
+	int plan;            // pipeline index
+	int phase;           // 0 = build rows of builds[build], 1 = probe rows of plans[plan]
+	int build;           // index into the build plan array (phase 0)
+	int need;            // build items of this pipeline (phase 1 waits for them)
+	i64 r0;
+	i64 r1;
+};
+
+struct FillSeg {
+	u64* p;
+	u64 n;
+	u64 v;
+};
+
+hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s);
+hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 nitems,
+	u32* d_ctl, int grid, size_t lds_bytes, hipStream_t s);
+
+} // namespace vh

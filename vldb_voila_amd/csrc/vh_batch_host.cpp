@@ -1,0 +1,245 @@
+// Host side of the persistent batched launch: lowers every pipeline like a
+// single query, lays out the work queue (all build items, then all probe
+// items) and enqueues one multi-buffer clear plus one persistent kernel, or
+// replays both from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/voila_hip.h"
+#include "vh_batch.hpp"
+#include "vh_engine.hpp"
+#include "vh_kernels.hpp"
+
+using namespace vh;
+
+#define HIP_CALL(x)                                                                  \
+	do {                                                                             \
+		hipError_t _e = (x);                                                         \
+		if (_e != hipSuccess)                                                        \
+			return fail(VH_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e));           \
+	} while (0)
+
+struct vh_batch {
+	std::vector<Query*> q;
+	std::vector<void*> allocs;
+	Plan* d_plans = nullptr;
+	BuildPlan* d_builds = nullptr;
+	BatchItem* d_items = nullptr;
+	FillSeg* d_segs = nullptr;
+	u32* d_ctl = nullptr;
+	u32 nitems = 0;
+	int nsegs = 0;
+	int grid = 0;
+	size_t lds = 0;
+	hipGraphExec_t graph = nullptr;
+	hipGraph_t graph_def = nullptr;
+	hipStream_t graph_stream = nullptr;
+	~vh_batch()
+	{
+		if (graph)
+			(void)hipGraphExecDestroy(graph);
+		if (graph_def)
+			(void)hipGraphDestroy(graph_def);
+		for (Query* x : q)
+			delete x;
+		for (void* p : allocs)
+			(void)hipFree(p);
+	}
+	int upload(void** dst, const void* src, size_t bytes)
+	{
+		hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
+		if (e != hipSuccess)
+			return fail(VH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+		allocs.push_back(*dst);
+		if (bytes && (e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess)
+			return fail(VH_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+		return VH_OK;
+	}
+};
+
+static int tile_rows_of(const vh_config& c)
+{
+	int shift = c.tile_shift;
+	if (shift < -6) shift = -6;
+	if (shift > 6) shift = 6;
+	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
+}
+
+extern "C" {
+
+int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const vh_config* cfgs, int32_t grid)
+{
+	if (!out || n <= 0 || !descs)
+		return fail(VH_ERR_INVALID, "vh_batch_new arguments");
+	*out = nullptr;
+	vh_batch* b = new (std::nothrow) vh_batch();
+	if (!b)
+		return fail(VH_ERR_NOMEM, "host allocation");
+	std::vector<Plan> plans;
+	std::vector<BuildPlan> builds;
+	std::vector<BatchItem> build_items, probe_items;
+	std::vector<FillSeg> segs;
+	int rc = VH_OK;
+	for (int i = 0; i < n && !rc; i++) {
+		Query* q = new (std::nothrow) Query();
+		if (!q) { rc = fail(VH_ERR_NOMEM, "host allocation"); break; }
+		b->q.push_back(q);
+		memcpy(&q->desc, &descs[i], sizeof(vh_pipeline_desc));
+		if (cfgs) {
+			q->cfg = cfgs[i];
+		} else {
+			memset(&q->cfg, 0, sizeof(q->cfg));
+			q->cfg.group_mode = VH_GROUP_AUTO;
+		}
+		vh_config lc = q->cfg;
+		if (lc.group_mode == VH_GROUP_REG)
+			lc.group_mode = VH_GROUP_LDS;      // register state cannot outlive an item
+		if ((rc = lower(&q->desc, &lc, q->L))) {
+			fail(rc, "pipeline %d: %s", i, last_error());
+			break;
+		}
+		Plan& P = q->L.plan;
+		if (P.mode == GB_REG)
+			P.mode = GB_LDS;            // register state cannot outlive an item
+		if ((rc = query_create(q))) {
+			fail(rc, "pipeline %d: %s", i, last_error());
+			break;
+		}
+		b->lds = std::max(b->lds, pipeline_lds_bytes(P));
+		const int T = tile_rows_of(q->cfg);
+		const int first_build = (int)builds.size();
+		int need = 0;
+		for (int j = 0; j < P.njoins; j++) {
+			builds.push_back(q->L.builds[j]);
+			for (i64 r0 = 0; r0 < q->L.builds[j].rows; r0 += T) {
+				build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, q->L.builds[j].rows)});
+				need++;
+			}
+			segs.push_back({q->slots[j], q->L.capacity[j], ~0ull});
+		}
+		for (i64 r0 = 0; r0 < P.rows; r0 += T)
+			probe_items.push_back({i, 1, -1, need, r0, std::min<i64>(r0 + T, P.rows)});
+		const u64 groups = q->L.groups;
+		const u64 ns = (u64)std::max(P.nsums, 1);
+		segs.push_back({(u64*)q->flags, 4, 0});
+		segs.push_back({P.out_lo, groups * ns, 0});
+		segs.push_back({(u64*)P.out_hi, groups * ns, 0});
+		segs.push_back({P.out_cnt, groups, 0});
+		if (P.mode == GB_HASH)
+			segs.push_back({P.hkeys, groups, ~0ull});
+		plans.push_back(P);
+	}
+	if (rc) {
+		delete b;
+		return rc;
+	}
+	std::vector<BatchItem> items = build_items;
+	items.insert(items.end(), probe_items.begin(), probe_items.end());
+	b->nitems = (u32)items.size();
+	const size_t ctl_words = (4 + (size_t)n + 1) / 2 + 1;
+	if ((rc = b->upload((void**)&b->d_ctl, nullptr, ctl_words * 8)) ||
+		(rc = b->upload((void**)&b->d_plans, plans.data(), plans.size() * sizeof(Plan))) ||
+		(rc = b->upload((void**)&b->d_builds, builds.data(), builds.size() * sizeof(BuildPlan))) ||
+		(rc = b->upload((void**)&b->d_items, items.data(), items.size() * sizeof(BatchItem)))) {
+		delete b;
+		return rc;
+	}
+	segs.push_back({(u64*)b->d_ctl, ctl_words, 0});
+	b->nsegs = (int)segs.size();
+	if ((rc = b->upload((void**)&b->d_segs, segs.data(), segs.size() * sizeof(FillSeg)))) {
+		delete b;
+		return rc;
+	}
+	int dev = 0, cus = 256;
+	(void)hipGetDevice(&dev);
+	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	b->grid = grid > 0 ? grid : cus * 4;
+	*out = b;
+	return VH_OK;
+}
+
+int vh_batch_run(vh_batch* b, void* stream)
+{
+	if (!b)
+		return fail(VH_ERR_INVALID, "null batch");
+	hipStream_t s = (hipStream_t)stream;
+	HIP_CALL(launch_multi_fill(b->d_segs, b->nsegs, s));
+	HIP_CALL(launch_batch(b->d_plans, b->d_builds, b->d_items, b->nitems, b->d_ctl, b->grid, b->lds, s));
+	for (Query* q : b->q)
+		q->ran = true;
+	return VH_OK;
+}
+
+// Capture vh_batch_run once into a hipGraph (on `stream`, which must not be
+// the null stream) and replay it on every call.
+int vh_batch_run_graph(vh_batch* b, void* stream)
+{
+	if (!b || !stream)
+		return fail(VH_ERR_INVALID, "vh_batch_run_graph needs a batch and a non-null stream");
+	hipStream_t s = (hipStream_t)stream;
+	if (!b->graph) {
+		HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+		hipError_t e1 = launch_multi_fill(b->d_segs, b->nsegs, s);
+		hipError_t e2 = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nitems, b->d_ctl, b->grid, b->lds, s);
+		hipGraph_t g = nullptr;
+		HIP_CALL(hipStreamEndCapture(s, &g));
+		if (e1 != hipSuccess || e2 != hipSuccess) {
+			(void)hipGraphDestroy(g);
+			return fail(VH_ERR_HIP, "capture of the batch failed");
+		}
+		b->graph_def = g;
+		HIP_CALL(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+	}
+	HIP_CALL(hipGraphLaunch(b->graph, s));
+	for (Query* q : b->q)
+		q->ran = true;
+	return VH_OK;
+}
+
+int vh_batch_result(vh_batch* b, int32_t i, void* stream, char* buf, size_t cap, size_t* len, int64_t* nrows)
+{
+	if (!b || i < 0 || i >= (int)b->q.size() || !len)
+		return fail(VH_ERR_INVALID, "vh_batch_result arguments");
+	HIP_CALL(hipStreamSynchronize((hipStream_t)stream));
+	u32 ctl[2];
+	HIP_CALL(hipMemcpy(ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+	if (ctl[1])
+		return fail(VH_ERR_STATE, "batched run: a probe item gave up waiting for its build");
+	int64_t rows = 0;
+	Query* q = b->q[i];
+	int rc = query_result(q, (hipStream_t)stream, q->result, rows);
+	if (rc)
+		return rc;
+	*len = q->result.size();
+	if (nrows)
+		*nrows = rows;
+	if (buf) {
+		if (cap <= q->result.size())
+			return fail(VH_ERR_CAPACITY, "result needs %zu bytes", q->result.size() + 1);
+		memcpy(buf, q->result.data(), q->result.size());
+		buf[q->result.size()] = 0;
+	}
+	return VH_OK;
+}
+
+int vh_batch_info(const vh_batch* b, int32_t* items, int32_t* grid, int64_t* lds_bytes)
+{
+	if (!b)
+		return fail(VH_ERR_INVALID, "null batch");
+	if (items) *items = (int32_t)b->nitems;
+	if (grid) *grid = b->grid;
+	if (lds_bytes) *lds_bytes = (int64_t)b->lds;
+	return VH_OK;
+}
+
+int vh_batch_free(vh_batch* b)
+{
+	delete b;
+	return VH_OK;
+}
+
+} // extern "C"

@@ -438,7 +438,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	const double lds_need = domain * P.nsums * 12 + domain * 4 + 16;
 	if (mode < 0) {
 		// built-in heuristic; the autotuner and the learned selector override it
-		if (domain <= kRegSlots && P.nsums <= kRegSums)
+		if (domain <= kRegSlots && P.nsums <= kRegSums &&
+			domain * P.nsums * 256 * 12 + domain * 256 * 4 <= 64 * 1024)
 			mode = GB_REG;
 		else if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
@@ -451,9 +452,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	}
 	if (mode == GB_LDSHASH && !(domain <= (double)(1 << 26) && P.nsums <= 4))
 		return fail(VH_ERR_UNSUPPORTED, "LDS hash group-by needs a dense domain <= 2^26 and <= 4 sums");
-	if (mode == GB_REG && !(domain <= kRegSlots && P.nsums <= kRegSums))
-		return fail(VH_ERR_UNSUPPORTED, "register group-by needs <= %d groups and <= %d sums",
-			kRegSlots, kRegSums);
+	if (mode == GB_REG && !(domain <= kRegSlots && P.nsums <= kRegSums &&
+		domain * P.nsums * 256 * 12 + domain * 256 * 4 <= 64 * 1024))
+		return fail(VH_ERR_UNSUPPORTED, "lane-private group-by needs <= %d groups, <= %d sums and "
+			"<= 64 KiB of LDS", kRegSlots, kRegSums);
 	if (mode == GB_LDS && lds_need > lds_budget)
 		return fail(VH_ERR_UNSUPPORTED, "LDS group-by needs %.0f bytes", lds_need);
 	if (mode == GB_GLOBAL && domain > (double)(1 << 26))
@@ -476,13 +478,12 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		P.domain = (uint64_t)domain;
 		L.groups = P.domain;
 	}
+	// exact 128-bit bookkeeping is needed when a sum is i128-typed (global
+	// atomics) or its lanes can carry (wide sums)
 	L.wide = false;
-	for (int s = 0; s < P.nsums; s++) {
-		if (mode == GB_REG || mode == GB_LDS)
-			L.wide |= P.sums[s].wide != 0;
-		else
-			L.wide |= P.sums[s].res128 != 0;   // GLOBAL, HASH, LDSHASH: exact atomics
-	}
+	for (int s = 0; s < P.nsums; s++)
+		L.wide |= P.sums[s].res128 != 0;
+	P.wide = L.wide ? 1 : 0;
 
 	// launch shape: enough blocks for 256 CUs, capped so partials stay small
 	int grid = cfg && cfg->grid > 0 ? cfg->grid : 0;
@@ -493,7 +494,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	L.grid = grid;
 	L.build_grid.assign(d->njoins, 0);
 	for (int j = 0; j < d->njoins; j++)
-		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 255) / 256, 4096));
+		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 1023) / 1024, 4096));
 	return VH_OK;
 }
 
@@ -596,7 +597,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
 	}
 	if (phase == -1 || phase == 2) {
-		HIP_TRY(launch_pipeline(P, q->dplan, L.grid, L.wide, s));
+		HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
 		q->ran = true;
 	}
 	return VH_OK;
@@ -643,6 +644,8 @@ int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
 		return fail(VH_ERR_CAPACITY, "group hash table full");
 	if (flags[0] & 8u)
 		return fail(VH_ERR_INVALID, "a group key lies outside its column's [min, max]");
+	if (flags[0] & 16u)
+		return fail(VH_ERR_UNSUPPORTED, "a JoinN before the last join matched duplicate build keys");
 	for (int j = 0; j < P.njoins - 1; j++)
 		if (d.joins[j].variant == VH_JOINN && flags[1 + j])
 			return fail(VH_ERR_UNSUPPORTED,

@@ -1,0 +1,601 @@
+// Device functions of the fused scan -> select -> join* -> group-by path,
+// shared by the one-pipeline kernel (vh_pipeline.hip) and the persistent
+// batched kernel (vh_batch.hip).
+//
+// Work is done chunk-at-a-time: a lane owns U rows of a tile (rows
+// base + u * blockDim + lane, so every load instruction of a wave reads 64
+// consecutive elements).  Each column is loaded for all U rows back to back,
+// each join issues the first slot load of all U rows before resolving any of
+// them, so a lane keeps U independent memory requests in flight.  This is
+// VOILA's vector-at-a-time evaluation (a 1024-tuple vector per primitive
+// call) moved into registers.
+#pragma once
+
+#include "vh_device.hpp"
+#include "vh_plan.hpp"
+
+namespace vh {
+
+static constexpr u64 kEmpty = ~0ull;
+static constexpr int kU = 4;              // rows per lane per chunk
+
+// --------------------------------------------------------------------------
+// per-row helpers (build side and tails)
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ bool preds_pass_row(const Pred* preds, int npreds, i64 row)
+{
+	bool ok = true;
+#pragma unroll
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < npreds) {
+			const Pred& P = preds[p];
+			ok = ok && compare(P.cmp, P.r.type, load1(P.r.ptr, P.r.type, row), P.value);
+		}
+	}
+	return ok;
+}
+
+// --------------------------------------------------------------------------
+// chunk state
+// --------------------------------------------------------------------------
+
+struct Chunk {
+	i64 row[kU];
+	bool alive[kU];
+	u32 pay[VH_MAX_JOINS][kU];   // payload - pmin of each join's match
+};
+
+// Typed loads of one column for the U rows of a chunk (exec-masked by
+// `alive`, so rows that no longer qualify fetch nothing).
+__device__ __forceinline__ void load_col_u(const void* p, int t, const Chunk& C, i64 out[kU])
+{
+#define VH_LOADU(T)                                                   \
+	_Pragma("unroll") for (int u = 0; u < kU; u++)                    \
+		out[u] = C.alive[u] ? (i64)((const T*)p)[C.row[u]] : 0;
+	switch (t) {
+	case VH_U8: VH_LOADU(uint8_t); break;
+	case VH_I8: VH_LOADU(int8_t); break;
+	case VH_U16: VH_LOADU(uint16_t); break;
+	case VH_I16: VH_LOADU(int16_t); break;
+	case VH_U32: VH_LOADU(uint32_t); break;
+	case VH_I32: VH_LOADU(int32_t); break;
+	default: VH_LOADU(i64); break;
+	}
+#undef VH_LOADU
+}
+
+__device__ __forceinline__ void role_u(const Plan& P, const Role& r, const Chunk& C, i64 out[kU])
+{
+	switch (r.kind) {
+	case ROLE_COL:
+		load_col_u(r.ptr, r.type, C, out);
+		break;
+	case ROLE_PAY: {
+		const i64 pmin = P.joins[r.slot].pmin;
+		switch (r.slot) {
+#define VH_PAYU(J) _Pragma("unroll") for (int u = 0; u < kU; u++) out[u] = (i64)C.pay[J][u] + pmin;
+		case 0: VH_PAYU(0); break;
+		case 1: VH_PAYU(1); break;
+		case 2: VH_PAYU(2); break;
+		default: VH_PAYU(3); break;
+#undef VH_PAYU
+		}
+		break;
+	}
+	case ROLE_CONST:
+#pragma unroll
+		for (int u = 0; u < kU; u++)
+			out[u] = r.c;
+		break;
+	default:
+#pragma unroll
+		for (int u = 0; u < kU; u++)
+			out[u] = 0;
+		break;
+	}
+}
+
+__device__ __forceinline__ void eval_sum_u(const Plan& P, const SumArg& s, const Chunk& C, u64 prod[kU])
+{
+#pragma unroll 1
+	for (int t = 0; t < 3; t++) {
+		if (t < s.nterms) {
+			const Term& T = s.t[t];
+			i64 v[kU];
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				v[u] = T.c;
+			if (T.sa) {
+				i64 a[kU];
+				role_u(P, T.a, C, a);
+#pragma unroll
+				for (int u = 0; u < kU; u++)
+					v[u] += T.sa > 0 ? a[u] : -a[u];
+			}
+			if (T.sb) {
+				i64 b[kU];
+				role_u(P, T.b, C, b);
+#pragma unroll
+				for (int u = 0; u < kU; u++)
+					v[u] += T.sb > 0 ? b[u] : -b[u];
+			}
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				prod[u] = t == 0 ? (u64)v[u] : prod[u] * (u64)v[u];
+		}
+	}
+}
+
+// --------------------------------------------------------------------------
+// aggregation state
+// --------------------------------------------------------------------------
+
+// LDS areas of the group-by modes (all per block):
+//   GB_REG   lane-private accumulators: lo[(g*nsums+s)*bd + tid] (u64),
+//            hi[...] (int, wide sums), cnt[g*bd + tid] (u32); no atomics
+//   GB_LDS   shared dense table: lo[s*D+g], hi[s*D+g], cnt[g] with LDS atomics
+//   GB_LDSHASH shared hash table of kLdsHashSlots over the dense index
+struct Lds {
+	u64* lo;
+	int* hi;
+	u32* cnt;
+	u32* keys;
+};
+
+// Where a row's group lives: a dense index g (GB_REG / GB_LDS / GB_GLOBAL),
+// an LDS hash slot (GB_LDSHASH; -1 = LDS full, use the global arrays at g)
+// or a global hash slot (GB_HASH).
+template <int MODE>
+__device__ __forceinline__ i64 locate(const Plan& P, const Lds& L, u64 g)
+{
+	if (MODE == GB_LDSHASH) {
+		const u32 key = (u32)g;
+		u32 slot = (u32)voila_hash((u64)key) & (kLdsHashSlots - 1);
+		for (int n = 0; n < kLdsHashProbes; n++) {
+			const u32 k = L.keys[slot];
+			if (k == key)
+				return slot;
+			if (k == 0xFFFFFFFFu) {
+				const u32 old = atomicCAS(&L.keys[slot], 0xFFFFFFFFu, key);
+				if (old == 0xFFFFFFFFu || old == key)
+					return slot;
+			}
+			slot = (slot + 1) & (kLdsHashSlots - 1);
+		}
+		return -1;
+	}
+	if (MODE == GB_HASH) {
+		u64 h = voila_hash(g) & P.hmask;
+		for (u64 n = 0; n <= P.hmask; n++) {
+			const u64 k = P.hkeys[h];
+			if (k == g)
+				return (i64)h;
+			if (k == kEmpty) {
+				const u64 old = atomicCAS(&P.hkeys[h], kEmpty, g);
+				if (old == kEmpty || old == g)
+					return (i64)h;
+			}
+			h = (h + 1) & P.hmask;
+		}
+		atomicOr(P.error, 1u);   // group hash table full
+		return -1;
+	}
+	return (i64)g;
+}
+
+__device__ __forceinline__ void add_lds_exact(u64* lo, int* hi, u64 v, bool track, bool sgn)
+{
+	if (track) {
+		const u64 o = atomicAdd(lo, v);
+		const int h = (int)((o + v) < o) - (int)(sgn && (i64)v < 0);
+		if (h)
+			atomicAdd(hi, h);
+	} else {
+		atomicAdd(lo, v);
+	}
+}
+
+__device__ __forceinline__ void add_global(u64* lo, i64* hi, u64 v, bool exact, bool sgn)
+{
+	if (exact)
+		atomic_add_i128(lo, hi, v, (sgn && (i64)v < 0) ? -1 : 0);
+	else
+		atomicAdd(lo, v);
+}
+
+template <int MODE>
+__device__ __forceinline__ void add_count(const Plan& P, const Lds& L, u64 g, i64 slot)
+{
+	if (MODE == GB_REG)
+		L.cnt[g * blockDim.x + threadIdx.x] += 1u;
+	else if (MODE == GB_LDS)
+		atomicAdd(&L.cnt[g], 1u);
+	else if (MODE == GB_LDSHASH && slot >= 0)
+		atomicAdd(&L.cnt[slot], 1u);
+	else if (MODE == GB_HASH)
+		atomicAdd(&P.out_cnt[slot], 1ull);
+	else
+		atomicAdd(&P.out_cnt[g], 1ull);
+}
+
+template <int MODE>
+__device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 g, i64 slot, u64 v)
+{
+	const SumArg& S = P.sums[s];
+	const bool track = P.wide && S.wide;
+	const bool sgn = S.arg_signed;
+	if (MODE == GB_REG) {
+		const u64 idx = (g * P.nsums + s) * blockDim.x + threadIdx.x;
+		const u64 o = L.lo[idx];
+		const u64 n = o + v;
+		L.lo[idx] = n;
+		if (track)
+			L.hi[idx] += (int)(n < o) - (int)(sgn && (i64)v < 0);
+	} else if (MODE == GB_LDS) {
+		const u64 idx = (u64)s * P.domain + g;
+		add_lds_exact(&L.lo[idx], &L.hi[idx], v, track, sgn);
+	} else if (MODE == GB_LDSHASH && slot >= 0) {
+		const u64 idx = (u64)s * kLdsHashSlots + (u64)slot;
+		add_lds_exact(&L.lo[idx], &L.hi[idx], v, track, sgn);
+	} else if (MODE == GB_HASH) {
+		const u64 idx = (u64)s * (P.hmask + 1) + (u64)slot;
+		add_global(&P.out_lo[idx], &P.out_hi[idx], v, P.wide && S.res128, sgn);
+	} else {
+		const u64 idx = (u64)s * P.domain + g;
+		add_global(&P.out_lo[idx], &P.out_hi[idx], v, P.wide && S.res128, sgn);
+	}
+}
+
+// Group index (dense modes) or packed key (hash mode) of the chunk's live
+// rows, their group slots, then one sum at a time: its argument for all U
+// rows, added to each row's group.
+template <int MODE>
+__device__ __forceinline__ void consume_u(const Plan& P, const Lds& L, const Chunk& C)
+{
+	u64 g[kU];
+	bool ok[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		g[u] = 0;
+		ok[u] = C.alive[u];
+	}
+#pragma unroll 1
+	for (int k = 0; k < VH_MAX_KEYS; k++) {
+		if (k < P.nkeys) {
+			i64 kv[kU];
+			role_u(P, P.keys[k].r, C, kv);
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				const u64 off = (u64)(kv[u] - P.keys[k].min);
+				ok[u] = ok[u] && off <= P.keys[k].range;
+				g[u] += MODE == GB_HASH ? (off << P.keys[k].shift) : off * P.keys[k].mult;
+			}
+		}
+	}
+	i64 slot[kU];
+	bool live[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		live[u] = C.alive[u];
+		slot[u] = -1;
+		if (!live[u])
+			continue;
+		if (!ok[u] || (MODE != GB_HASH && g[u] >= P.domain)) {
+			atomicOr(P.error, 8u);   // group key outside its column's [min, max]
+			live[u] = false;
+			continue;
+		}
+		slot[u] = locate<MODE>(P, L, g[u]);
+		if (MODE == GB_HASH && slot[u] < 0) {
+			live[u] = false;
+			continue;
+		}
+		add_count<MODE>(P, L, g[u], slot[u]);
+	}
+#pragma unroll 1
+	for (int s = 0; s < VH_MAX_AGGRS; s++) {
+		if (s < P.nsums) {
+			u64 v[kU];
+			eval_sum_u(P, P.sums[s], C, v);
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				if (live[u])
+					add_sum<MODE>(P, L, s, g[u], slot[u], v[u]);
+		}
+	}
+}
+
+// --------------------------------------------------------------------------
+// one chunk: select -> joins -> group-by
+// --------------------------------------------------------------------------
+
+template <int MODE>
+__device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk& C, u32 dups_mask)
+{
+	// select: conjunctive predicates, one column at a time
+#pragma unroll 1
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < P.npreds) {
+			const Pred& Q = P.preds[p];
+			i64 x[kU];
+			load_col_u(Q.r.ptr, Q.r.type, C, x);
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				C.alive[u] = C.alive[u] && compare(Q.cmp, Q.r.type, x[u], Q.value);
+		}
+	}
+	// joins: first slot of every live row in flight, then resolve
+	u64 h_last[kU];
+	u32 off_last[kU];
+#pragma unroll
+	for (int j = 0; j < VH_MAX_JOINS; j++) {
+		if (j < P.njoins) {
+			const JoinT& J = P.joins[j];
+			i64 key[kU];
+			role_u(P, J.key, C, key);
+			u64 h[kU];
+			u32 off[kU];
+			u64 s[kU];
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				if (C.alive[u] && (key[u] < J.kmin || key[u] > J.kmax))
+					C.alive[u] = false;
+				off[u] = (u32)(key[u] - J.kmin);
+				h[u] = voila_hash((u64)off[u]) & J.mask;
+			}
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				s[u] = C.alive[u] ? J.slots[h[u]] : kEmpty;
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				if (!C.alive[u])
+					continue;
+				u64 n = 0;
+				while (s[u] != kEmpty && (u32)s[u] != off[u] && n++ < J.mask) {
+					h[u] = (h[u] + 1) & J.mask;
+					s[u] = J.slots[h[u]];
+				}
+				if (s[u] == kEmpty || (u32)s[u] != off[u])
+					C.alive[u] = false;
+				else
+					C.pay[j][u] = (u32)(s[u] >> 32);
+			}
+			if (!J.last && J.variant == VH_JOINN && (dups_mask >> j) & 1u) {
+				// a first-match join may not fan out: the host reruns elsewhere
+#pragma unroll
+				for (int u = 0; u < kU; u++)
+					if (C.alive[u])
+						atomicOr(P.error, 16u);
+			}
+			if (J.last) {
+#pragma unroll
+				for (int u = 0; u < kU; u++) {
+					h_last[u] = h[u];
+					off_last[u] = off[u];
+				}
+			}
+		}
+	}
+	consume_u<MODE>(P, L, C);
+
+	// JoinN: further matches of the last join follow the first one in the
+	// probe sequence; each is aggregated on its own
+	if (P.njoins > 0) {
+		const int last = P.njoins - 1;
+		const JoinT& J = P.joins[last];
+		if (J.variant == VH_JOINN) {
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				if (!C.alive[u])
+					continue;
+				u64 h = (h_last[u] + 1) & J.mask;
+				for (u64 n = 0; n < J.mask; n++) {
+					const u64 s = J.slots[h];
+					if (s == kEmpty)
+						break;
+					if ((u32)s == off_last[u]) {
+						Chunk one = C;
+#pragma unroll
+						for (int w = 0; w < kU; w++)
+							one.alive[w] = w == u;
+						switch (last) {
+						case 0: one.pay[0][u] = (u32)(s >> 32); break;
+						case 1: one.pay[1][u] = (u32)(s >> 32); break;
+						case 2: one.pay[2][u] = (u32)(s >> 32); break;
+						default: one.pay[3][u] = (u32)(s >> 32); break;
+						}
+						consume_u<MODE>(P, L, one);
+					}
+					h = (h + 1) & J.mask;
+				}
+			}
+		}
+	}
+}
+
+// Rows [r0, r1) in chunks of kU * blockDim.
+template <int MODE>
+__device__ __forceinline__ void process_range(const Plan& P, const Lds& L, i64 r0, i64 r1, i64 tile_stride,
+	u32 dups_mask)
+{
+	const i64 tile = (i64)kU * blockDim.x;
+	for (i64 base = r0; base < r1; base += tile_stride) {
+		Chunk C;
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			C.row[u] = base + (i64)u * blockDim.x + threadIdx.x;
+			C.alive[u] = C.row[u] < r1 && C.row[u] < base + tile;
+		}
+		process_chunk<MODE>(P, L, C, dups_mask);
+	}
+}
+
+// LDS layout of each mode (sizes in pipeline_lds_bytes).
+template <int MODE>
+__device__ __forceinline__ Lds lds_layout(const Plan& P, u64* lds)
+{
+	Lds L = {nullptr, nullptr, nullptr, nullptr};
+	if (MODE == GB_REG) {
+		const u64 n = P.domain * P.nsums * blockDim.x;
+		L.lo = lds;
+		L.hi = (int*)(lds + n);
+		L.cnt = (u32*)(L.hi + n);
+	} else if (MODE == GB_LDS) {
+		const u64 n = P.domain * P.nsums;
+		L.lo = lds;
+		L.hi = (int*)(lds + n);
+		L.cnt = (u32*)(L.hi + n);
+	} else if (MODE == GB_LDSHASH) {
+		const u64 n = (u64)P.nsums * kLdsHashSlots;
+		L.lo = lds;
+		L.hi = (int*)(lds + n);
+		L.cnt = (u32*)(L.hi + n);
+		L.keys = L.cnt + kLdsHashSlots;
+	}
+	return L;
+}
+
+template <int MODE>
+__device__ __forceinline__ void lds_init(const Plan& P, const Lds& L)
+{
+	u64 n = 0, groups = 0;
+	if (MODE == GB_REG) { n = P.domain * P.nsums * blockDim.x; groups = P.domain * blockDim.x; }
+	else if (MODE == GB_LDS) { n = P.domain * P.nsums; groups = P.domain; }
+	else if (MODE == GB_LDSHASH) { n = (u64)P.nsums * kLdsHashSlots; groups = kLdsHashSlots; }
+	for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
+		L.lo[i] = 0;
+		L.hi[i] = 0;
+	}
+	for (u64 i = threadIdx.x; i < groups; i += blockDim.x) {
+		L.cnt[i] = 0;
+		if (MODE == GB_LDSHASH)
+			L.keys[i] = 0xFFFFFFFFu;
+	}
+}
+
+// GB_LDSHASH: add this block's LDS groups to the global arrays.
+__device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
+{
+	for (int i = threadIdx.x; i < kLdsHashSlots; i += blockDim.x) {
+		const u32 key = L.keys[i];
+		if (key == 0xFFFFFFFFu)
+			continue;
+		atomicAdd(&P.out_cnt[key], (u64)L.cnt[i]);
+		for (int s = 0; s < P.nsums; s++) {
+			const SumArg& S = P.sums[s];
+			const u64 lo = L.lo[s * kLdsHashSlots + i];
+			const i64 hi = (P.wide && S.wide) ? (i64)L.hi[s * kLdsHashSlots + i]
+				: (S.arg_signed ? (i64)lo >> 63 : 0);
+			const u64 idx = (u64)s * P.domain + key;
+			if (S.res128)
+				atomic_add_i128(&P.out_lo[idx], &P.out_hi[idx], lo, hi);
+			else
+				atomicAdd(&P.out_lo[idx], lo);
+		}
+	}
+}
+
+// GB_LDS in the batched kernel: add a work item's dense LDS groups to the
+// global arrays.
+__device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
+{
+	const u64 D = P.domain;
+	for (u64 g = threadIdx.x; g < D; g += blockDim.x) {
+		const u32 c = L.cnt[g];
+		if (!c)
+			continue;
+		atomicAdd(&P.out_cnt[g], (u64)c);
+		for (int s = 0; s < P.nsums; s++) {
+			const SumArg& S = P.sums[s];
+			const u64 lo = L.lo[s * D + g];
+			const i64 hi = (P.wide && S.wide) ? (i64)L.hi[s * D + g] : (S.arg_signed ? (i64)lo >> 63 : 0);
+			if (S.res128)
+				atomic_add_i128(&P.out_lo[s * D + g], &P.out_hi[s * D + g], lo, hi);
+			else
+				atomicAdd(&P.out_lo[s * D + g], lo);
+		}
+	}
+}
+
+// --------------------------------------------------------------------------
+// build side
+// --------------------------------------------------------------------------
+
+// Build side of one row: select, then insert (key - kmin, payload - pmin)
+// with CAS into linear-probing slots.  Replaces write_pos/write +
+// bucket_build (relalg_translator.cpp:828-928, runtime_struct.cpp:1283-1324).
+__device__ __forceinline__ void build_row(const BuildPlan& B, i64 row)
+{
+	if (!preds_pass_row(B.preds, B.npreds, row))
+		return;
+	const i64 key = load1(B.key, B.key_type, row);
+	if (key < B.kmin || key > B.kmax) {
+		atomicOr(B.error, 4u);
+		return;
+	}
+	const u32 off = (u32)(key - B.kmin);
+	const u32 pay = B.npay ? (u32)(load1(B.pay, B.pay_type, row) - B.pmin) : 0u;
+	const u64 val = (u64)off | ((u64)pay << 32);
+	u64 h = voila_hash((u64)off) & B.mask;
+	for (u64 n = 0; n <= B.mask; n++) {
+		const u64 old = atomicCAS(&B.slots[h], kEmpty, val);
+		if (old == kEmpty)
+			break;
+		if ((u32)old == off)
+			*B.dups = 1u;
+		h = (h + 1) & B.mask;
+	}
+}
+
+// Build side of one chunk (U rows per lane): the first CAS of every row is
+// issued before any collision is resolved, so U atomics are in flight.
+__device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 end)
+{
+	i64 row[kU];
+	bool ok[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		row[u] = base + (i64)u * blockDim.x + threadIdx.x;
+		ok[u] = row[u] < end;
+	}
+#pragma unroll
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < B.npreds) {
+			const Pred& Q = B.preds[p];
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				ok[u] = ok[u] && compare(Q.cmp, Q.r.type, load1(Q.r.ptr, Q.r.type, row[u]), Q.value);
+		}
+	}
+	u64 val[kU], h[kU], old[kU];
+	u32 off[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		const i64 key = ok[u] ? load1(B.key, B.key_type, row[u]) : B.kmin;
+		const u32 pay = (ok[u] && B.npay) ? (u32)(load1(B.pay, B.pay_type, row[u]) - B.pmin) : 0u;
+		if (ok[u] && (key < B.kmin || key > B.kmax)) {
+			atomicOr(B.error, 4u);
+			ok[u] = false;
+		}
+		off[u] = (u32)(key - B.kmin);
+		val[u] = (u64)off[u] | ((u64)pay << 32);
+		h[u] = voila_hash((u64)off[u]) & B.mask;
+	}
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		old[u] = ok[u] ? atomicCAS(&B.slots[h[u]], kEmpty, val[u]) : kEmpty;
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		if (!ok[u])
+			continue;
+		for (u64 n = 0; old[u] != kEmpty && n < B.mask; n++) {
+			if ((u32)old[u] == off[u])
+				*B.dups = 1u;
+			h[u] = (h[u] + 1) & B.mask;
+			old[u] = atomicCAS(&B.slots[h[u]], kEmpty, val[u]);
+		}
+	}
+}
+
+} // namespace vh

@@ -10,7 +10,7 @@ namespace vh {
 hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s);
 hipError_t launch_build(const BuildPlan* dB, int grid, hipStream_t s);
 // P: host copy (launch shape); dP: the same plan in device memory
-hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, bool wide, hipStream_t s);
+hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 size_t pipeline_lds_bytes(const Plan& P);
 
 } // namespace vh

@@ -94,7 +94,7 @@ struct Plan {
 	int njoins;
 	int nkeys;
 	int nsums;
-	int vec4;            // all probe columns 16-byte aligned
+	int wide;            // some sum needs exact 128-bit bookkeeping (see SumArg)
 	int mode;            // enum GroupMode
 	Pred preds[VH_MAX_PREDS];
 	JoinT joins[VH_MAX_JOINS];

@@ -155,3 +155,40 @@ def gen_star(fact_rows=60_000_000, zipf_part=0.0, seed=42, dims_scale=1.0):
         "f_cost": uniform_int(seed, 36, fact_rows, 10_000, 1_000_000, np.int64),
     }
     return fact, date, customer, supplier, part
+
+
+def batch_params(count=256, seed=42, probe_range=(1e4, 1e7), build_range=(1e3, 2e6),
+                 group_range=(1, 1e6)):
+    """Parameters of `count` uneven select -> join -> group-by pipelines:
+    probe / build rows and group counts log-uniform, selectivity uniform in
+    [0.01, 0.99], key skew theta from {0, 0.5, 1.0, 1.5}."""
+    u = [unit_double(seed, 41 + k, count) for k in range(5)]
+
+    def logu(x, lo, hi):
+        return np.exp(np.log(lo) + x * (np.log(hi) - np.log(lo)))
+    out = []
+    thetas = [0.0, 0.5, 1.0, 1.5]
+    for i in range(count):
+        out.append({
+            "probe_rows": int(logu(u[0][i], *probe_range)),
+            "build_rows": int(logu(u[1][i], *build_range)),
+            "selectivity": 0.01 + 0.98 * float(u[2][i]),
+            "groups": max(1, int(logu(u[3][i], *group_range))),
+            "theta": thetas[int(u[4][i] * 4) % 4],
+        })
+    return out
+
+
+def gen_batch_pipeline(params: dict, seed=42, index=0):
+    """Inputs of one batched pipeline: build (key unique 1..m, w U[0,100)),
+    probe (fk Zipf(theta) over [1, 2m] so about half the probes match under
+    no skew, a U[0,1000) for the selection, g U[0,G), v int64)."""
+    m, n = params["build_rows"], params["probe_rows"]
+    s = seed * 1000 + index
+    build = {"key": (permutation(s, 51, m) + 1).astype(np.int32),
+             "w": uniform_int(s, 52, m, 0, 100, np.int32)}
+    probe = {"fk": zipf_ranks(s, 53, n, 2 * m, params["theta"]).astype(np.int32),
+             "a": uniform_int(s, 54, n, 0, 1000, np.int32),
+             "g": uniform_int(s, 55, n, 0, params["groups"], np.int32),
+             "v": uniform_int(s, 56, n, -10**9, 10**12, np.int64)}
+    return build, probe

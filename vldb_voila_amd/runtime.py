@@ -152,11 +152,12 @@ class Event:
             pass
 
 
-def make_config(group_mode=-1, grid=0, table_shift=0):
+def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0):
     c = abi.vh_config()
     c.group_mode = group_mode
     c.grid = grid
     c.table_shift = table_shift
+    c.tile_shift = tile_shift
     return c
 
 
@@ -217,3 +218,62 @@ def run_pipeline(pipeline_builder, config=None) -> str:
         return q.result()
     finally:
         q.free()
+
+
+class Batch:
+    """vh_batch: many pipelines in one persistent launch (optionally replayed
+    from a captured hipGraph)."""
+
+    def __init__(self, descs, configs=None, grid=0):
+        init()
+        L = lib()
+        L.vh_batch_new.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
+        L.vh_batch_run.argtypes = [C.c_void_p, C.c_void_p]
+        L.vh_batch_run_graph.argtypes = [C.c_void_p, C.c_void_p]
+        L.vh_batch_result.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_char_p, C.c_size_t,
+                                      C.POINTER(C.c_size_t), C.POINTER(C.c_int64)]
+        L.vh_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int64)]
+        L.vh_batch_free.argtypes = [C.c_void_p]
+        n = len(descs)
+        arr = (abi.vh_pipeline_desc * n)(*descs)
+        cfg = None
+        if configs is not None:
+            cfg = (abi.vh_config * n)(*configs)
+        self._keep = (arr, cfg)
+        p = C.c_void_p()
+        check(L.vh_batch_new(C.byref(p), n, C.addressof(arr), C.addressof(cfg) if cfg is not None else None,
+                             grid))
+        self.ptr = p.value
+        self.n = n
+
+    def run(self, stream: Stream | None = None):
+        check(lib().vh_batch_run(self.ptr, stream.ptr if stream else None))
+
+    def run_graph(self, stream: Stream):
+        check(lib().vh_batch_run_graph(self.ptr, stream.ptr))
+
+    def result(self, i, stream: Stream | None = None) -> str:
+        n = C.c_size_t(0)
+        rows = C.c_int64(0)
+        s = stream.ptr if stream else None
+        check(lib().vh_batch_result(self.ptr, i, s, None, 0, C.byref(n), C.byref(rows)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().vh_batch_result(self.ptr, i, s, buf, n.value + 1, C.byref(n), C.byref(rows)))
+        return buf.value.decode()
+
+    def info(self):
+        it, g, l = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().vh_batch_info(self.ptr, C.byref(it), C.byref(g), C.byref(l)))
+        return {"items": it.value, "grid": g.value, "lds_bytes": l.value}
+
+    def free(self):
+        if self.ptr:
+            lib().vh_batch_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -174,14 +174,13 @@ def groupby_statements(keys, values, n):
 
 def _selunion(L, a, an, b, bn, out, n, cnt):
     """ComplexFuncs::selunion (runtime.cpp:67-169) as flags over positions:
-    mark both lists in a u8 vector, compact it (ascending, no duplicates)."""
+    clear a u8 vector, mark both lists, compact it (ascending, no
+    duplicates)."""
     flags = Vec(n, np.uint8)
-    zero = DeviceBuffer.from_numpy(np.zeros(max(n, 1), dtype=np.uint8))
-    check(lib().vh_memcpy_h2d(flags.ptr, np.zeros(max(n, 1), dtype=np.uint8).ctypes.data, max(n, 1)))
     one = DeviceBuffer.from_numpy(np.ones(max(n, 1), dtype=np.uint8))
-    # flags[i] = 1 for i in a and in b: a map over the selection with constant 1
+    # flags[i] = (1 != 1) = 0 everywhere, then 1 at the positions of a and b
+    check(L.vh_vec_cmp(abi.VH_NE, abi.VH_U8, None, n, flags.ptr, one.ptr, None, 1, None))
     for s, sn in ((a, an), (b, bn)):
         check(L.vh_vec_cmp(abi.VH_EQ, abi.VH_U8, s.ptr, sn, flags.ptr, one.ptr, None, 1, None))
     check(L.vh_vec_seltrue(None, n, out.ptr, flags.ptr, cnt.ptr, None))
-    del zero
     return _count(cnt)

@@ -57,3 +57,16 @@ def star_pipeline(fact, date, customer, supplier, part, addrs=None, region=STAR_
            names=[])
     p.groupby(["c_nation", "s_nation", "d_year"], [("sum", col("f_revenue") - col("f_cost")), ("count",)])
     return p
+
+
+def batch_pipeline(params, build, probe, build_addrs=None, probe_addrs=None) -> Pipeline:
+    """select a < selectivity * 1000; join fk = key (JoinN); group by g:
+    sum(v), sum(w), count."""
+    cut = int(round(params["selectivity"] * 1000))
+    r = table_from_arrays("build", build, build_addrs)
+    s = table_from_arrays("probe", probe, probe_addrs)
+    p = Pipeline(s)
+    p.select("a", "<", cut)
+    p.join(r, "key", "fk", payload=["w"], variant=abi.VH_JOINN, names=["w"])
+    p.groupby(["g"], [("sum", col("v")), ("sum", col("w")), ("count",)])
+    return p
