@@ -156,8 +156,9 @@ enum vh_group_mode {
 	VH_GROUP_LDS = 1,      /* dense groups in a per-block LDS table           */
 	VH_GROUP_GLOBAL = 2,   /* dense groups, global atomics                    */
 	VH_GROUP_HASH = 3,     /* packed key, global CAS open addressing          */
-	VH_GROUP_LDSHASH = 4   /* per-block LDS hash table over the dense index,
+	VH_GROUP_LDSHASH = 4,  /* per-block LDS hash table over the dense index,
 	                          global atomics when it is full                  */
+	VH_GROUP_DENSE = 5     /* VH_GROUP_REG when it fits, else VH_GROUP_LDS    */
 };
 
 typedef struct vh_config {
@@ -300,6 +301,63 @@ int vh_batch_result(vh_batch* b, int32_t i, void* stream, char* buf, size_t cap,
 	int64_t* nrows);
 int vh_batch_info(const vh_batch* b, int32_t* items, int32_t* grid, int64_t* lds_bytes);
 int vh_batch_free(vh_batch* b);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuner: exhaustive timing of the variant space.  The reference's      */
+/* explorer compiles and times every flavor and keeps the fastest            */
+/* (explorer.cpp:112-427, best pick in SQL at print_base_flavors.py:32-40). */
+/* ------------------------------------------------------------------------ */
+
+int vh_variant_count(void);                        /* 48 */
+int vh_variant_config(int32_t variant, vh_config* out);
+/* ms[i]: fastest of `reps` timed runs of cfgs[i] (full query run on
+ * `stream`), or -1 when the variant does not apply to the pipeline. */
+int vh_tune_time(const vh_pipeline_desc* desc, const vh_config* cfgs, int32_t n, int32_t reps,
+	float* ms, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned selector: FP32 MLP 6 -> 64 -> 64 -> 48 trained on the GPU.        */
+/* No counterpart in the reference; it replaces the exhaustive search as     */
+/* the first guess and falls back to it when its confidence is low.          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vh_mlp vh_mlp;
+int vh_mlp_new(vh_mlp** m, int32_t batch_capacity, const float* init_params /* 7,728 floats */);
+int vh_mlp_free(vh_mlp* m);
+int vh_mlp_params(vh_mlp* m, float* host_params);
+/* x [B][6], mask [B][48] (1 = variant applies, NULL = all) and prob_out
+ * [B][48] are device buffers. */
+int vh_mlp_predict(vh_mlp* m, const float* x, const uint8_t* mask, int32_t B, float* prob_out,
+	void* stream);
+/* Forward + softmax cross-entropy on labels y [B] + backward; Adam update
+ * when lr > 0.  grads_out (host, 7,728 floats) and loss may be NULL. */
+int vh_mlp_train_step(vh_mlp* m, const float* x, const uint8_t* mask, const int32_t* y, int32_t B,
+	float lr, float* grads_out, float* loss, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Relational front end ("Flavor::Hip").  Plans are the reference's          */
+/* relalg operators (relalg.hpp:12-181) in S-expression form:               */
+/*   (hashaggr hash|global (keys k...) (aggrs (sum E) (count) ...) PLAN)    */
+/*   (hashjoin join01|joinn LEFT (keys k) (payl c...) RIGHT (keys k) (payl c...)) */
+/*   (select PRED PLAN)  (project (assign name E)... PLAN)  (scan table c...) */
+/* E := column | integer | (+|-|* E E);  PRED := (<|<=|>|>=|=|!= col int)    */
+/* | (and PRED PRED).  Columns are named as scanned ("table.col" or "col"). */
+/* The catalog mirrors runtime::Database (Database.hpp:28-200): per column  */
+/* a type, device data and min/max (Import.cpp:175-186).                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct vh_db vh_db;
+int vh_db_new(vh_db** db);
+int vh_db_free(vh_db* db);
+int vh_db_add_column(vh_db* db, const char* table, const char* column, int32_t type,
+	const void* device_data, int64_t rows, int64_t min, int64_t max);
+/* Lower a plan to a pipeline descriptor (VH_ERR_UNSUPPORTED for shapes the
+ * fused path does not cover). */
+int vh_plan_lower(vh_db* db, const char* plan, vh_pipeline_desc* out);
+/* vh_query_new on the lowered plan (the caller then runs it as any query). */
+int vh_plan_query_new(vh_query** q, vh_db* db, const char* plan, const vh_config* cfg);
+/* The VOILA statement sequences the plan becomes and how they are fused. */
+int vh_plan_explain(vh_db* db, const char* plan, char* buf, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,88 @@
+"""Learned selector kernels against the float64 CPU oracle: forward,
+gradients (analytic) and the Adam update; training lowers the loss; the
+autotuner's timing entry point returns one time per applicable variant."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp as omlp
+from vldb_voila_amd import abi, datagen, runtime, selector, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    print("gpu selector: init device 0", flush=True)
+    runtime.init(0)
+
+
+def batch(seed, B=64, masked=True):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(B, 6)).astype(np.float32)
+    mask = (rng.random((B, 48)) < 0.7).astype(np.uint8) if masked else np.ones((B, 48), np.uint8)
+    y = np.array([rng.choice(np.nonzero(m)[0]) for m in mask], dtype=np.int32)
+    return x, y, mask
+
+
+def test_forward_matches_oracle():
+    print("mlp forward", flush=True)
+    p0 = omlp.init_params(5)
+    net = selector.MLP(p0, 128)
+    x, y, mask = batch(1, 128)
+    got = net.predict(x, mask)
+    want, _ = omlp.forward(p0, x, mask)
+    assert np.allclose(got, want, atol=2e-6, rtol=1e-4)
+
+
+def test_gradients_match_oracle():
+    print("mlp gradients", flush=True)
+    p0 = omlp.init_params(6)
+    net = selector.MLP(p0, 256)
+    x, y, mask = batch(2, 200)
+    loss, g = net.train_step(x, y, mask, lr=0.0, want_grads=True)
+    wl, wg = omlp.loss_and_grad(p0, x, y, mask)
+    assert abs(loss - wl) <= 1e-5 + 1e-5 * abs(wl)
+    # float32 with a fixed summation order vs float64: tolerance 1e-4 relative
+    assert np.allclose(g, wg, atol=2e-6, rtol=1e-3), np.abs(g - wg).max()
+
+
+def test_adam_step_matches_oracle():
+    print("mlp adam", flush=True)
+    p0 = omlp.init_params(7)
+    net = selector.MLP(p0, 64)
+    x, y, mask = batch(3, 64)
+    _, g = net.train_step(x, y, mask, lr=1e-2, want_grads=True)
+    want, _, _ = omlp.adam(p0.astype(np.float64), g.astype(np.float64), np.zeros(omlp.NPARAMS),
+                           np.zeros(omlp.NPARAMS), 1, 1e-2)
+    assert np.allclose(net.params(), want, atol=1e-6, rtol=1e-5)
+
+
+def test_training_lowers_loss():
+    print("mlp training", flush=True)
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(512, 6)).astype(np.float32)
+    # a learnable rule: label from the sign pattern of two features
+    y = ((X[:, 0] > 0) * 2 + (X[:, 3] > 0)).astype(np.int32) * 7
+    times = np.ones((512, 48), dtype=np.float32) * 2
+    times[np.arange(512), y] = 1
+    net, mu, sd, losses = selector.train(X, times, epochs=40, lr=3e-3, batch=128)
+    assert losses[-1] < 0.3 * losses[0]
+    p = net.predict((X - mu) / sd)
+    assert (p.argmax(1) == y).mean() > 0.9
+
+
+def test_tune_time_reports_each_variant():
+    print("tune timing", flush=True)
+    b, p = datagen.gen_join_aggregate(20_000, 80_000, seed=3)
+    db, dp = runtime.upload(b), runtime.upload(p)
+    d = workloads.join_aggregate_pipeline(b, p, runtime.addrs(db), runtime.addrs(dp)).desc()
+    ms = selector.time_variants(d, reps=1)
+    assert ms.shape == (48,)
+    assert (ms > 0).sum() >= 36          # dense LDS, LDS hash, global and hash all apply here
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, p).desc()))
+    best = int(np.where(ms > 0, ms, np.inf).argmin())
+    cfg = selector.variant_configs()[best]
+    got = oracle.sorted_rows(runtime.run_pipeline(
+        workloads.join_aggregate_pipeline(b, p, runtime.addrs(db), runtime.addrs(dp)), cfg))
+    assert got == want

@@ -79,6 +79,7 @@ class vh_pipeline_desc(C.Structure):
 
 
 VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH, VH_GROUP_LDSHASH = -1, 0, 1, 2, 3, 4
+VH_GROUP_DENSE = 5
 
 
 class vh_config(C.Structure):

@@ -8,8 +8,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoila_hip.so")
-SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_batch.hip", "vh_engine.cpp", "vh_capi.cpp",
-           "vh_capi_prims.cpp", "vh_batch_host.cpp"]
+SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_batch.hip", "vh_mlp.hip", "vh_engine.cpp",
+           "vh_capi.cpp", "vh_capi_prims.cpp", "vh_batch_host.cpp", "vh_capi_tune.cpp", "vh_relalg.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

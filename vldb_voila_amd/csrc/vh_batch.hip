@@ -1,23 +1,22 @@
-// Persistent batched execution of many independent pipelines in one launch.
+// Persistent batched execution of many independent pipelines.
 //
 // The reference runs pipelines strictly one after the other
 // (runtime_framework.cpp:172-198), each with its own flavor
 // (QueryConfig::pipeline_default_blend, runtime_framework.hpp:78).  Here the
-// build and probe work of every pipeline is cut into row ranges ("items")
-// placed in one queue, build items first.  Each workgroup takes the next
-// item with an atomic counter until the queue is empty.  A probe item waits
-// until all build items of its pipeline have finished; those items were
-// taken earlier by running workgroups that never wait, so the wait ends.
-// Group-by state lives in LDS per item and is added to the pipeline's global
-// result arrays when the item ends.
+// build and probe work of every pipeline is cut into row ranges ("items").
+// One persistent launch works through all build items, a second one through
+// all probe items; each workgroup takes the next item from an atomic counter
+// until its queue is empty.  No workgroup ever waits for another: the second
+// launch starts after the first has finished, which orders every build
+// before every probe.  Group-by state lives in LDS per item and is added to
+// the pipeline's global result arrays when the item ends.  Both launches and
+// the clearing pass are captured into one hipGraph by vh_batch_run_graph.
 #include "vh_batch.hpp"
 #include "vh_device.hpp"
 #include "vh_fused.hpp"
 #include "vh_plan.hpp"
 
 namespace vh {
-
-static constexpr u64 kSpinLimit = 1ull << 28;
 
 template <int MODE>
 __device__ __forceinline__ void probe_item_m(const Plan& P, i64 r0, i64 r1, u64* lds, u32 dups_mask)
@@ -48,46 +47,27 @@ __device__ void probe_item(const Plan& P, i64 r0, i64 r1, u64* lds)
 	}
 }
 
+// Items [first, first + count) of the queue; ctl[0] is this launch's counter.
 __global__ __launch_bounds__(256) void batch_kernel(const Plan* __restrict__ plans,
-	const BuildPlan* __restrict__ builds, const BatchItem* __restrict__ items, u32 nitems,
-	u32* __restrict__ ctl)
+	const BuildPlan* __restrict__ builds, const BatchItem* __restrict__ items, u32 first, u32 count,
+	u32* __restrict__ counter)
 {
 	extern __shared__ u64 lds[];
 	__shared__ u32 item_idx;
-	u32* next_item = ctl;            // ctl[0]: queue head
-	u32* error = ctl + 1;            // ctl[1]: wait limit hit
-	u32* build_done = ctl + 4;       // ctl[4 + p]: finished build items of pipeline p
 	for (;;) {
 		if (threadIdx.x == 0)
-			item_idx = atomicAdd(next_item, 1u);
+			item_idx = atomicAdd(counter, 1u);
 		__syncthreads();
-		const u32 it = item_idx;
+		const u32 k = item_idx;
 		__syncthreads();
-		if (it >= nitems)
+		if (k >= count)
 			break;
-		const BatchItem I = items[it];
+		const BatchItem I = items[first + k];
 		if (I.phase == 0) {
 			const BuildPlan& B = builds[I.build];
 			for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
 				build_chunk(B, base, I.r1);
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-			__syncthreads();
-			if (threadIdx.x == 0)
-				__hip_atomic_fetch_add(&build_done[I.plan], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 		} else {
-			if (threadIdx.x == 0) {
-				u64 spins = 0;
-				while (__hip_atomic_load(&build_done[I.plan], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
-					(u32)I.need) {
-					__builtin_amdgcn_s_sleep(2);
-					if (++spins > kSpinLimit) {
-						atomicOr(error, 1u);
-						break;
-					}
-				}
-			}
-			__syncthreads();
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 			probe_item(plans[I.plan], I.r0, I.r1, lds);
 		}
 	}
@@ -113,11 +93,13 @@ hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s)
 	return hipGetLastError();
 }
 
-hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 nitems,
-	u32* d_ctl, int grid, size_t lds_bytes, hipStream_t s)
+hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 first,
+	u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s)
 {
-	hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, d_plans, d_builds, d_items, nitems,
-		d_ctl);
+	if (!count)
+		return hipSuccess;
+	hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, d_plans, d_builds, d_items, first, count,
+		d_counter);
 	return hipGetLastError();
 }
 

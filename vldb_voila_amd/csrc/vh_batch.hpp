@@ -11,7 +11,7 @@ struct BatchItem {
 	int plan;            // pipeline index
 	int phase;           // 0 = build rows of builds[build], 1 = probe rows of plans[plan]
 	int build;           // index into the build plan array (phase 0)
-	int need;            // build items of this pipeline (phase 1 waits for them)
+	int reserved;
 	i64 r0;
 	i64 r1;
 };
@@ -23,7 +23,8 @@ struct FillSeg {
 };
 
 hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s);
-hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 nitems,
-	u32* d_ctl, int grid, size_t lds_bytes, hipStream_t s);
+// Items [first, first + count) with the launch's own queue counter.
+hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 first,
+	u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s);
 
 } // namespace vh

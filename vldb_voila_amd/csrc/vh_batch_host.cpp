@@ -1,7 +1,7 @@
-// Host side of the persistent batched launch: lowers every pipeline like a
-// single query, lays out the work queue (all build items, then all probe
-// items) and enqueues one multi-buffer clear plus one persistent kernel, or
-// replays both from a captured hipGraph.
+// Host side of the batched launch: lowers every pipeline like a single
+// query, lays out the work queue (all build items, then all probe items) and
+// enqueues one multi-buffer clear plus two persistent launches (builds, then
+// probes), or replays the three from a captured hipGraph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +32,7 @@ struct vh_batch {
 	FillSeg* d_segs = nullptr;
 	u32* d_ctl = nullptr;
 	u32 nitems = 0;
+	u32 nbuild = 0;
 	int nsegs = 0;
 	int grid = 0;
 	size_t lds = 0;
@@ -55,7 +56,7 @@ struct vh_batch {
 		if (e != hipSuccess)
 			return fail(VH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
 		allocs.push_back(*dst);
-		if (bytes && (e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess)
+		if (src && bytes && (e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice)) != hipSuccess)
 			return fail(VH_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
 		return VH_OK;
 	}
@@ -67,6 +68,18 @@ static int tile_rows_of(const vh_config& c)
 	if (shift < -6) shift = -6;
 	if (shift > 6) shift = 6;
 	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
+}
+
+// clear, then every build item, then every probe item
+static hipError_t enqueue(vh_batch* b, hipStream_t s)
+{
+	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
+	if (e == hipSuccess)
+		e = launch_batch(b->d_plans, b->d_builds, b->d_items, 0, b->nbuild, b->d_ctl, b->grid, 0, s);
+	if (e == hipSuccess)
+		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild, b->nitems - b->nbuild, b->d_ctl + 1,
+			b->grid, b->lds, s);
+	return e;
 }
 
 extern "C" {
@@ -112,17 +125,14 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		b->lds = std::max(b->lds, pipeline_lds_bytes(P));
 		const int T = tile_rows_of(q->cfg);
 		const int first_build = (int)builds.size();
-		int need = 0;
 		for (int j = 0; j < P.njoins; j++) {
 			builds.push_back(q->L.builds[j]);
-			for (i64 r0 = 0; r0 < q->L.builds[j].rows; r0 += T) {
+			for (i64 r0 = 0; r0 < q->L.builds[j].rows; r0 += T)
 				build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, q->L.builds[j].rows)});
-				need++;
-			}
 			segs.push_back({q->slots[j], q->L.capacity[j], ~0ull});
 		}
 		for (i64 r0 = 0; r0 < P.rows; r0 += T)
-			probe_items.push_back({i, 1, -1, need, r0, std::min<i64>(r0 + T, P.rows)});
+			probe_items.push_back({i, 1, -1, 0, r0, std::min<i64>(r0 + T, P.rows)});
 		const u64 groups = q->L.groups;
 		const u64 ns = (u64)std::max(P.nsums, 1);
 		segs.push_back({(u64*)q->flags, 4, 0});
@@ -140,7 +150,8 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	std::vector<BatchItem> items = build_items;
 	items.insert(items.end(), probe_items.begin(), probe_items.end());
 	b->nitems = (u32)items.size();
-	const size_t ctl_words = (4 + (size_t)n + 1) / 2 + 1;
+	b->nbuild = (u32)build_items.size();
+	const size_t ctl_words = 2;   // two queue counters (u32), one per launch
 	if ((rc = b->upload((void**)&b->d_ctl, nullptr, ctl_words * 8)) ||
 		(rc = b->upload((void**)&b->d_plans, plans.data(), plans.size() * sizeof(Plan))) ||
 		(rc = b->upload((void**)&b->d_builds, builds.data(), builds.size() * sizeof(BuildPlan))) ||
@@ -167,8 +178,7 @@ int vh_batch_run(vh_batch* b, void* stream)
 	if (!b)
 		return fail(VH_ERR_INVALID, "null batch");
 	hipStream_t s = (hipStream_t)stream;
-	HIP_CALL(launch_multi_fill(b->d_segs, b->nsegs, s));
-	HIP_CALL(launch_batch(b->d_plans, b->d_builds, b->d_items, b->nitems, b->d_ctl, b->grid, b->lds, s));
+	HIP_CALL(enqueue(b, s));
 	for (Query* q : b->q)
 		q->ran = true;
 	return VH_OK;
@@ -183,11 +193,10 @@ int vh_batch_run_graph(vh_batch* b, void* stream)
 	hipStream_t s = (hipStream_t)stream;
 	if (!b->graph) {
 		HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-		hipError_t e1 = launch_multi_fill(b->d_segs, b->nsegs, s);
-		hipError_t e2 = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nitems, b->d_ctl, b->grid, b->lds, s);
+		hipError_t e1 = enqueue(b, s);
 		hipGraph_t g = nullptr;
 		HIP_CALL(hipStreamEndCapture(s, &g));
-		if (e1 != hipSuccess || e2 != hipSuccess) {
+		if (e1 != hipSuccess) {
 			(void)hipGraphDestroy(g);
 			return fail(VH_ERR_HIP, "capture of the batch failed");
 		}
@@ -205,10 +214,6 @@ int vh_batch_result(vh_batch* b, int32_t i, void* stream, char* buf, size_t cap,
 	if (!b || i < 0 || i >= (int)b->q.size() || !len)
 		return fail(VH_ERR_INVALID, "vh_batch_result arguments");
 	HIP_CALL(hipStreamSynchronize((hipStream_t)stream));
-	u32 ctl[2];
-	HIP_CALL(hipMemcpy(ctl, b->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
-	if (ctl[1])
-		return fail(VH_ERR_STATE, "batched run: a probe item gave up waiting for its build");
 	int64_t rows = 0;
 	Query* q = b->q[i];
 	int rc = query_result(q, (hipStream_t)stream, q->result, rows);

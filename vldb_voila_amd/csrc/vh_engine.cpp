@@ -436,6 +436,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	int mode = cfg ? cfg->group_mode : -1;
 	const size_t lds_budget = 64 * 1024;
 	const double lds_need = domain * P.nsums * 12 + domain * 4 + 16;
+	const bool reg_fits = domain <= kRegSlots && P.nsums <= kRegSums &&
+		domain * P.nsums * 256 * 12 + domain * 256 * 4 <= 64 * 1024;
+	if (mode == VH_GROUP_DENSE)   // dense LDS state: lane-private if it fits, else shared
+		mode = reg_fits ? GB_REG : GB_LDS;
 	if (mode < 0) {
 		// built-in heuristic; the autotuner and the learned selector override it
 		if (domain <= kRegSlots && P.nsums <= kRegSums &&
@@ -644,6 +648,8 @@ int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
 		return fail(VH_ERR_CAPACITY, "group hash table full");
 	if (flags[0] & 8u)
 		return fail(VH_ERR_INVALID, "a group key lies outside its column's [min, max]");
+	if (flags[0] & 32u)
+		return fail(VH_ERR_CAPACITY, "a hash table probe sequence exceeded its bound (damaged or full table)");
 	if (flags[0] & 16u)
 		return fail(VH_ERR_UNSUPPORTED, "a JoinN before the last join matched duplicate build keys");
 	for (int j = 0; j < P.njoins - 1; j++)

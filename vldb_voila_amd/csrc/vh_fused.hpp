@@ -18,6 +18,10 @@ namespace vh {
 
 static constexpr u64 kEmpty = ~0ull;
 static constexpr int kU = 4;              // rows per lane per chunk
+// Longest linear-probe sequence a kernel follows.  Tables stay below 70% load
+// (calc_num_buckets), where sequences are a few slots long; a longer one
+// means a damaged table and is reported (error bit 32) instead of followed.
+static constexpr u64 kMaxProbe = 1ull << 16;
 
 // --------------------------------------------------------------------------
 // per-row helpers (build side and tails)
@@ -167,7 +171,7 @@ __device__ __forceinline__ i64 locate(const Plan& P, const Lds& L, u64 g)
 	}
 	if (MODE == GB_HASH) {
 		u64 h = voila_hash(g) & P.hmask;
-		for (u64 n = 0; n <= P.hmask; n++) {
+		for (u64 n = 0; n <= P.hmask && n < kMaxProbe; n++) {
 			const u64 k = P.hkeys[h];
 			if (k == g)
 				return (i64)h;
@@ -352,10 +356,13 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				if (!C.alive[u])
 					continue;
 				u64 n = 0;
-				while (s[u] != kEmpty && (u32)s[u] != off[u] && n++ < J.mask) {
+				while (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {
+					n++;
 					h[u] = (h[u] + 1) & J.mask;
 					s[u] = J.slots[h[u]];
 				}
+				if (n >= kMaxProbe)
+					atomicOr(P.error, 32u);
 				if (s[u] == kEmpty || (u32)s[u] != off[u])
 					C.alive[u] = false;
 				else
@@ -390,7 +397,11 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				if (!C.alive[u])
 					continue;
 				u64 h = (h_last[u] + 1) & J.mask;
-				for (u64 n = 0; n < J.mask; n++) {
+				for (u64 n = 0;; n++) {
+					if (n >= kMaxProbe) {
+						atomicOr(P.error, 32u);
+						break;
+					}
 					const u64 s = J.slots[h];
 					if (s == kEmpty)
 						break;
@@ -539,6 +550,10 @@ __device__ __forceinline__ void build_row(const BuildPlan& B, i64 row)
 	const u64 val = (u64)off | ((u64)pay << 32);
 	u64 h = voila_hash((u64)off) & B.mask;
 	for (u64 n = 0; n <= B.mask; n++) {
+		if (n >= kMaxProbe) {
+			atomicOr(B.error, 32u);
+			break;
+		}
 		const u64 old = atomicCAS(&B.slots[h], kEmpty, val);
 		if (old == kEmpty)
 			break;
@@ -589,7 +604,11 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 	for (int u = 0; u < kU; u++) {
 		if (!ok[u])
 			continue;
-		for (u64 n = 0; old[u] != kEmpty && n < B.mask; n++) {
+		for (u64 n = 0; old[u] != kEmpty; n++) {
+			if (n >= kMaxProbe || n >= B.mask) {
+				atomicOr(B.error, 32u);
+				break;
+			}
 			if ((u32)old[u] == off[u])
 				*B.dups = 1u;
 			h[u] = (h[u] + 1) & B.mask;

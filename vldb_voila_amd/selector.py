@@ -1,0 +1,177 @@
+"""Learned flavor selector and the exhaustive autotuner it falls back to.
+
+The reference picks flavors by exploring them all (explorer.cpp:112-427)
+and choosing the fastest offline (print_base_flavors.py:32-40).  Here the
+autotuner times the 48 kernel variants of a pipeline on the GPU
+(vh_tune_time), and a small MLP trained on those timings (vh_mlp_*: forward,
+backward and Adam are HIP kernels) predicts the fastest variant from six
+pipeline features.  Predictions below 0.5 confidence fall back to the
+exhaustive search.  Only the choice of kernel changes: results stay
+bit-exact.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .runtime import DeviceBuffer, check, init, lib
+
+N_VARIANTS = 48
+N_FEATURES = 6
+N_PARAMS = 7728
+CONFIDENCE = 0.5
+
+
+def _bind():
+    L = lib()
+    L.vh_variant_config.argtypes = [C.c_int32, C.c_void_p]
+    L.vh_tune_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+    L.vh_mlp_new.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_void_p]
+    L.vh_mlp_free.argtypes = [C.c_void_p]
+    L.vh_mlp_params.argtypes = [C.c_void_p, C.c_void_p]
+    L.vh_mlp_predict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    L.vh_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
+                                    C.c_void_p, C.POINTER(C.c_float), C.c_void_p]
+    return L
+
+
+def variant_configs():
+    L = _bind()
+    out = []
+    for v in range(N_VARIANTS):
+        c = abi.vh_config()
+        check(L.vh_variant_config(v, C.byref(c)))
+        out.append(c)
+    return out
+
+
+def time_variants(desc, reps=3, stream=None, configs=None) -> np.ndarray:
+    """Fastest run (ms) of each variant; -1 where the variant does not apply."""
+    init()
+    L = _bind()
+    cfgs = configs or variant_configs()
+    arr = (abi.vh_config * len(cfgs))(*cfgs)
+    ms = np.zeros(len(cfgs), dtype=np.float32)
+    check(L.vh_tune_time(C.byref(desc), C.addressof(arr), len(cfgs), reps, ms.ctypes.data,
+                         stream.ptr if stream else None))
+    return ms
+
+
+def features(params: dict, key_width=4) -> np.ndarray:
+    """log10 probe rows, log10 build rows, selectivity, log10 groups, theta,
+    key width (bytes)."""
+    return np.array([math.log10(max(params["probe_rows"], 1)), math.log10(max(params["build_rows"], 1)),
+                     params["selectivity"], math.log10(max(params["groups"], 1)), params["theta"],
+                     key_width], dtype=np.float32)
+
+
+class MLP:
+    """vh_mlp: the selector network on the GPU."""
+
+    def __init__(self, init_params: np.ndarray, batch_capacity=1024):
+        init()
+        self.L = _bind()
+        p = C.c_void_p()
+        ip = np.ascontiguousarray(init_params, dtype=np.float32)
+        assert ip.size == N_PARAMS
+        check(self.L.vh_mlp_new(C.byref(p), batch_capacity, ip.ctypes.data))
+        self.ptr = p.value
+        self.cap = batch_capacity
+
+    def params(self) -> np.ndarray:
+        out = np.empty(N_PARAMS, dtype=np.float32)
+        check(self.L.vh_mlp_params(self.ptr, out.ctypes.data))
+        return out
+
+    def predict(self, x: np.ndarray, mask: np.ndarray | None = None) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B = x.shape[0]
+        dx = DeviceBuffer.from_numpy(x)
+        dm = DeviceBuffer.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)) if mask is not None else None
+        dp = DeviceBuffer(B * N_VARIANTS * 4)
+        check(self.L.vh_mlp_predict(self.ptr, dx.ptr, dm.ptr if dm else None, B, dp.ptr, None))
+        return dp.to_numpy(np.float32, B * N_VARIANTS).reshape(B, N_VARIANTS)
+
+    def train_step(self, x, y, mask=None, lr=0.0, want_grads=False):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        B = x.shape[0]
+        dx, dy = DeviceBuffer.from_numpy(x), DeviceBuffer.from_numpy(y)
+        dm = DeviceBuffer.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)) if mask is not None else None
+        grads = np.empty(N_PARAMS, dtype=np.float32) if want_grads else None
+        loss = C.c_float()
+        check(self.L.vh_mlp_train_step(self.ptr, dx.ptr, dm.ptr if dm else None, dy.ptr, B, lr,
+                                       grads.ctypes.data if grads is not None else None, C.byref(loss), None))
+        return loss.value, grads
+
+    def free(self):
+        if self.ptr:
+            self.L.vh_mlp_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def normalise(X: np.ndarray):
+    """Feature scaling (per-feature mean / std of the training set)."""
+    mu = X.mean(0)
+    sd = X.std(0)
+    sd[sd == 0] = 1
+    return mu.astype(np.float32), sd.astype(np.float32)
+
+
+def train(X, times, epochs=200, lr=3e-3, batch=256, seed=0, init_params=None):
+    """Train on the explorer's timing table: label = fastest applicable
+    variant, mask = applicable variants.  Returns (MLP, mu, sd, losses)."""
+    X = np.asarray(X, dtype=np.float32)
+    times = np.asarray(times, dtype=np.float32)
+    mask = (times > 0).astype(np.uint8)
+    y = np.where(times > 0, times, np.inf).argmin(1).astype(np.int32)
+    mu, sd = normalise(X)
+    Xn = (X - mu) / sd
+    if init_params is None:
+        from_seed = np.random.default_rng(seed)
+        init_params = _he_init(from_seed)
+    net = MLP(init_params, batch_capacity=max(batch, 1))
+    rng = np.random.default_rng(seed + 1)
+    losses = []
+    n = len(Xn)
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        tot = 0.0
+        for s in range(0, n, batch):
+            idx = order[s:s + batch]
+            l, _ = net.train_step(Xn[idx], y[idx], mask[idx], lr=lr)
+            tot += l * len(idx)
+        losses.append(tot / n)
+    return net, mu, sd, losses
+
+
+def _he_init(rng):
+    parts = []
+    for fan_in, shape in ((6, (6, 64)), (None, (64,)), (64, (64, 64)), (None, (64,)), (64, (64, 48)),
+                          (None, (48,))):
+        if fan_in is None:
+            parts.append(np.zeros(shape))
+        else:
+            lim = math.sqrt(6.0 / fan_in)
+            parts.append(rng.uniform(-lim, lim, shape))
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+def choose(net, mu, sd, feats, mask, fallback):
+    """Selector decision: predicted variant when its probability >= 0.5,
+    else fallback() (the exhaustive autotuner).  Returns (variant, used_fallback)."""
+    p = net.predict(((np.asarray(feats, dtype=np.float32) - mu) / sd)[None, :],
+                    np.asarray(mask, dtype=np.uint8)[None, :])[0]
+    v = int(p.argmax())
+    if p[v] >= CONFIDENCE and mask[v]:
+        return v, False
+    return int(fallback()), True
