@@ -87,13 +87,21 @@ class MLP:
         return out
 
     def predict(self, x: np.ndarray, mask: np.ndarray | None = None) -> np.ndarray:
+        """Probabilities (B, 48); rows go through in chunks of the network's
+        batch capacity."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         B = x.shape[0]
-        dx = DeviceBuffer.from_numpy(x)
-        dm = DeviceBuffer.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)) if mask is not None else None
-        dp = DeviceBuffer(B * N_VARIANTS * 4)
-        check(self.L.vh_mlp_predict(self.ptr, dx.ptr, dm.ptr if dm else None, B, dp.ptr, None))
-        return dp.to_numpy(np.float32, B * N_VARIANTS).reshape(B, N_VARIANTS)
+        out = np.empty((B, N_VARIANTS), dtype=np.float32)
+        for s in range(0, B, self.cap):
+            xs = np.ascontiguousarray(x[s:s + self.cap])
+            b = xs.shape[0]
+            dx = DeviceBuffer.from_numpy(xs)
+            dm = (DeviceBuffer.from_numpy(np.ascontiguousarray(mask[s:s + self.cap], dtype=np.uint8))
+                  if mask is not None else None)
+            dp = DeviceBuffer(b * N_VARIANTS * 4)
+            check(self.L.vh_mlp_predict(self.ptr, dx.ptr, dm.ptr if dm else None, b, dp.ptr, None))
+            out[s:s + b] = dp.to_numpy(np.float32, b * N_VARIANTS).reshape(b, N_VARIANTS)
+        return out
 
     def train_step(self, x, y, mask=None, lr=0.0, want_grads=False):
         x = np.ascontiguousarray(x, dtype=np.float32)

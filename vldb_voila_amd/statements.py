@@ -184,3 +184,72 @@ def _selunion(L, a, an, b, bn, out, n, cnt):
         check(L.vh_vec_cmp(abi.VH_EQ, abi.VH_U8, s.ptr, sn, flags.ptr, one.ptr, None, 1, None))
     check(L.vh_vec_seltrue(None, n, out.ptr, flags.ptr, cnt.ptr, None))
     return _count(cnt)
+
+
+def join_statements(build_key, build_pays, nb, probe_key, n, variant):
+    """HashJoin of relalg_translator.cpp:828-1111 as statements.
+
+    Build (pipelines A+B, :828-928): hash the build keys, write_pos + write
+    the key, payload and hash columns, bucket_build.  Probe (pipeline C,
+    :933-1111): bucket_lookup, then while tuples are active: check the key,
+    gather the payloads of the matches, drop the matched tuples when the join
+    is Join01 (first match only), bucket_next, keep the non-empty buckets.
+
+    build_key / probe_key: (vh_type, device ptr); build_pays: list of
+    (vh_type, device ptr).  Returns (positions, [payload arrays]): one entry
+    per (probe position, matching build row), int64 payloads."""
+    L = _bind()
+    npay = len(build_pays)
+    t = Table(1 + npay, [0] * (1 + npay), max(nb, 1))
+    bhash = Vec(nb, np.uint64)
+    cnt = DeviceBuffer(16)
+    if nb:
+        check(L.vh_vec_hash(build_key[0], None, nb, bhash.ptr, build_key[1], None))
+        base = C.c_int64()
+        check(L.vh_vec_write_pos(t.ptr, nb, C.byref(base)))
+        check(L.vh_vec_write(t.ptr, 0, base.value, None, nb, build_key[0], build_key[1], None))
+        for j, (pt, pp) in enumerate(build_pays):
+            check(L.vh_vec_write(t.ptr, 1 + j, base.value, None, nb, pt, pp, None))
+        check(L.vh_vec_write(t.ptr, -1, base.value, None, nb, abi.VH_U64, bhash.ptr, None))
+    check(L.vh_vec_bucket_build(t.ptr, None))
+
+    phash = Vec(n, np.uint64)
+    bucket = Vec(n, np.uint64)
+    eq, nz = Vec(n, np.uint8), Vec(n, np.uint8)
+    active, active2, match = (Vec(n, np.int32) for _ in range(3))
+    gathered = [Vec(n, np.int64) for _ in range(npay)]
+    positions, pays = [], [[] for _ in range(npay)]
+    check(L.vh_vec_hash(probe_key[0], None, n, phash.ptr, probe_key[1], None))
+    check(L.vh_vec_bucket_lookup(t.ptr, None, n, bucket.ptr, phash.ptr, None))
+    check(L.vh_vec_cmp(abi.VH_NE, abi.VH_U64, None, n, nz.ptr, bucket.ptr, None, 0, None))
+    check(L.vh_vec_seltrue(None, n, active.ptr, nz.ptr, cnt.ptr, None))
+    act_n = _count(cnt)
+    rounds = 0
+    while act_n > 0:
+        rounds += 1
+        check(L.vh_vec_check(t.ptr, 0, active.ptr, act_n, eq.ptr, bucket.ptr, probe_key[0], probe_key[1], None))
+        check(L.vh_vec_seltrue(active.ptr, act_n, match.ptr, eq.ptr, cnt.ptr, None))
+        m_n = _count(cnt)
+        if m_n:
+            for j in range(npay):
+                check(L.vh_vec_gather(t.ptr, 1 + j, match.ptr, m_n, gathered[j].ptr, abi.VH_I64, bucket.ptr, None))
+            pos = match.buf.to_numpy(np.int32, m_n)
+            positions.append(pos)
+            for j in range(npay):
+                pays[j].append(gathered[j].buf.to_numpy(np.int64, n)[pos])
+        if variant == abi.VH_JOIN01:
+            # matched tuples are done: active = selfalse(eq) | active
+            check(L.vh_vec_selfalse(active.ptr, act_n, active2.ptr, eq.ptr, cnt.ptr, None))
+            active, active2 = active2, active
+            act_n = _count(cnt)
+        check(L.vh_vec_bucket_next(t.ptr, active.ptr, act_n, bucket.ptr, bucket.ptr, None))
+        check(L.vh_vec_cmp(abi.VH_NE, abi.VH_U64, active.ptr, act_n, nz.ptr, bucket.ptr, None, 0, None))
+        check(L.vh_vec_seltrue(active.ptr, act_n, active2.ptr, nz.ptr, cnt.ptr, None))
+        active, active2 = active2, active
+        act_n = _count(cnt)
+        if rounds > max(nb, 1) + 1:
+            raise RuntimeError("join probe loop does not terminate")
+    t.free()
+    if not positions:
+        return np.zeros(0, np.int32), [np.zeros(0, np.int64) for _ in range(npay)]
+    return np.concatenate(positions), [np.concatenate(p) for p in pays]
