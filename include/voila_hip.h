@@ -16,8 +16,10 @@
 #ifndef VOILA_HIP_H
 #define VOILA_HIP_H
 
+#ifndef __HIPCC_RTC__   /* hipRTC provides these itself */
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -161,12 +163,39 @@ enum vh_group_mode {
 	VH_GROUP_DENSE = 5     /* VH_GROUP_REG when it fits, else VH_GROUP_LDS    */
 };
 
+/* Physical layout of a join table.  Both give the same matches. */
+enum vh_join_layout {
+	VH_LAYOUT_AUTO = 0,    /* direct when the key range is dense (range <=
+	                          4 x build rows + 65536), else hash            */
+	VH_LAYOUT_HASH = 1,    /* open addressing, 64-bit slots (key | payload)   */
+	VH_LAYOUT_DIRECT = 2   /* array indexed by key - min holding the payload
+	                          in 1, 2 or 4 bytes; repeated keys spill to a
+	                          hash table; falls back to hash above 2^30 keys */
+};
+
+/* Which probe kernel runs a query.  The generated kernel is HIP source
+ * written for the pipeline (column types, operator sequence, join layouts
+ * and group-by strategy fixed; pointers and constants read from the plan),
+ * compiled with hipRTC for the device at vh_query_new and cached per
+ * source text; the generic kernel interprets the same plan.  Results are
+ * identical. */
+enum vh_kernel {
+	VH_KERNEL_AUTO = 0,      /* generated, or generic if hipRTC fails       */
+	VH_KERNEL_GENERIC = 1,
+	VH_KERNEL_JIT = 2,       /* generated; vh_query_new fails without it    */
+	VH_KERNEL_JIT_LAZY = 3   /* generated, columns after the predicates are
+	                            loaded only for qualifying rows             */
+};
+
 typedef struct vh_config {
 	int32_t group_mode;    /* enum vh_group_mode                               */
 	int32_t grid;          /* probe-kernel workgroups, 0 = auto                */
 	int32_t table_shift;   /* join table size vs calc_num_buckets: <<n / >>-n  */
 	int32_t tile_shift;    /* batched work item = 65536 << n rows (n < 0: >>)  */
-	int32_t reserved[4];
+	int32_t join_layout;   /* enum vh_join_layout                              */
+	int32_t kernel;        /* enum vh_kernel (single queries; batches run the
+	                          generic kernel)                                  */
+	int32_t reserved[2];
 } vh_config;
 
 /* ------------------------------------------------------------------------ */
@@ -218,6 +247,16 @@ int vh_query_reset(vh_query* q);
 int vh_query_free(vh_query* q);
 /* Mode the lowering chose (enum vh_group_mode) and the probe-kernel grid. */
 int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t* groups);
+/* Probe kernel in use (VH_KERNEL_GENERIC or VH_KERNEL_JIT) and the generated
+ * HIP source (empty for the generic kernel), NUL-terminated into buf when
+ * cap > *len; *len = bytes excluding the NUL.  The reference likewise
+ * generates C++ per query and compiles it with g++ -O3 -shared
+ * (codegen.cpp:211-395, compiler.cpp:19-55). */
+int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, size_t* len);
+/* The generated kernel source for a pipeline without creating a query (no
+ * device needed): lowering only, same buffer convention. */
+int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap,
+	size_t* len);
 
 /* Type the reference's TypingPass gives [min, max] (typing_pass.cpp:9-22). */
 int vh_type_from_minmax(double min, double max);

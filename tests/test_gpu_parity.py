@@ -29,24 +29,46 @@ def device():
     runtime.init(0)
 
 
+@pytest.mark.parametrize("kernel", [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT])
 @pytest.mark.parametrize("mode", [abi.VH_GROUP_AUTO, abi.VH_GROUP_REG] + MODES[1:])
-def test_filter_groupby_small(mode):
-    print(f"filter_groupby 300k mode={mode}", flush=True)
+def test_filter_groupby_small(mode, kernel):
+    print(f"filter_groupby 300k mode={mode} kernel={kernel}", flush=True)
     t = datagen.gen_filter_groupby(300_000, seed=11)
     want = oracle.sorted_rows(oracle.bruteforce(workloads.filter_groupby_pipeline(t).desc()))
     p, devs = on_gpu([t], lambda a: workloads.filter_groupby_pipeline(t, a))
-    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode, kernel=kernel)))
     assert got == want
 
 
+KERNELS = [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("mode", MODES)
-def test_join_aggregate_small(mode):
-    print(f"join_aggregate 50k/200k mode={mode}", flush=True)
+def test_join_aggregate_small(mode, kernel):
+    print(f"join_aggregate 50k/200k mode={mode} kernel={kernel}", flush=True)
     b, pr = datagen.gen_join_aggregate(50_000, 200_000, seed=13)
     want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
     p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
-    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
+    got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode, kernel=kernel)))
     assert got == want
+
+
+def test_generated_kernel_is_the_default():
+    """vh_query_new compiles the generated kernel; its source names the
+    plan's choices (direct byte table for the dense join)."""
+    print("generated kernel in use", flush=True)
+    b, pr = datagen.gen_join_aggregate(5_000, 20_000, seed=2)
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    q = runtime.Query(p.desc())
+    kernel, src = q.kernel()
+    assert kernel == abi.VH_KERNEL_JIT
+    assert "vh_generated" in src and "(const uint8_t*)J.dir" in src
+    q.run()
+    assert oracle.sorted_rows(q.result()) == oracle.sorted_rows(
+        oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    g = runtime.Query(p.desc(), runtime.make_config(kernel=abi.VH_KERNEL_GENERIC))
+    assert g.kernel() == (abi.VH_KERNEL_GENERIC, "")
 
 
 def test_filter_groupby_full_size():
@@ -205,3 +227,65 @@ def test_key_outside_declared_range_is_an_error():
     p.probe.columns[0].vmax = 3   # lie about max
     with pytest.raises(runtime.VoilaHipError):
         runtime.run_pipeline(p)
+
+
+LAYOUTS = [abi.VH_LAYOUT_HASH, abi.VH_LAYOUT_DIRECT, abi.VH_LAYOUT_AUTO]
+
+
+@pytest.mark.parametrize("prange", [200, 60_000, 3_000_000_000])
+def test_join_layouts_payload_widths(prange):
+    """Direct tables hold the payload in 1, 2 or 4 bytes (by its range);
+    repeated keys spill into the overflow slots.  Every layout and group-by
+    mode gives the oracle's rows, for JoinN with duplicates and Join01."""
+    print(f"join layouts payload range {prange}", flush=True)
+    rng = np.random.default_rng(prange % 1000 + 1)
+    nb, n = 8000, 60_000
+    for variant in (abi.VH_JOINN, abi.VH_JOIN01):
+        if variant == abi.VH_JOINN:
+            bk = rng.integers(10, 5000, nb).astype(np.int32)           # repeated keys
+        else:
+            bk = (rng.permutation(nb) * 3 + 7).astype(np.int32)       # unique, sparse
+        build = {"k": bk, "p": rng.integers(0, prange, nb).astype(np.int64)}
+        probe = {"fk": rng.integers(0, 3 * nb + 20, n).astype(np.int32),
+                 "v": rng.integers(-1000, 1000, n).astype(np.int64)}
+
+        def make(bb, pp):
+            p = Pipeline(table_from_arrays("s", probe, pp))
+            p.join(table_from_arrays("r", build, bb), "k", "fk", payload=["p"], variant=variant, names=["p"])
+            p.groupby(["p"] if prange < 100_000 else [], [("sum", col("v")), ("count",)])
+            return p
+        want = oracle.sorted_rows(oracle.bruteforce(make(None, None).desc()))
+        p, devs = on_gpu([build, probe], make)
+        for layout in LAYOUTS:
+            for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH):
+                got = oracle.sorted_rows(runtime.run_pipeline(
+                    p, runtime.make_config(group_mode=mode, join_layout=layout)))
+                assert got == want, (variant, layout, mode)
+
+
+def test_direct_overflow_cleared_between_runs():
+    """One query, build data rewritten in place between runs: repeated keys,
+    then unique keys, then repeated keys again.  The overflow slots a run
+    filled must not leak matches into the next one."""
+    print("direct layout overflow reuse", flush=True)
+    rng = np.random.default_rng(21)
+    nb, n = 4000, 40_000
+    dup_keys = rng.integers(0, 1000, nb).astype(np.int32)
+    uniq_keys = rng.permutation(nb).astype(np.int32)
+    dup_keys[0], dup_keys[1] = 0, nb - 1            # same [min, max] as the unique keys
+    build = {"k": dup_keys.copy(), "p": rng.integers(0, 50, nb).astype(np.int32)}
+    probe = {"fk": rng.integers(0, nb, n).astype(np.int32), "v": rng.integers(0, 100, n).astype(np.int64)}
+    bdev, pdev = runtime.upload(build), runtime.upload(probe)
+
+    def make(keys, bb=None, pp=None):
+        b = dict(build, k=keys)
+        p = Pipeline(table_from_arrays("s", probe, pp))
+        p.join(table_from_arrays("r", b, bb), "k", "fk", payload=["p"], variant=abi.VH_JOINN, names=["p"])
+        p.groupby(["p"], [("sum", col("v")), ("count",)])
+        return p
+    q = runtime.Query(make(dup_keys, runtime.addrs(bdev), runtime.addrs(pdev)).desc(),
+                      runtime.make_config(join_layout=abi.VH_LAYOUT_DIRECT))
+    for keys in (dup_keys, uniq_keys, dup_keys, uniq_keys):
+        bdev["k"].copy_from(keys)
+        q.run()
+        assert oracle.sorted_rows(q.result()) == oracle.sorted_rows(oracle.bruteforce(make(keys).desc()))

@@ -81,7 +81,19 @@ class vh_pipeline_desc(C.Structure):
 VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH, VH_GROUP_LDSHASH = -1, 0, 1, 2, 3, 4
 VH_GROUP_DENSE = 5
 
+# enum vh_join_layout
+VH_LAYOUT_AUTO = 0
+VH_LAYOUT_HASH = 1
+VH_LAYOUT_DIRECT = 2
+
+# enum vh_kernel
+VH_KERNEL_AUTO = 0
+VH_KERNEL_GENERIC = 1
+VH_KERNEL_JIT = 2
+VH_KERNEL_JIT_LAZY = 3
+
 
 class vh_config(C.Structure):
     _fields_ = [("group_mode", C.c_int32), ("grid", C.c_int32), ("table_shift", C.c_int32),
-                ("tile_shift", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("tile_shift", C.c_int32), ("join_layout", C.c_int32), ("kernel", C.c_int32),
+                ("reserved", C.c_int32 * 2)]

@@ -9,13 +9,44 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoila_hip.so")
 SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_batch.hip", "vh_mlp.hip", "vh_engine.cpp",
-           "vh_capi.cpp", "vh_capi_prims.cpp", "vh_batch_host.cpp", "vh_capi_tune.cpp", "vh_relalg.cpp"]
+           "vh_capi.cpp", "vh_capi_prims.cpp", "vh_batch_host.cpp", "vh_capi_tune.cpp", "vh_relalg.cpp",
+           "vh_jit.cpp"]
+# Headers the generated kernels include, embedded in the library as text
+# (name as #included, path): hipRTC compiles against these copies.
+JIT_HEADERS = [("vh_fused.hpp", os.path.join(CSRC, "vh_fused.hpp")),
+               ("vh_device.hpp", os.path.join(CSRC, "vh_device.hpp")),
+               ("vh_plan.hpp", os.path.join(CSRC, "vh_plan.hpp")),
+               ("../../include/voila_hip.h", os.path.join(HERE, "..", "include", "voila_hip.h"))]
+JIT_INC = os.path.join(CSRC, "vh_jit_headers.inc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+def write_jit_headers():
+    """vh_jit_headers.inc: the device headers as raw string literals."""
+    delim = "VHJIT"
+    parts = ["// generated by build.py: device headers for hipRTC\n",
+             f"static const int kJitHeaders = {len(JIT_HEADERS)};\n",
+             "static const char* const kJitHeaderNames[] = {" +
+             ", ".join(f'"{n}"' for n, _ in JIT_HEADERS) + "};\n",
+             "static const char* const kJitHeaderText[] = {\n"]
+    for _, path in JIT_HEADERS:
+        with open(path) as f:
+            text = f.read()
+        if f"){delim}\"" in text:
+            raise RuntimeError(f"{path} contains the raw-string delimiter")
+        parts.append(f'R"{delim}({text}){delim}",\n')
+    parts.append("};\n")
+    data = "".join(parts)
+    old = open(JIT_INC).read() if os.path.exists(JIT_INC) else None
+    if old != data:
+        with open(JIT_INC, "w") as f:
+            f.write(data)
+
+
 def build(verbose=False):
+    write_jit_headers()
     objs, procs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
@@ -32,7 +63,8 @@ def build(verbose=False):
     if failed:
         raise RuntimeError(f"hipcc failed on {failed}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         sys.stderr.write(r.stdout + r.stderr)

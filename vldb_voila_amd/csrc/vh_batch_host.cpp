@@ -118,7 +118,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		Plan& P = q->L.plan;
 		if (P.mode == GB_REG)
 			P.mode = GB_LDS;            // register state cannot outlive an item
-		if ((rc = query_create(q))) {
+		if ((rc = query_create(q, false))) {
 			fail(rc, "pipeline %d: %s", i, last_error());
 			break;
 		}
@@ -130,6 +130,8 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 			for (i64 r0 = 0; r0 < q->L.builds[j].rows; r0 += T)
 				build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, q->L.builds[j].rows)});
 			segs.push_back({q->slots[j], q->L.capacity[j], ~0ull});
+			if (q->dir[j])
+				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
 		}
 		for (i64 r0 = 0; r0 < P.rows; r0 += T)
 			probe_items.push_back({i, 1, -1, 0, r0, std::min<i64>(r0 + T, P.rows)});

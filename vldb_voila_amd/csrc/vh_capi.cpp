@@ -6,6 +6,7 @@
 
 #include "../../include/voila_hip.h"
 #include "vh_engine.hpp"
+#include "vh_jit.hpp"
 
 using namespace vh;
 
@@ -198,6 +199,43 @@ int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t
 		*grid = q->q.L.grid;
 	if (groups)
 		*groups = (int64_t)q->q.L.groups;
+	return VH_OK;
+}
+
+int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap, size_t* len)
+{
+	if (!desc || !len)
+		return fail(VH_ERR_INVALID, "vh_generate_source arguments");
+	Lowered L;
+	int rc = lower(desc, cfg, L);
+	if (rc)
+		return rc;
+	const std::string src = jit_source(L.plan, !cfg || cfg->kernel != VH_KERNEL_JIT_LAZY, kJitUnroll);
+	*len = src.size();
+	if (buf) {
+		if (cap <= src.size())
+			return fail(VH_ERR_CAPACITY, "source needs %zu bytes", src.size() + 1);
+		memcpy(buf, src.data(), src.size());
+		buf[src.size()] = 0;
+	}
+	return VH_OK;
+}
+
+int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, size_t* len)
+{
+	if (!q)
+		return fail(VH_ERR_INVALID, "null query");
+	const std::string& src = q->q.jit_src;
+	if (kernel)
+		*kernel = q->q.jit_fn ? VH_KERNEL_JIT : VH_KERNEL_GENERIC;
+	if (len)
+		*len = src.size();
+	if (buf) {
+		if (cap <= src.size())
+			return fail(VH_ERR_CAPACITY, "source needs %zu bytes", src.size() + 1);
+		memcpy(buf, src.data(), src.size());
+		buf[src.size()] = 0;
+	}
 	return VH_OK;
 }
 

@@ -20,12 +20,14 @@ using namespace vh;
 			return fail(VH_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e));           \
 	} while (0)
 
-// Variant index -> knobs: 4 group-by strategies x 3 join-table sizes x
+// Variant index -> knobs: 4 group-by strategies x 3 join-table layouts x
 // 4 probe grid sizes = 48 (the analog of the explorer's blend space,
 // explorer_helper.cpp:6-17).
 static const int kModes[4] = {VH_GROUP_DENSE, VH_GROUP_LDSHASH, VH_GROUP_GLOBAL, VH_GROUP_HASH};
-static const int kShifts[3] = {-1, 0, 1};
-static const int kGrids[4] = {256, 1024, 2048, 8192};
+// join table: {hash layout, hash layout at twice the slots, direct when dense}
+static const int kTableShift[3] = {0, 1, 0};
+static const int kTableLayout[3] = {VH_LAYOUT_HASH, VH_LAYOUT_HASH, VH_LAYOUT_AUTO};
+static const int kGrids[4] = {256, 512, 1024, 2048};
 
 struct vh_mlp {
 	float* P = nullptr;   // params
@@ -53,7 +55,8 @@ int vh_variant_config(int32_t v, vh_config* out)
 		return fail(VH_ERR_INVALID, "variant %d", v);
 	memset(out, 0, sizeof(*out));
 	out->group_mode = kModes[v / 12];
-	out->table_shift = kShifts[(v / 4) % 3];
+	out->table_shift = kTableShift[(v / 4) % 3];
+	out->join_layout = kTableLayout[(v / 4) % 3];
 	out->grid = kGrids[v % 4];
 	return VH_OK;
 }

@@ -2,8 +2,10 @@
 // VOILA's hash, 64-lane wave primitives and exact 128-bit accumulation.
 #pragma once
 
+#ifndef __HIPCC_RTC__   // hipRTC provides these itself
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "../../include/voila_hip.h"
 

@@ -13,6 +13,7 @@
 
 #include "../../include/voila_hip.h"
 #include "vh_engine.hpp"
+#include "vh_jit.hpp"
 #include "vh_kernels.hpp"
 
 namespace vh {
@@ -91,6 +92,21 @@ static int scan_type(int t, int64_t mn)
 static bool type_signed(int t)
 {
 	return t == VH_I8 || t == VH_I16 || t == VH_I32 || t == VH_I64 || t == VH_I128;
+}
+
+// Compute units of the current device (256 on MI355X), looked up once.
+static int device_cus()
+{
+	static int cus = 0;
+	if (!cus) {
+		int dev = 0, n = 0;
+		if (hipGetDevice(&dev) == hipSuccess &&
+			hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+			cus = n;
+		else
+			cus = 256;
+	}
+	return cus;
 }
 
 static const double kMaxCard = 1099511627776.0;   // QueryConfig::max_card = 2^40
@@ -313,6 +329,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	P.njoins = d->njoins;
 	L.builds.assign(d->njoins, BuildPlan{});
 	L.capacity.assign(d->njoins, 0);
+	L.dir_words.assign(d->njoins, 0);
+	const int layout = cfg ? cfg->join_layout : VH_LAYOUT_AUTO;
+	if (layout < VH_LAYOUT_AUTO || layout > VH_LAYOUT_DIRECT)
+		return fail(VH_ERR_INVALID, "join_layout %d", layout);
 	for (int j = 0; j < d->njoins; j++) {
 		const vh_join_desc& jd = d->joins[j];
 		BuildPlan& B = L.builds[j];
@@ -365,6 +385,17 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		L.capacity[j] = cap;
 		B.mask = cap - 1;
 
+		// direct layout: an entry per key of [kmin, kmax], payload - pmin in
+		// the fewest bytes that keep all-ones free as "no row"
+		const double krange = (double)kc.max - (double)kc.min + 1;
+		const double prange = jd.npayload
+			? (double)jd.build.cols[jd.payload[0]].max - (double)jd.build.cols[jd.payload[0]].min : 0;
+		const bool dense = krange <= 4.0 * (double)std::max<int64_t>(jd.build.rows, 0) + 65536.0;
+		if (layout != VH_LAYOUT_HASH && krange <= (double)(1u << 30) && (layout == VH_LAYOUT_DIRECT || dense)) {
+			B.dir_w = prange <= 254 ? 1 : (prange <= 65534 ? 2 : 4);
+			L.dir_words[j] = ((uint64_t)krange * (uint64_t)B.dir_w + 7) / 8;
+		}
+
 		JoinT& J = P.joins[j];
 		J.mask = cap - 1;
 		if ((rc = role_of_vcol(d, jd.probe_key, J.key)))
@@ -372,6 +403,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		J.kmin = kc.min;
 		J.kmax = kc.max;
 		J.pmin = B.pmin;
+		J.dir_w = B.dir_w;
 		J.npay = jd.npayload;
 		J.variant = jd.variant;
 		J.last = j == d->njoins - 1;
@@ -489,11 +521,12 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		L.wide |= P.sums[s].res128 != 0;
 	P.wide = L.wide ? 1 : 0;
 
-	// launch shape: enough blocks for 256 CUs, capped so partials stay small
+	// launch shape: one resident wave of workgroups (4 per CU at this
+	// kernel's occupancy), fewer when there are not 1024 rows per workgroup
 	int grid = cfg && cfg->grid > 0 ? cfg->grid : 0;
 	if (!grid) {
 		int64_t want = (P.rows + 1023) / 1024;
-		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 2048));
+		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4 * device_cus()));
 	}
 	L.grid = grid;
 	L.build_grid.assign(d->njoins, 0);
@@ -525,22 +558,30 @@ int Query::alloc(void** p, size_t bytes)
 	return VH_OK;
 }
 
-int query_create(Query* q)
+int query_create(Query* q, bool want_generated)
 {
 	Lowered& L = q->L;
 	Plan& P = L.plan;
 	int rc;
-	// flags: [0] error, [1 + j] dups of join j, [8] completion counter
+	// flags: [0] error, [1 + j] dups of join j
 	if ((rc = q->alloc((void**)&q->flags, 16 * sizeof(u32))))
 		return rc;
 	HIP_TRY(hipMemset(q->flags, 0, 16 * sizeof(u32)));
 	P.error = q->flags;
-	P.done = q->flags + 8;
 	for (int j = 0; j < P.njoins; j++) {
 		if ((rc = q->alloc((void**)&q->slots[j], L.capacity[j] * sizeof(u64))))
 			return rc;
+		// the run's clear pass refills the slots only after a run that used
+		// them, so they start out empty here
+		HIP_TRY(hipMemset(q->slots[j], 0xFF, L.capacity[j] * sizeof(u64)));
 		P.joins[j].slots = q->slots[j];
 		P.joins[j].dups = q->flags + 1 + j;
+		if (L.dir_words[j]) {
+			if ((rc = q->alloc((void**)&q->dir[j], L.dir_words[j] * sizeof(u64))))
+				return rc;
+			P.joins[j].dir = q->dir[j];
+			L.builds[j].dir = q->dir[j];
+		}
 		L.builds[j].slots = q->slots[j];
 		L.builds[j].dups = q->flags + 1 + j;
 		L.builds[j].error = q->flags;
@@ -571,6 +612,21 @@ int query_create(Query* q)
 			return rc;
 		HIP_TRY(hipMemcpy(q->dbuild[j], &L.builds[j], sizeof(BuildPlan), hipMemcpyHostToDevice));
 	}
+	// the generated probe kernel, unless the caller asked for the generic one
+	const int kernel = q->cfg.kernel;
+	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_LAZY)
+		return fail(VH_ERR_INVALID, "kernel %d", kernel);
+	if (want_generated && kernel != VH_KERNEL_GENERIC) {
+		q->jit_src = jit_source(P, kernel != VH_KERNEL_JIT_LAZY, kJitUnroll);
+		std::string log;
+		rc = jit_compile(q->jit_src, &q->jit_fn, log);
+		if (rc && kernel != VH_KERNEL_AUTO)
+			return rc;
+		if (rc) {   // AUTO: the generic kernel computes the same result
+			q->jit_fn = nullptr;
+			q->jit_src.clear();
+		}
+	}
 	return VH_OK;
 }
 
@@ -581,10 +637,17 @@ int query_run(Query* q, hipStream_t s, int phase)
 	if (phase < -1 || phase > 2)
 		return fail(VH_ERR_INVALID, "phase %d", phase);
 	if (phase == -1 || phase == 0) {
-	// clear: error/dup flags, join slots, atomically accumulated outputs
-	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
-	for (int j = 0; j < P.njoins; j++)
-		HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
+	// clear: join tables, atomically accumulated outputs, then the
+	// error/dup flags.  A direct table's overflow slots are refilled only
+	// when the last build spilled into them (its dups flag, still set here).
+	for (int j = 0; j < P.njoins; j++) {
+		if (q->dir[j]) {
+			HIP_TRY(launch_fill(q->dir[j], L.dir_words[j], ~0ull, s));
+			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s, q->flags + 1 + j));
+		} else {
+			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
+		}
+	}
 	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH) {
 		const u64 groups = L.groups;
 		const int nsums = std::max(P.nsums, 1);
@@ -594,6 +657,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 		if (P.mode == GB_HASH)
 			HIP_TRY(launch_fill(P.hkeys, groups, ~0ull, s));
 	}
+	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
 	}
 	if (phase == -1 || phase == 1) {
 		for (int j = 0; j < P.njoins; j++)
@@ -601,7 +665,10 @@ int query_run(Query* q, hipStream_t s, int phase)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
 	}
 	if (phase == -1 || phase == 2) {
-		HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
+		if (q->jit_fn)
+			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, s));
+		else
+			HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
 		q->ran = true;
 	}
 	return VH_OK;

@@ -29,6 +29,7 @@ struct Lowered {
 	Plan plan;
 	std::vector<BuildPlan> builds;
 	std::vector<uint64_t> capacity;      // join table slots
+	std::vector<uint64_t> dir_words;     // direct layout: 64-bit words of dir[] (0 = hash layout)
 	std::vector<int> build_grid;
 	std::vector<int> sum_of_aggr;        // aggregate -> sum index (-1: count)
 	uint64_t key_range[VH_MAX_KEYS] = {};
@@ -44,11 +45,14 @@ struct Query {
 	Lowered L;
 	u32* flags = nullptr;
 	u64* slots[VH_MAX_JOINS] = {};
+	u64* dir[VH_MAX_JOINS] = {};
 	Plan* dplan = nullptr;
 	BuildPlan* dbuild[VH_MAX_JOINS] = {};
 	std::vector<void*> allocs;
 	bool ran = false;
 	std::string result;
+	hipFunction_t jit_fn = nullptr;   // generated probe kernel (null: generic)
+	std::string jit_src;
 	~Query();
 	int alloc(void** p, size_t bytes);
 };
@@ -58,7 +62,8 @@ const char* last_error();
 int type_from_minmax(double mn, double mx);
 int type_pipeline(const vh_pipeline_desc* d, Typing& ty);
 int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
-int query_create(Query* q);
+// want_generated: compile the generated probe kernel (per cfg.kernel)
+int query_create(Query* q, bool want_generated = true);
 int query_run(Query* q, hipStream_t s, int phase);
 int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows);
 

@@ -24,22 +24,40 @@ static constexpr int kU = 4;              // rows per lane per chunk
 static constexpr u64 kMaxProbe = 1ull << 16;
 
 // --------------------------------------------------------------------------
-// per-row helpers (build side and tails)
+// direct join tables: dir[key - kmin] = payload - pmin in w = 1, 2 or 4 bytes
 // --------------------------------------------------------------------------
 
-__device__ __forceinline__ bool preds_pass_row(const Pred* preds, int npreds, i64 row)
+__device__ __forceinline__ u32 dir_empty(int w)
 {
-	bool ok = true;
-#pragma unroll
-	for (int p = 0; p < VH_MAX_PREDS; p++) {
-		if (p < npreds) {
-			const Pred& P = preds[p];
-			ok = ok && compare(P.cmp, P.r.type, load1(P.r.ptr, P.r.type, row), P.value);
-		}
-	}
-	return ok;
+	return w == 1 ? 0xFFu : (w == 2 ? 0xFFFFu : 0xFFFFFFFFu);
 }
 
+__device__ __forceinline__ u32 dir_load(const void* dir, int w, u32 off)
+{
+	if (w == 1)
+		return ((const uint8_t*)dir)[off];
+	if (w == 2)
+		return ((const uint16_t*)dir)[off];
+	return ((const u32*)dir)[off];
+}
+
+// The 32-bit word holding key offset `off`, and the bit mask of its entry.
+__device__ __forceinline__ u32* dir_word(void* dir, int w, u32 off, u32& mask, u32& shift)
+{
+	const u64 byte = (u64)off * (u64)w;
+	shift = (u32)(byte & 3) * 8;
+	mask = (w == 4 ? 0xFFFFFFFFu : ((1u << (8 * w)) - 1u)) << shift;
+	return (u32*)((char*)dir + (byte & ~3ull));
+}
+
+// Claiming an entry takes one atomic AND, no retry loop: entries start
+// all-ones and a payload is never all-ones, so AND-ing (payload in the entry,
+// ones elsewhere) into the word returns all-ones in the entry to exactly one
+// row, the first.  A later row with the same key sees a value other than
+// all-ones, knows it is a repeat, and ORs back the value it saw: its AND can
+// only have cleared bits of the first row's payload, and every bit cleared
+// while set is set again by the row whose AND cleared it, after that AND.
+// After the launch every entry holds its first row's payload exactly.
 // --------------------------------------------------------------------------
 // chunk state
 // --------------------------------------------------------------------------
@@ -348,6 +366,23 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				off[u] = (u32)(key[u] - J.kmin);
 				h[u] = voila_hash((u64)off[u]) & J.mask;
 			}
+			if (J.dir_w) {
+				// direct layout: one load per row; repeated keys (if any) are
+				// found by the JoinN tail in the overflow slots from h
+				u32 v[kU];
+				const u32 empty = dir_empty(J.dir_w);
+#pragma unroll
+				for (int u = 0; u < kU; u++)
+					v[u] = C.alive[u] ? dir_load(J.dir, J.dir_w, off[u]) : empty;
+#pragma unroll
+				for (int u = 0; u < kU; u++) {
+					if (v[u] == empty)
+						C.alive[u] = false;
+					else
+						C.pay[j][u] = v[u];
+					h[u] = (h[u] - 1) & J.mask;
+				}
+			} else {
 #pragma unroll
 			for (int u = 0; u < kU; u++)
 				s[u] = C.alive[u] ? J.slots[h[u]] : kEmpty;
@@ -368,6 +403,7 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				else
 					C.pay[j][u] = (u32)(s[u] >> 32);
 			}
+			}
 			if (!J.last && J.variant == VH_JOINN && (dups_mask >> j) & 1u) {
 				// a first-match join may not fan out: the host reruns elsewhere
 #pragma unroll
@@ -387,11 +423,13 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 	consume_u<MODE>(P, L, C);
 
 	// JoinN: further matches of the last join follow the first one in the
-	// probe sequence; each is aggregated on its own
+	// probe sequence (hash layout) or sit in the overflow slots from
+	// voila_hash(key) on (direct layout); each is aggregated on its own.
+	// Without a repeated build key there are none.
 	if (P.njoins > 0) {
 		const int last = P.njoins - 1;
 		const JoinT& J = P.joins[last];
-		if (J.variant == VH_JOINN) {
+		if (J.variant == VH_JOINN && ((dups_mask >> last) & 1u)) {
 #pragma unroll
 			for (int u = 0; u < kU; u++) {
 				if (!C.alive[u])
@@ -507,6 +545,75 @@ __device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
 	}
 }
 
+// End of a one-pipeline launch: GB_LDSHASH adds its LDS groups to the
+// global arrays; GB_REG / GB_LDS write the block's exact partials
+// (element-major, see merge_kernel) for the merge launch that follows.
+template <int MODE>
+__device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
+{
+	if (MODE == GB_LDSHASH) {
+		__syncthreads();
+		ldshash_flush(P, L);
+		return;
+	}
+	if (MODE != GB_REG && MODE != GB_LDS)
+		return;
+
+	// ---- publish this block's exact partials --------------------------------
+	__syncthreads();
+	const u64 D = P.domain;
+	const u64 nb = gridDim.x;
+	u64* mine = P.partials + blockIdx.x;     // element k of this block: mine[k * nb]
+	if (MODE == GB_REG) {
+		const int lane = threadIdx.x % kWave;
+		const int wave = threadIdx.x / kWave;
+		const int nwaves = blockDim.x / kWave;
+		const int E = (int)(D * P.nsums + D);
+		const u32 bd = blockDim.x;
+		for (int e = wave; e < E; e += nwaves) {
+			if ((u64)e < D * P.nsums) {
+				const u64 s = (u64)e / D, g = (u64)e % D;
+				const SumArg& S = P.sums[s];
+				u64 lo = 0;
+				i64 hi = 0;
+				for (u32 t = lane; t < bd; t += kWave) {
+					const u64 idx = (g * P.nsums + s) * bd + t;
+					const u64 l = L.lo[idx];
+					// a lane's sum of <= 32-bit arguments fits 64 bits: sign extend it
+					const i64 h = (P.wide && S.wide) ? (i64)L.hi[idx] : (S.arg_signed ? (i64)l >> 63 : 0);
+					const u64 nl = lo + l;
+					hi += h + (nl < lo ? 1 : 0);
+					lo = nl;
+				}
+				wave_sum_i128(lo, hi);
+				if (lane == 0) {
+					mine[(2 * e) * nb] = lo;
+					mine[(2 * e + 1) * nb] = (u64)hi;
+				}
+			} else {
+				const u64 g = (u64)e - D * P.nsums;
+				u64 c = 0;
+				for (u32 t = lane; t < bd; t += kWave)
+					c += L.cnt[g * bd + t];
+				c = wave_sum_u64(c);
+				if (lane == 0)
+					mine[(2 * D * P.nsums + g) * nb] = c;
+			}
+		}
+	} else {
+		const u64 n = D * P.nsums;
+		for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
+			const SumArg& S = P.sums[i / D];
+			const u64 lo = L.lo[i];
+			mine[(2 * i) * nb] = lo;
+			mine[(2 * i + 1) * nb] = (P.wide && S.wide) ? (u64)(i64)L.hi[i]
+				: (S.arg_signed ? (u64)((i64)lo >> 63) : 0ull);
+		}
+		for (u64 i = threadIdx.x; i < D; i += blockDim.x)
+			mine[(2 * n + i) * nb] = L.cnt[i];
+	}
+}
+
 // GB_LDS in the batched kernel: add a work item's dense LDS groups to the
 // global arrays.
 __device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
@@ -533,38 +640,11 @@ __device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
 // build side
 // --------------------------------------------------------------------------
 
-// Build side of one row: select, then insert (key - kmin, payload - pmin)
-// with CAS into linear-probing slots.  Replaces write_pos/write +
-// bucket_build (relalg_translator.cpp:828-928, runtime_struct.cpp:1283-1324).
-__device__ __forceinline__ void build_row(const BuildPlan& B, i64 row)
-{
-	if (!preds_pass_row(B.preds, B.npreds, row))
-		return;
-	const i64 key = load1(B.key, B.key_type, row);
-	if (key < B.kmin || key > B.kmax) {
-		atomicOr(B.error, 4u);
-		return;
-	}
-	const u32 off = (u32)(key - B.kmin);
-	const u32 pay = B.npay ? (u32)(load1(B.pay, B.pay_type, row) - B.pmin) : 0u;
-	const u64 val = (u64)off | ((u64)pay << 32);
-	u64 h = voila_hash((u64)off) & B.mask;
-	for (u64 n = 0; n <= B.mask; n++) {
-		if (n >= kMaxProbe) {
-			atomicOr(B.error, 32u);
-			break;
-		}
-		const u64 old = atomicCAS(&B.slots[h], kEmpty, val);
-		if (old == kEmpty)
-			break;
-		if ((u32)old == off)
-			*B.dups = 1u;
-		h = (h + 1) & B.mask;
-	}
-}
-
-// Build side of one chunk (U rows per lane): the first CAS of every row is
-// issued before any collision is resolved, so U atomics are in flight.
+// Build side of one chunk (U rows per lane): select, then insert (key - kmin,
+// payload - pmin).  Replaces write_pos/write + bucket_build
+// (relalg_translator.cpp:828-928, runtime_struct.cpp:1283-1324).  The first
+// CAS of every row is issued before any collision is resolved, so U atomics
+// are in flight.
 __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 end)
 {
 	i64 row[kU];
@@ -596,6 +676,33 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 		off[u] = (u32)(key - B.kmin);
 		val[u] = (u64)off[u] | ((u64)pay << 32);
 		h[u] = voila_hash((u64)off[u]) & B.mask;
+	}
+	if (B.dir_w) {
+		// direct layout: claim dir[off]; a key seen before goes on to the
+		// overflow slots below
+		u32* word[kU];
+		u32 mask[kU], bits[kU], seen[kU];
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			u32 shift;
+			word[u] = dir_word(B.dir, B.dir_w, off[u], mask[u], shift);
+			bits[u] = ((u32)(val[u] >> 32) << shift) & mask[u];
+		}
+#pragma unroll
+		for (int u = 0; u < kU; u++)
+			seen[u] = ok[u] ? atomicAnd(word[u], bits[u] | ~mask[u]) : 0xFFFFFFFFu;
+		bool spill = false;
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			if (ok[u] && (seen[u] & mask[u]) != mask[u])
+				atomicOr(word[u], seen[u] & mask[u]);   // a repeated key: restore
+			else
+				ok[u] = false;                          // first row of its key
+			spill = spill || ok[u];
+		}
+		if (!spill)
+			return;
+		*B.dups = 1u;
 	}
 #pragma unroll
 	for (int u = 0; u < kU; u++)

@@ -1,0 +1,458 @@
+// Generated probe kernels.
+//
+// The reference turns every query into C++ for its flavor and compiles it
+// with g++ (codegen.cpp:211-395, compiler.cpp:19-55).  Here a lowered Plan
+// becomes HIP source for one fused kernel: the column types, the predicate,
+// join and aggregate sequence, each join's table layout and the group-by
+// strategy are fixed in the text, while pointers and constants stay in the
+// Plan the kernel reads (so queries that differ only in data share one
+// compiled kernel).  hipRTC compiles it for the device's gfx target; the
+// code object is cached per (target, source) for the life of the process.
+//
+// The generated code calls the same device helpers as the generic kernel
+// (vh_fused.hpp: compare, voila_hash, locate / add_count / add_sum,
+// lds_layout / lds_init, block_epilogue), with the plan's choices as
+// literals, so both compute identical results.  What the generated kernel
+// gains: every probe column is loaded for all U rows of a chunk in one go,
+// before the predicates, so a chunk costs one round trip to memory for its
+// columns plus one per join, and register use is what this plan needs.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/voila_hip.h"
+#include "vh_engine.hpp"
+#include "vh_jit.hpp"
+#include "vh_kernels.hpp"
+
+namespace vh {
+
+#include "vh_jit_headers.inc"   // kJitHeaderNames / kJitHeaderText (build.py)
+
+namespace {
+
+const char* ctype_of(int t)
+{
+	switch (t) {
+	case VH_U8: return "uint8_t";
+	case VH_I8: return "int8_t";
+	case VH_U16: return "uint16_t";
+	case VH_I16: return "int16_t";
+	case VH_U32: return "uint32_t";
+	case VH_I32: return "int32_t";
+	default: return "i64";   // 64-bit columns are read as i64, like load1
+	}
+}
+
+struct Col {
+	const void* ptr;
+	int type;
+	std::string path;   // expression for the column's address in the kernel
+	bool pred;          // read by a predicate (loaded first)
+};
+
+struct Gen {
+	const Plan& P;
+	bool eager;
+	int U;
+	std::vector<Col> cols;
+	std::string out;
+
+	Gen(const Plan& p, bool e, int u) : P(p), eager(e), U(u) {}
+
+	void line(const std::string& s) { out += s; out += '\n'; }
+
+	int col_index(const Role& r, const std::string& path, bool pred)
+	{
+		for (size_t i = 0; i < cols.size(); i++)
+			if (cols[i].ptr == r.ptr && cols[i].type == r.type) {
+				cols[i].pred = cols[i].pred || pred;
+				return (int)i;
+			}
+		cols.push_back({r.ptr, r.type, path + ".ptr", pred});
+		return (int)cols.size() - 1;
+	}
+
+	void note_role(const Role& r, const std::string& path, bool pred)
+	{
+		if (r.kind == ROLE_COL)
+			col_index(r, path, pred);
+	}
+
+	// Value of a role for row u as an i64 expression.  tail_pay names the
+	// payload variable that replaces join `tail_join`'s in the JoinN tail.
+	std::string role(const Role& r, const std::string& path, int tail_join = -1,
+		const char* tail_pay = nullptr)
+	{
+		char b[256];
+		switch (r.kind) {
+		case ROLE_COL:
+			snprintf(b, sizeof b, "((i64)c%d[u])", col_index(r, path, false));
+			return b;
+		case ROLE_PAY:
+			if (r.slot == tail_join)
+				snprintf(b, sizeof b, "((i64)%s + P.joins[%d].pmin)", tail_pay, r.slot);
+			else
+				snprintf(b, sizeof b, "((i64)pay%d[u] + P.joins[%d].pmin)", r.slot, r.slot);
+			return b;
+		case ROLE_CONST:
+			return "(" + path + ".c)";
+		default:
+			return "((i64)0)";
+		}
+	}
+
+	std::string consume_body(int mode, int tail_join, const char* tail_pay)
+	{
+		std::string s;
+		char b[512];
+		s += "\t\t\tu64 g = 0;\n\t\t\tbool ok = true;\n";
+		for (int k = 0; k < P.nkeys; k++) {
+			snprintf(b, sizeof b, "P.keys[%d]", k);
+			const std::string kp = b;
+			s += "\t\t\t{\n\t\t\t\tconst u64 o = (u64)(" + role(P.keys[k].r, kp + ".r", tail_join, tail_pay) +
+				" - " + kp + ".min);\n";
+			s += "\t\t\t\tok = ok && o <= " + kp + ".range;\n";
+			if (mode == GB_HASH)
+				s += "\t\t\t\tg += o << " + kp + ".shift;\n";
+			else
+				s += "\t\t\t\tg += o * " + kp + ".mult;\n";
+			s += "\t\t\t}\n";
+		}
+		if (mode == GB_HASH)
+			s += "\t\t\tif (!ok) { atomicOr(P.error, 8u); return; }\n";
+		else
+			s += "\t\t\tif (!ok || g >= P.domain) { atomicOr(P.error, 8u); return; }\n";
+		snprintf(b, sizeof b, "\t\t\tconst i64 slot = locate<%d>(P, L, g);\n", mode);
+		s += b;
+		if (mode == GB_HASH)
+			s += "\t\t\tif (slot < 0) return;\n";
+		snprintf(b, sizeof b, "\t\t\tadd_count<%d>(P, L, g, slot);\n", mode);
+		s += b;
+		for (int si = 0; si < P.nsums; si++) {
+			const SumArg& S = P.sums[si];
+			std::string prod;
+			for (int t = 0; t < S.nterms; t++) {
+				const Term& T = S.t[t];
+				snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
+				const std::string tp = b;
+				std::string term = "(" + tp + ".c";
+				if (T.sa)
+					term += (T.sa > 0 ? " + " : " - ") + role(T.a, tp + ".a", tail_join, tail_pay);
+				if (T.sb)
+					term += (T.sb > 0 ? " + " : " - ") + role(T.b, tp + ".b", tail_join, tail_pay);
+				term += ")";
+				prod += (t ? " * (u64)" : "(u64)") + term;
+			}
+			if (prod.empty())
+				prod = "(u64)0";
+			snprintf(b, sizeof b, "\t\t\tadd_sum<%d>(P, L, %d, g, slot, %s);\n", mode, si, prod.c_str());
+			s += b;
+		}
+		return s;
+	}
+
+	std::string source(int mode)
+	{
+		char b[512];
+		// register every probe column: predicates first, then join keys,
+		// group keys and sum arguments
+		for (int p = 0; p < P.npreds; p++) {
+			snprintf(b, sizeof b, "P.preds[%d].r", p);
+			note_role(P.preds[p].r, b, true);
+		}
+		for (int j = 0; j < P.njoins; j++) {
+			snprintf(b, sizeof b, "P.joins[%d].key", j);
+			note_role(P.joins[j].key, b, false);
+		}
+		for (int k = 0; k < P.nkeys; k++) {
+			snprintf(b, sizeof b, "P.keys[%d].r", k);
+			note_role(P.keys[k].r, b, false);
+		}
+		for (int si = 0; si < P.nsums; si++)
+			for (int t = 0; t < P.sums[si].nterms; t++) {
+				const Term& T = P.sums[si].t[t];
+				snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
+				if (T.sa)
+					note_role(T.a, std::string(b) + ".a", false);
+				if (T.sb)
+					note_role(T.b, std::string(b) + ".b", false);
+			}
+
+		const int last = P.njoins - 1;
+		const bool tail = P.njoins > 0 && P.joins[last].variant == VH_JOINN;
+
+		line("// generated probe kernel");
+		line("typedef __hip_internal::int8_t int8_t;");
+		line("typedef __hip_internal::uint8_t uint8_t;");
+		line("typedef __hip_internal::int16_t int16_t;");
+		line("typedef __hip_internal::uint16_t uint16_t;");
+		line("typedef __hip_internal::int32_t int32_t;");
+		line("typedef __hip_internal::uint32_t uint32_t;");
+		line("typedef __hip_internal::int64_t int64_t;");
+		line("typedef __hip_internal::uint64_t uint64_t;");
+		line("#include \"vh_fused.hpp\"");
+		line("using namespace vh;");
+		snprintf(b, sizeof b, "static constexpr int U = %d;", U);
+		line(b);
+		line("extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)");
+		line("{");
+		line("\tconst Plan& P = *Pp;");
+		line("\textern __shared__ u64 lds[];");
+		snprintf(b, sizeof b, "\tconst Lds L = lds_layout<%d>(P, lds);", mode);
+		line(b);
+		snprintf(b, sizeof b, "\tlds_init<%d>(P, L);", mode);
+		line(b);
+		line("\tu32 dups_mask = 0;");
+		for (int j = 0; j < P.njoins; j++)
+			if (P.joins[j].variant == VH_JOINN) {
+				snprintf(b, sizeof b, "\tif (*P.joins[%d].dups) dups_mask |= %uu;", j, 1u << j);
+				line(b);
+			}
+		line("\t(void)dups_mask;");
+		line("\t__syncthreads();");
+		line("\tconst i64 rows = P.rows;");
+		line("\tconst i64 tile = (i64)U * blockDim.x;");
+		line("\tfor (i64 base = (i64)blockIdx.x * tile; base < rows; base += (i64)gridDim.x * tile) {");
+		line("\t\ti64 row[U];");
+		line("\t\tbool alive[U];");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) {");
+		line("\t\t\trow[u] = base + (i64)u * blockDim.x + threadIdx.x;");
+		line("\t\t\talive[u] = row[u] < rows;");
+		line("\t\t}");
+		// column loads: predicate columns, then (eager) the rest
+		auto load = [&](int i, const char* guard) {
+			const Col& c = cols[i];
+			snprintf(b, sizeof b, "\t\t%s c%d[U];", ctype_of(c.type), i);
+			line(b);
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%d[u] = %s ? ((const %s*)%s)[row[u]] : (%s)0;", i,
+				guard, ctype_of(c.type), c.path.c_str(), ctype_of(c.type));
+			line(b);
+		};
+		for (size_t i = 0; i < cols.size(); i++)
+			if (cols[i].pred)
+				load((int)i, "alive[u]");
+		if (eager)
+			for (size_t i = 0; i < cols.size(); i++)
+				if (!cols[i].pred)
+					load((int)i, "alive[u]");
+		for (int p = 0; p < P.npreds; p++) {
+			snprintf(b, sizeof b, "P.preds[%d].r", p);
+			const std::string v = role(P.preds[p].r, b);
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) alive[u] = alive[u] && compare(%d, %d, %s, P.preds[%d].value);",
+				P.preds[p].cmp, P.preds[p].r.type, v.c_str(), p);
+			line(b);
+		}
+		if (!eager)
+			for (size_t i = 0; i < cols.size(); i++)
+				if (!cols[i].pred)
+					load((int)i, "alive[u]");
+
+		// joins
+		for (int j = 0; j < P.njoins; j++) {
+			const JoinT& J = P.joins[j];
+			snprintf(b, sizeof b, "\t\tu32 pay%d[U];", j);
+			line(b);
+			line("\t\t{");
+			snprintf(b, sizeof b, "\t\t\tconst JoinT& J = P.joins[%d];", j);
+			line(b);
+			line("\t\t\tu32 off[U];");
+			line("#pragma unroll");
+			line("\t\t\tfor (int u = 0; u < U; u++) {");
+			snprintf(b, sizeof b, "P.joins[%d].key", j);
+			line("\t\t\t\tconst i64 key = " + role(J.key, b) + ";");
+			line("\t\t\t\tif (key < J.kmin || key > J.kmax) alive[u] = false;");
+			line("\t\t\t\toff[u] = (u32)(key - J.kmin);");
+			line("\t\t\t}");
+			if (J.dir_w) {
+				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
+				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = alive[u] ? (u32)((const %s*)J.dir)[off[u]] : %s;",
+					j, et, ev);
+				line(b);
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (pay%d[u] == %s) alive[u] = false;", j, ev);
+				line(b);
+				if (j == last && tail) {
+					// repeated keys sit in the overflow slots from voila_hash(off) on
+					line("\t\t\tu64 th[U];");
+					line("\t\t\tu32 toff[U];");
+					line("#pragma unroll");
+					line("\t\t\tfor (int u = 0; u < U; u++) { th[u] = voila_hash((u64)off[u]) & J.mask; toff[u] = off[u]; }");
+				}
+			} else {
+				line("\t\t\tu64 h[U], s[U];");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & J.mask;");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = alive[u] ? J.slots[h[u]] : kEmpty;");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) {");
+				snprintf(b, sizeof b, "\t\t\t\tpay%d[u] = 0;", j);
+				line(b);
+				line("\t\t\t\tif (!alive[u]) continue;");
+				line("\t\t\t\tu64 n = 0;");
+				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {");
+				line("\t\t\t\t\tn++;");
+				line("\t\t\t\t\th[u] = (h[u] + 1) & J.mask;");
+				line("\t\t\t\t\ts[u] = J.slots[h[u]];");
+				line("\t\t\t\t}");
+				line("\t\t\t\tif (n >= kMaxProbe) atomicOr(P.error, 32u);");
+				line("\t\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) alive[u] = false;");
+				snprintf(b, sizeof b, "\t\t\t\telse pay%d[u] = (u32)(s[u] >> 32);", j);
+				line(b);
+				line("\t\t\t}");
+				if (j == last && tail) {
+					line("\t\t\tu64 th[U];");
+					line("\t\t\tu32 toff[U];");
+					line("#pragma unroll");
+					line("\t\t\tfor (int u = 0; u < U; u++) { th[u] = (h[u] + 1) & J.mask; toff[u] = off[u]; }");
+				}
+			}
+			if (j != last && J.variant == VH_JOINN) {
+				// a first-match join may not fan out: reported, as the generic kernel does
+				snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u)", j);
+				line(b);
+				line("#pragma unroll");
+				line("\t\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) atomicOr(P.error, 16u);");
+			}
+			if (j == last) {
+				// aggregate inside the last join's scope (its tail needs th / toff)
+				const std::string tp = "tp";
+				line("\t\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
+				out += consume_body(mode, last, "tp");
+				line("\t\t\t};");
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u, pay%d[u]);", j);
+				line(b);
+				if (tail) {
+					snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u) {", j);
+					line(b);
+					line("#pragma unroll");
+					line("\t\t\t\tfor (int u = 0; u < U; u++) {");
+					line("\t\t\t\t\tif (!alive[u]) continue;");
+					line("\t\t\t\t\tu64 hh = th[u];");
+					line("\t\t\t\t\tfor (u64 n = 0;; n++) {");
+					line("\t\t\t\t\t\tif (n >= kMaxProbe) { atomicOr(P.error, 32u); break; }");
+					line("\t\t\t\t\t\tconst u64 sv = J.slots[hh];");
+					line("\t\t\t\t\t\tif (sv == kEmpty) break;");
+					line("\t\t\t\t\t\tif ((u32)sv == toff[u]) consume(u, (u32)(sv >> 32));");
+					line("\t\t\t\t\t\thh = (hh + 1) & J.mask;");
+					line("\t\t\t\t\t}");
+					line("\t\t\t\t}");
+					line("\t\t\t}");
+				}
+			}
+			line("\t\t}");
+		}
+		if (P.njoins == 0) {
+			line("\t\t{");
+			line("\t\t\tauto consume = [&](int u) __attribute__((always_inline)) {");
+			out += consume_body(mode, -1, nullptr);
+			line("\t\t\t};");
+			line("#pragma unroll");
+			line("\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u);");
+			line("\t\t}");
+		}
+		line("\t}");
+		snprintf(b, sizeof b, "\tblock_epilogue<%d>(P, L);", mode);
+		line(b);
+		line("}");
+		return out;
+	}
+};
+
+struct Compiled {
+	hipModule_t mod = nullptr;
+	hipFunction_t fn = nullptr;
+};
+
+std::mutex g_mu;
+std::map<std::string, Compiled> g_cache;
+
+std::string device_arch()
+{
+	int dev = 0;
+	hipDeviceProp_t prop;
+	if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+		return "";
+	std::string a = prop.gcnArchName;
+	const size_t c = a.find(':');
+	return c == std::string::npos ? a : a.substr(0, c);
+}
+
+} // namespace
+
+std::string jit_source(const Plan& P, bool eager, int unroll)
+{
+	Gen g(P, eager, unroll);
+	return g.source(P.mode);
+}
+
+int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log)
+{
+	const std::string arch = device_arch();
+	if (arch.empty())
+		return fail(VH_ERR_NODEVICE, "no device to generate a kernel for");
+	const std::string key = arch + "\n" + src;
+	std::lock_guard<std::mutex> lock(g_mu);
+	auto it = g_cache.find(key);
+	if (it != g_cache.end()) {
+		*fn = it->second.fn;
+		return VH_OK;
+	}
+	hiprtcProgram prog;
+	if (hiprtcCreateProgram(&prog, src.c_str(), "vh_generated.hip", kJitHeaders, kJitHeaderText, kJitHeaderNames) !=
+		HIPRTC_SUCCESS)
+		return fail(VH_ERR_HIP, "hiprtcCreateProgram failed");
+	const std::string archopt = "--offload-arch=" + arch;
+	const char* opts[] = {archopt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+		"-packed-fp32-ops"};
+	const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+	size_t n = 0;
+	hiprtcGetProgramLogSize(prog, &n);
+	log.assign(n, '\0');
+	if (n)
+		hiprtcGetProgramLog(prog, &log[0]);
+	if (rc != HIPRTC_SUCCESS) {
+		hiprtcDestroyProgram(&prog);
+		return fail(VH_ERR_UNSUPPORTED, "hipRTC: %s: %.400s", hiprtcGetErrorString(rc), log.c_str());
+	}
+	size_t sz = 0;
+	hiprtcGetCodeSize(prog, &sz);
+	std::vector<char> code(sz);
+	hiprtcGetCode(prog, code.data());
+	hiprtcDestroyProgram(&prog);
+	Compiled c;
+	hipError_t e = hipModuleLoadData(&c.mod, code.data());
+	if (e == hipSuccess)
+		e = hipModuleGetFunction(&c.fn, c.mod, "vh_generated");
+	if (e != hipSuccess)
+		return fail(VH_ERR_HIP, "loading the generated kernel: %s", hipGetErrorString(e));
+	g_cache[key] = c;
+	*fn = c.fn;
+	return VH_OK;
+}
+
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, hipStream_t s)
+{
+	const Plan* arg = dP;
+	void* args[] = {(void*)&arg};
+	hipError_t e = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, 256, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
+		nullptr);
+	if (e != hipSuccess)
+		return e;
+	return launch_merge(P, dP, grid, s);
+}
+
+} // namespace vh

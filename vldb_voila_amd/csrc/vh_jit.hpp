@@ -1,0 +1,26 @@
+// Generated probe kernels (vh_jit.cpp): HIP source for one lowered plan,
+// compiled with hipRTC and cached per (gfx target, source text).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "vh_plan.hpp"
+
+namespace vh {
+
+static constexpr int kJitUnroll = 4;   // rows per lane per chunk of generated kernels
+
+// Source of the fused probe kernel for P (its mode, joins, keys and sums).
+// eager: load every probe column before the predicates (else columns not
+// read by a predicate are loaded only for qualifying rows); unroll = rows
+// per lane per chunk.
+std::string jit_source(const Plan& P, bool eager, int unroll);
+// Compile (or fetch from the cache) `src` for the current device.
+int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log);
+// Launch a compiled kernel over P (dP: its device copy), then the merge of
+// per-block partials where the mode has them.
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, hipStream_t s);
+
+} // namespace vh

@@ -7,10 +7,14 @@
 
 namespace vh {
 
-hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s);
+// p[0, n) = v; with only_if, only when *only_if != 0 (read on the device)
+hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s, const u32* only_if = nullptr);
 hipError_t launch_build(const BuildPlan* dB, int grid, hipStream_t s);
-// P: host copy (launch shape); dP: the same plan in device memory
+// P: host copy (launch shape); dP: the same plan in device memory.
+// launch_pipeline runs the generic fused kernel, then launch_merge.
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s);
+// GB_REG / GB_LDS: merge the per-block partials of a `grid`-block launch
+hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 size_t pipeline_lds_bytes(const Plan& P);
 
 } // namespace vh

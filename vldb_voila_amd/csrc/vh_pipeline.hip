@@ -24,57 +24,57 @@ namespace vh {
 
 static constexpr int kBlock = 256;
 
-// Exact merge of per-block partials by the last block to finish.
-// Partial layout per block: lo/hi of (s * domain + g) for all sums, then cnt[g].
-__device__ void merge_partials(const Plan& P, u64* red_lo, i64* red_hi)
+// Exact merge of the per-block partials, one workgroup per output element.
+// Partials are element-major: sum element e (= s * domain + g) keeps its low
+// words at partials[(2e) * nb + b] and high words at partials[(2e+1) * nb + b],
+// count g at partials[(2 * domain * nsums + g) * nb + b], b = block.  A launch
+// of its own after the pipeline kernel: no block waits for another, so no
+// device-scope fence or completion counter sits on the hot path.
+__global__ __launch_bounds__(kBlock) void merge_kernel(const Plan* __restrict__ Pp, u32 nb)
 {
+	const Plan& P = *Pp;
+	__shared__ u64 red_lo[kBlock / kWave];
+	__shared__ i64 red_hi[kBlock / kWave];
 	const u64 D = P.domain;
-	const int E = (int)(D * P.nsums + D);
-	const u64 W = 2 * D * P.nsums + D;        // words per block partial
-	const int nb = gridDim.x;
-	int G = (int)blockDim.x / E;
-	if (G < 1)
-		G = 1;
-	for (int base = 0; base < E; base += (int)blockDim.x / G) {
-		const int e = base + (int)threadIdx.x / G;
-		const int sub = (int)threadIdx.x % G;
-		const bool mine = e < E && (int)threadIdx.x / G < (int)blockDim.x / G;
-		u64 lo = 0;
-		i64 hi = 0;
-		if (mine) {
-			if ((u64)e < D * P.nsums) {
-				for (int b = sub; b < nb; b += G) {
-					const u64* pb = P.partials + (u64)b * W;
-					const u64 l = __atomic_load_n(&pb[2 * e], __ATOMIC_RELAXED);
-					const i64 h = (i64)__atomic_load_n(&pb[2 * e + 1], __ATOMIC_RELAXED);
-					const u64 nl = lo + l;
-					hi += h + (nl < lo ? 1 : 0);
-					lo = nl;
-				}
-			} else {
-				const u64 g = (u64)e - D * P.nsums;
-				for (int b = sub; b < nb; b += G)
-					lo += __atomic_load_n(&P.partials[(u64)b * W + 2 * D * P.nsums + g], __ATOMIC_RELAXED);
-			}
+	const u64 Dn = D * (u64)P.nsums;
+	const u64 e = blockIdx.x;
+	const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+	u64 lo = 0;
+	i64 hi = 0;
+	if (e < Dn) {
+		const u64* pl = P.partials + 2 * e * nb;
+		const u64* ph = pl + nb;
+		for (u32 b = threadIdx.x; b < nb; b += blockDim.x) {
+			const u64 l = pl[b];
+			const u64 nl = lo + l;
+			hi += (i64)ph[b] + (nl < lo ? 1 : 0);
+			lo = nl;
 		}
-		red_lo[threadIdx.x] = lo;
-		red_hi[threadIdx.x] = hi;
-		__syncthreads();
-		if (sub == 0 && mine) {
-			for (int k = 1; k < G; k++) {
-				const u64 l = red_lo[threadIdx.x + k];
-				const u64 nl = lo + l;
-				hi += red_hi[threadIdx.x + k] + (nl < lo ? 1 : 0);
-				lo = nl;
-			}
-			if ((u64)e < D * P.nsums) {
-				P.out_lo[e] = lo;
-				P.out_hi[e] = hi;
-			} else {
-				P.out_cnt[(u64)e - D * P.nsums] = lo;
-			}
-		}
-		__syncthreads();
+	} else {
+		const u64* pc = P.partials + (2 * Dn + (e - Dn)) * nb;
+		for (u32 b = threadIdx.x; b < nb; b += blockDim.x)
+			lo += pc[b];
+	}
+	wave_sum_i128(lo, hi);
+	if (lane == 0) {
+		red_lo[wave] = lo;
+		red_hi[wave] = hi;
+	}
+	__syncthreads();
+	if (threadIdx.x != 0)
+		return;
+	lo = 0;
+	hi = 0;
+	for (int w = 0; w < (int)(blockDim.x / kWave); w++) {
+		const u64 nl = lo + red_lo[w];
+		hi += red_hi[w] + (nl < lo ? 1 : 0);
+		lo = nl;
+	}
+	if (e < Dn) {
+		P.out_lo[e] = lo;
+		P.out_hi[e] = hi;
+	} else {
+		P.out_cnt[e - Dn] = lo;
 	}
 }
 
@@ -83,9 +83,6 @@ __global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict
 {
 	const Plan& P = *Pp;
 	extern __shared__ u64 lds[];
-	__shared__ u64 red_lo[kBlock];
-	__shared__ i64 red_hi[kBlock];
-	__shared__ u32 is_last;
 
 	const Lds L = lds_layout<MODE>(P, lds);
 	lds_init<MODE>(P, L);
@@ -98,82 +95,7 @@ __global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict
 	const i64 tile = (i64)kU * blockDim.x;
 	process_range<MODE>(P, L, (i64)blockIdx.x * tile, P.rows, (i64)gridDim.x * tile, dups_mask);
 
-	if (MODE == GB_LDSHASH) {
-		__syncthreads();
-		ldshash_flush(P, L);
-		return;
-	}
-	if (MODE != GB_REG && MODE != GB_LDS)
-		return;
-
-	// ---- publish this block's exact partials --------------------------------
-	__syncthreads();
-	const u64 D = P.domain;
-	const u64 W = 2 * D * P.nsums + D;
-	u64* mine = P.partials + (u64)blockIdx.x * W;
-	if (MODE == GB_REG) {
-		const int lane = threadIdx.x % kWave;
-		const int wave = threadIdx.x / kWave;
-		const int nwaves = blockDim.x / kWave;
-		const int E = (int)(D * P.nsums + D);
-		const u32 bd = blockDim.x;
-		for (int e = wave; e < E; e += nwaves) {
-			if ((u64)e < D * P.nsums) {
-				const u64 s = (u64)e / D, g = (u64)e % D;
-				const SumArg& S = P.sums[s];
-				u64 lo = 0;
-				i64 hi = 0;
-				for (u32 t = lane; t < bd; t += kWave) {
-					const u64 idx = (g * P.nsums + s) * bd + t;
-					const u64 l = L.lo[idx];
-					// a lane's sum of <= 32-bit arguments fits 64 bits: sign extend it
-					const i64 h = (P.wide && S.wide) ? (i64)L.hi[idx] : (S.arg_signed ? (i64)l >> 63 : 0);
-					const u64 nl = lo + l;
-					hi += h + (nl < lo ? 1 : 0);
-					lo = nl;
-				}
-				wave_sum_i128(lo, hi);
-				if (lane == 0) {
-					mine[2 * e] = lo;
-					mine[2 * e + 1] = (u64)hi;
-				}
-			} else {
-				const u64 g = (u64)e - D * P.nsums;
-				u64 c = 0;
-				for (u32 t = lane; t < bd; t += kWave)
-					c += L.cnt[g * bd + t];
-				c = wave_sum_u64(c);
-				if (lane == 0)
-					mine[2 * D * P.nsums + g] = c;
-			}
-		}
-	} else {
-		const u64 n = D * P.nsums;
-		for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
-			const SumArg& S = P.sums[i / D];
-			const u64 lo = L.lo[i];
-			mine[2 * i] = lo;
-			mine[2 * i + 1] = (P.wide && S.wide) ? (u64)(i64)L.hi[i]
-				: (S.arg_signed ? (u64)((i64)lo >> 63) : 0ull);
-		}
-		for (u64 i = threadIdx.x; i < D; i += blockDim.x)
-			mine[2 * n + i] = L.cnt[i];
-	}
-
-	// ---- last block merges --------------------------------------------------
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		const u32 prev = __hip_atomic_fetch_add(P.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-		is_last = prev == gridDim.x - 1;
-	}
-	__syncthreads();
-	if (!is_last)
-		return;
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-	merge_partials(P, red_lo, red_hi);
-	if (threadIdx.x == 0)
-		__hip_atomic_store(P.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	block_epilogue<MODE>(P, L);
 }
 
 __global__ __launch_bounds__(kBlock) void build_kernel(const BuildPlan* __restrict__ Bp)
@@ -184,8 +106,10 @@ __global__ __launch_bounds__(kBlock) void build_kernel(const BuildPlan* __restri
 		build_chunk(B, base, B.rows);
 }
 
-__global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v)
+__global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v, const u32* only_if)
 {
+	if (only_if && *only_if == 0u)
+		return;
 	const u64 stride = (u64)gridDim.x * blockDim.x;
 	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
 		p[i] = v;
@@ -193,14 +117,14 @@ __global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v)
 
 // ---- launchers ----------------------------------------------------------
 
-hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s)
+hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s, const u32* only_if)
 {
 	if (!n)
 		return hipSuccess;
 	u64 blocks = (n + kBlock - 1) / kBlock;
 	if (blocks > 4096)
 		blocks = 4096;
-	hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v);
+	hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v, only_if);
 	return hipGetLastError();
 }
 
@@ -221,6 +145,15 @@ size_t pipeline_lds_bytes(const Plan& P)
 	return 0;
 }
 
+hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s)
+{
+	if (P.mode != GB_REG && P.mode != GB_LDS)
+		return hipSuccess;
+	const u64 elems = P.domain * (u64)P.nsums + P.domain;
+	hipLaunchKernelGGL(merge_kernel, dim3((unsigned)elems), dim3(kBlock), 0, s, dP, (u32)grid);
+	return hipGetLastError();
+}
+
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s)
 {
 	const size_t lds = pipeline_lds_bytes(P);
@@ -233,7 +166,10 @@ hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t 
 	default: VH_LAUNCH(GB_HASH); break;
 	}
 #undef VH_LAUNCH
-	return hipGetLastError();
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		return e;
+	return launch_merge(P, dP, grid, s);
 }
 
 } // namespace vh

@@ -3,7 +3,9 @@
 // it with scalar instructions only.
 #pragma once
 
+#ifndef __HIPCC_RTC__   // hipRTC provides these itself
 #include <stdint.h>
+#endif
 
 #include "../../include/voila_hip.h"
 
@@ -52,11 +54,18 @@ struct Pred {
 	int pad;
 };
 
-// Open-addressing join table: 64-bit slots, low word = key - kmin, high word
-// = payload - pmin (both < 2^32 - 1), all-ones = empty.  Linear probing.
+// Join table.  Hash layout: open addressing over 64-bit slots, low word =
+// key - kmin, high word = payload - pmin (both < 2^32 - 1), all-ones = empty,
+// linear probing from voila_hash(key - kmin).  Direct layout (dir_w != 0):
+// dir[key - kmin] holds payload - pmin in dir_w bytes (all-ones = no row);
+// a key's second and later rows go to the hash slots, which then serve only
+// as its overflow (dups set).
 struct JoinT {
 	u64* slots;
 	u64 mask;
+	const void* dir;
+	int dir_w;
+	int pad0;
 	Role key;            // probe-side key
 	i64 kmin;
 	i64 kmax;
@@ -105,9 +114,8 @@ struct Plan {
 	u64* out_lo;
 	i64* out_hi;
 	u64* out_cnt;
-	// per-block partials and completion counter (GB_REG / GB_LDS)
+	// per-block partials, element-major (GB_REG / GB_LDS), merged by merge_kernel
 	u64* partials;
-	u32* done;
 	// hash mode: packed keys, EMPTY = ~0
 	u64* hkeys;
 	u64 hmask;
@@ -128,6 +136,9 @@ struct BuildPlan {
 	i64 pmin;
 	u64* slots;
 	u64 mask;
+	void* dir;           // direct layout (see JoinT), dir_w bytes per key
+	int dir_w;
+	int pad0;
 	u32* dups;
 	u32* error;          // bit 2: key outside [kmin, kmax]
 };

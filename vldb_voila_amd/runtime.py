@@ -47,6 +47,7 @@ def lib():
         L.vh_query_reset.argtypes = [vp]
         L.vh_query_free.argtypes = [vp]
         L.vh_query_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+        L.vh_query_kernel.argtypes = [vp, C.POINTER(i32), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.vh_type_from_minmax.argtypes = [C.c_double, C.c_double]
         _LIB = L
     return _LIB
@@ -83,6 +84,14 @@ class DeviceBuffer:
         if arr.nbytes:
             check(lib().vh_memcpy_h2d(b.ptr, arr.ctypes.data, arr.nbytes))
         return b
+
+    def copy_from(self, arr: np.ndarray):
+        """Overwrite the buffer's first arr.nbytes bytes with `arr`."""
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if arr.nbytes:
+            check(lib().vh_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
 
     def to_numpy(self, dtype, count):
         out = np.empty(count, dtype=dtype)
@@ -152,12 +161,14 @@ class Event:
             pass
 
 
-def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0):
+def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0):
     c = abi.vh_config()
     c.group_mode = group_mode
     c.grid = grid
     c.table_shift = table_shift
     c.tile_shift = tile_shift
+    c.join_layout = join_layout
+    c.kernel = kernel
     return c
 
 
@@ -193,6 +204,14 @@ class Query:
         m, g, n = C.c_int32(), C.c_int32(), C.c_int64()
         check(lib().vh_query_info(self.ptr, C.byref(m), C.byref(g), C.byref(n)))
         return {"group_mode": m.value, "grid": g.value, "groups": n.value}
+
+    def kernel(self):
+        """(enum vh_kernel in use, generated HIP source or '')."""
+        k, n = C.c_int32(), C.c_size_t()
+        check(lib().vh_query_kernel(self.ptr, C.byref(k), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().vh_query_kernel(self.ptr, None, buf, n.value + 1, C.byref(n)))
+        return k.value, buf.value.decode()
 
     def reset(self):
         check(lib().vh_query_reset(self.ptr))
