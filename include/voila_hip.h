@@ -234,8 +234,10 @@ int vh_query_new(vh_query** q, const vh_pipeline_desc* desc, const vh_config* cf
 int vh_query_run(vh_query* q, void* stream);
 /* One phase of vh_query_run, in order: 0 = clear tables and flags,
  * 1 = build every join table (pipelines A+B of relalg_translator.cpp:828-928),
- * 2 = fused probe + aggregate (pipeline C, :933-1111, and the group-by).
- * Lets a caller time the probe kernel alone; the result is the same. */
+ * 2 = fused probe + aggregate kernel (pipeline C, :933-1111, and the
+ * group-by), 3 = merge of per-block partial aggregates (a no-op for the
+ * group-by strategies that accumulate globally).  Lets a caller time the
+ * probe kernel alone; running 0..3 gives vh_query_run's result. */
 int vh_query_run_phase(vh_query* q, void* stream, int phase);
 /* Synchronise `stream` and render QueryResult text: one row per group,
  * '|'-separated, numbers as voila_cast<T>::to_cstr prints them

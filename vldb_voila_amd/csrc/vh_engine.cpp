@@ -394,6 +394,22 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		if (layout != VH_LAYOUT_HASH && krange <= (double)(1u << 30) && (layout == VH_LAYOUT_DIRECT || dense)) {
 			B.dir_w = prange <= 254 ? 1 : (prange <= 65534 ? 2 : 4);
 			L.dir_words[j] = ((uint64_t)krange * (uint64_t)B.dir_w + 7) / 8;
+			// partitioned build: slices of about 8192 rows (one workgroup
+			// each), at most kMaxSliceBytes of LDS and kMaxBuildParts of them
+			const uint64_t R = (uint64_t)krange;
+			const uint64_t brows = (uint64_t)std::max<int64_t>(jd.build.rows, 1);
+			int shift = 10;
+			while (shift < 30 && ((uint64_t)1 << shift) * brows < R * 8192)
+				shift++;
+			while (shift > 2 && ((uint64_t)1 << shift) * (uint64_t)B.dir_w > (uint64_t)kMaxSliceBytes)
+				shift--;
+			const uint64_t parts = (R + ((uint64_t)1 << shift) - 1) >> shift;
+			const uint64_t chunks = (brows + kBuildChunkRows - 1) / kBuildChunkRows;
+			if (parts <= (uint64_t)kMaxBuildParts && chunks <= (uint64_t)kMaxBuildChunks) {
+				B.pshift = shift;
+				B.nparts = (int)parts;
+				B.nchunks = jd.build.rows > 0 ? (int)chunks : 0;
+			}
 		}
 
 		JoinT& J = P.joins[j];
@@ -473,11 +489,11 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	if (mode == VH_GROUP_DENSE)   // dense LDS state: lane-private if it fits, else shared
 		mode = reg_fits ? GB_REG : GB_LDS;
 	if (mode < 0) {
-		// built-in heuristic; the autotuner and the learned selector override it
-		if (domain <= kRegSlots && P.nsums <= kRegSums &&
-			domain * P.nsums * 256 * 12 + domain * 256 * 4 <= 64 * 1024)
-			mode = GB_REG;
-		else if (lds_need <= 32 * 1024)
+		// built-in heuristic; the autotuner and the learned selector override
+		// it.  A shared LDS table beats lane-private accumulators even at a
+		// handful of groups: the latter's LDS footprint (groups x sums x 12
+		// bytes per lane) caps the workgroups a CU can hold.
+		if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
 		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
 			mode = GB_LDSHASH;
@@ -558,7 +574,7 @@ int Query::alloc(void** p, size_t bytes)
 	return VH_OK;
 }
 
-int query_create(Query* q, bool want_generated)
+int query_create(Query* q, bool single)
 {
 	Lowered& L = q->L;
 	Plan& P = L.plan;
@@ -579,8 +595,19 @@ int query_create(Query* q, bool want_generated)
 		if (L.dir_words[j]) {
 			if ((rc = q->alloc((void**)&q->dir[j], L.dir_words[j] * sizeof(u64))))
 				return rc;
+			HIP_TRY(hipMemset(q->dir[j], 0xFF, L.dir_words[j] * sizeof(u64)));
 			P.joins[j].dir = q->dir[j];
 			L.builds[j].dir = q->dir[j];
+		}
+		BuildPlan& B = L.builds[j];
+		if (!single)
+			B.nparts = 0;   // batched builds claim entries in place
+		if (B.nparts) {
+			const size_t g = (size_t)std::max(B.nchunks, 1);
+			if ((rc = q->alloc((void**)&B.pbuf, g * kBuildChunkRows * sizeof(u64))) ||
+				(rc = q->alloc((void**)&B.pcnt, g * (size_t)B.nparts * sizeof(u32))) ||
+				(rc = q->alloc((void**)&B.poff, g * (size_t)B.nparts * sizeof(u32))))
+				return rc;
 		}
 		L.builds[j].slots = q->slots[j];
 		L.builds[j].dups = q->flags + 1 + j;
@@ -616,7 +643,7 @@ int query_create(Query* q, bool want_generated)
 	const int kernel = q->cfg.kernel;
 	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_LAZY)
 		return fail(VH_ERR_INVALID, "kernel %d", kernel);
-	if (want_generated && kernel != VH_KERNEL_GENERIC) {
+	if (single && kernel != VH_KERNEL_GENERIC) {
 		q->jit_src = jit_source(P, kernel != VH_KERNEL_JIT_LAZY, kJitUnroll);
 		std::string log;
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
@@ -634,7 +661,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 {
 	Lowered& L = q->L;
 	Plan& P = L.plan;
-	if (phase < -1 || phase > 2)
+	if (phase < -1 || phase > 3)
 		return fail(VH_ERR_INVALID, "phase %d", phase);
 	if (phase == -1 || phase == 0) {
 	// clear: join tables, atomically accumulated outputs, then the
@@ -642,7 +669,9 @@ int query_run(Query* q, hipStream_t s, int phase)
 	// when the last build spilled into them (its dups flag, still set here).
 	for (int j = 0; j < P.njoins; j++) {
 		if (q->dir[j]) {
-			HIP_TRY(launch_fill(q->dir[j], L.dir_words[j], ~0ull, s));
+			// a partitioned build writes every entry of the table itself
+			if (!L.builds[j].nparts)
+				HIP_TRY(launch_fill(q->dir[j], L.dir_words[j], ~0ull, s));
 			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s, q->flags + 1 + j));
 		} else {
 			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
@@ -660,15 +689,21 @@ int query_run(Query* q, hipStream_t s, int phase)
 	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
 	}
 	if (phase == -1 || phase == 1) {
-		for (int j = 0; j < P.njoins; j++)
-			if (L.builds[j].rows > 0)
+		for (int j = 0; j < P.njoins; j++) {
+			if (L.builds[j].nparts)
+				HIP_TRY(launch_part_build(L.builds[j], q->dbuild[j], s));
+			else if (L.builds[j].rows > 0)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
+		}
 	}
 	if (phase == -1 || phase == 2) {
 		if (q->jit_fn)
 			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, s));
 		else
 			HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
+	}
+	if (phase == -1 || phase == 3) {
+		HIP_TRY(launch_merge(P, q->dplan, L.grid, s));
 		q->ran = true;
 	}
 	return VH_OK;

@@ -62,8 +62,9 @@ const char* last_error();
 int type_from_minmax(double mn, double mx);
 int type_pipeline(const vh_pipeline_desc* d, Typing& ty);
 int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
-// want_generated: compile the generated probe kernel (per cfg.kernel)
-int query_create(Query* q, bool want_generated = true);
+// single: a query of its own (generated probe kernel per cfg.kernel,
+// partitioned builds); false for the pipelines of a batch
+int query_create(Query* q, bool single = true);
 int query_run(Query* q, hipStream_t s, int phase);
 int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows);
 

@@ -58,6 +58,20 @@ __device__ __forceinline__ u32* dir_word(void* dir, int w, u32 off, u32& mask, u
 // only have cleared bits of the first row's payload, and every bit cleared
 // while set is set again by the row whose AND cleared it, after that AND.
 // After the launch every entry holds its first row's payload exactly.
+// Conjunctive predicates of one row.
+__device__ __forceinline__ bool preds_pass_row(const Pred* preds, int npreds, i64 row)
+{
+	bool ok = true;
+#pragma unroll
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < npreds) {
+			const Pred& Q = preds[p];
+			ok = ok && compare(Q.cmp, Q.r.type, load1(Q.r.ptr, Q.r.type, row), Q.value);
+		}
+	}
+	return ok;
+}
+
 // --------------------------------------------------------------------------
 // chunk state
 // --------------------------------------------------------------------------

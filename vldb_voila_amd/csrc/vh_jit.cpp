@@ -136,26 +136,33 @@ struct Gen {
 		snprintf(b, sizeof b, "\t\t\tadd_count<%d>(P, L, g, slot);\n", mode);
 		s += b;
 		for (int si = 0; si < P.nsums; si++) {
-			const SumArg& S = P.sums[si];
-			std::string prod;
-			for (int t = 0; t < S.nterms; t++) {
-				const Term& T = S.t[t];
-				snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
-				const std::string tp = b;
-				std::string term = "(" + tp + ".c";
-				if (T.sa)
-					term += (T.sa > 0 ? " + " : " - ") + role(T.a, tp + ".a", tail_join, tail_pay);
-				if (T.sb)
-					term += (T.sb > 0 ? " + " : " - ") + role(T.b, tp + ".b", tail_join, tail_pay);
-				term += ")";
-				prod += (t ? " * (u64)" : "(u64)") + term;
-			}
-			if (prod.empty())
-				prod = "(u64)0";
-			snprintf(b, sizeof b, "\t\t\tadd_sum<%d>(P, L, %d, g, slot, %s);\n", mode, si, prod.c_str());
+			snprintf(b, sizeof b, "\t\t\tadd_sum<%d>(P, L, %d, g, slot, %s);\n", mode, si,
+				sum_expr(si, tail_join, tail_pay).c_str());
 			s += b;
 		}
 		return s;
+	}
+
+	// Argument of sum si for row u: the product of its linear terms (u64
+	// wrap-around, as eval_sum_u computes it).
+	std::string sum_expr(int si, int tail_join, const char* tail_pay)
+	{
+		char b[256];
+		const SumArg& S = P.sums[si];
+		std::string prod;
+		for (int t = 0; t < S.nterms; t++) {
+			const Term& T = S.t[t];
+			snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
+			const std::string tp = b;
+			std::string term = "(" + tp + ".c";
+			if (T.sa)
+				term += (T.sa > 0 ? " + " : " - ") + role(T.a, tp + ".a", tail_join, tail_pay);
+			if (T.sb)
+				term += (T.sb > 0 ? " + " : " - ") + role(T.b, tp + ".b", tail_join, tail_pay);
+			term += ")";
+			prod += (t ? " * (u64)" : "(u64)") + term;
+		}
+		return prod.empty() ? "(u64)0" : prod;
 	}
 
 	std::string source(int mode)
@@ -205,9 +212,10 @@ struct Gen {
 		line("{");
 		line("\tconst Plan& P = *Pp;");
 		line("\textern __shared__ u64 lds[];");
-		snprintf(b, sizeof b, "\tconst Lds L = lds_layout<%d>(P, lds);", mode);
+		const int lmode = mode;
+		snprintf(b, sizeof b, "\tconst Lds L = lds_layout<%d>(P, lds);", lmode);
 		line(b);
-		snprintf(b, sizeof b, "\tlds_init<%d>(P, L);", mode);
+		snprintf(b, sizeof b, "\tlds_init<%d>(P, L);", lmode);
 		line(b);
 		line("\tu32 dups_mask = 0;");
 		for (int j = 0; j < P.njoins; j++)
@@ -330,7 +338,7 @@ struct Gen {
 				// aggregate inside the last join's scope (its tail needs th / toff)
 				const std::string tp = "tp";
 				line("\t\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
-				out += consume_body(mode, last, "tp");
+				out += consume_body(lmode, last, "tp");
 				line("\t\t\t};");
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u, pay%d[u]);", j);
@@ -365,7 +373,7 @@ struct Gen {
 			line("\t\t}");
 		}
 		line("\t}");
-		snprintf(b, sizeof b, "\tblock_epilogue<%d>(P, L);", mode);
+		snprintf(b, sizeof b, "\tblock_epilogue<%d>(P, L);", lmode);
 		line(b);
 		line("}");
 		return out;
@@ -448,11 +456,8 @@ hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int
 {
 	const Plan* arg = dP;
 	void* args[] = {(void*)&arg};
-	hipError_t e = hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, 256, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
+	return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, 256, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
 		nullptr);
-	if (e != hipSuccess)
-		return e;
-	return launch_merge(P, dP, grid, s);
 }
 
 } // namespace vh

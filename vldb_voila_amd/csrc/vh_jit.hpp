@@ -19,8 +19,8 @@ static constexpr int kJitUnroll = 4;   // rows per lane per chunk of generated k
 std::string jit_source(const Plan& P, bool eager, int unroll);
 // Compile (or fetch from the cache) `src` for the current device.
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log);
-// Launch a compiled kernel over P (dP: its device copy), then the merge of
-// per-block partials where the mode has them.
+// Launch a compiled kernel over P (dP: its device copy).  GB_REG / GB_LDS
+// leave per-block partials for launch_merge.
 hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, hipStream_t s);
 
 } // namespace vh

@@ -10,8 +10,16 @@ namespace vh {
 // p[0, n) = v; with only_if, only when *only_if != 0 (read on the device)
 hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s, const u32* only_if = nullptr);
 hipError_t launch_build(const BuildPlan* dB, int grid, hipStream_t s);
+// Partitioned build of a direct table (B.nparts > 0): per-chunk LDS sort,
+// then one LDS-built slice per partition.  B: host copy; dB: device copy.
+hipError_t launch_part_build(const BuildPlan& B, const BuildPlan* dB, hipStream_t s);
+static constexpr int kMaxBuildParts = 4096;      // partitions of one build
+static constexpr int kMaxBuildChunks = 2048;     // 4096-row chunks of one build
+static constexpr int kBuildChunkRows = 4096;
+static constexpr int kMaxSliceBytes = 32768;     // LDS slice of a partition
 // P: host copy (launch shape); dP: the same plan in device memory.
-// launch_pipeline runs the generic fused kernel, then launch_merge.
+// launch_pipeline runs the generic fused kernel; GB_REG / GB_LDS then need
+// launch_merge.
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 // GB_REG / GB_LDS: merge the per-block partials of a `grid`-block launch
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s);

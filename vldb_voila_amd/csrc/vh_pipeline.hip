@@ -18,6 +18,7 @@
 // relalg_translator.cpp:582-656).
 #include "vh_device.hpp"
 #include "vh_fused.hpp"
+#include "vh_kernels.hpp"
 #include "vh_plan.hpp"
 
 namespace vh {
@@ -106,6 +107,195 @@ __global__ __launch_bounds__(kBlock) void build_kernel(const BuildPlan* __restri
 		build_chunk(B, base, B.rows);
 }
 
+// ---- partitioned build of a direct table ---------------------------------
+//
+// Device-scope atomics are the cost of a direct table built in place: every
+// row claims its entry with one, and they retire at a fixed device-wide rate
+// however small the table.  The single-query path instead partitions the
+// build rows by key range so that each partition's slice of the table fits
+// in LDS, in two launches and without global atomics:
+//   part_scatter_kernel  one workgroup per chunk of kPartChunk rows sorts
+//                        its rows by partition in LDS (counting sort), writes
+//                        them back contiguously and records, per partition,
+//                        the run's length and offset in the chunk;
+//   part_build_kernel    one workgroup per partition gathers its runs from
+//                        every chunk, claims entries in an LDS copy of its
+//                        slice (the claim rule of build_chunk, with LDS
+//                        atomics) and stores the finished slice.
+// Repeated keys still go to the overflow slots with global atomics.
+
+static constexpr int kMaxParts = 4096;
+static constexpr int kPartBlock = 1024;               // workgroup size of both launches
+static constexpr int kPartChunk = kPartBlock * kU;    // rows per scatter workgroup
+static constexpr int kMaxChunks = 2048;
+static_assert(kPartChunk == kBuildChunkRows && kMaxChunks == kMaxBuildChunks && kMaxParts == kMaxBuildParts,
+	"partitioned build limits shared with the host");
+
+// Predicates and key range of one build row; off = key - kmin.
+__device__ __forceinline__ bool build_sel(const BuildPlan& B, i64 row, u32& off)
+{
+	if (!preds_pass_row(B.preds, B.npreds, row))
+		return false;
+	const i64 key = load1(B.key, B.key_type, row);
+	if (key < B.kmin || key > B.kmax) {
+		atomicOr(B.error, 4u);
+		return false;
+	}
+	off = (u32)(key - B.kmin);
+	return true;
+}
+
+// Linear-probing insert of a repeated key into the overflow slots.
+__device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64 val)
+{
+	u64 h = voila_hash((u64)off) & B.mask;
+	for (u64 n = 0;; n++) {
+		if (n >= kMaxProbe || n > B.mask) {
+			atomicOr(B.error, 32u);
+			return;
+		}
+		if (atomicCAS(&B.slots[h], kEmpty, val) == kEmpty)
+			return;
+		h = (h + 1) & B.mask;
+	}
+}
+
+// In-place exclusive prefix sum of a[0, n) in LDS; every thread gets the
+// total.  Each thread scans a contiguous segment, then the segment sums.
+__device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* wsum)
+{
+	const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+	const int b0 = (int)threadIdx.x * per;
+	u32 t = 0;
+	for (int i = b0; i < b0 + per && i < n; i++)
+		t += a[i];
+	wsum[threadIdx.x] = t;
+	__syncthreads();
+	for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+		const u32 v = threadIdx.x >= (unsigned)d ? wsum[threadIdx.x - d] : 0u;
+		__syncthreads();
+		wsum[threadIdx.x] += v;
+		__syncthreads();
+	}
+	u32 run = wsum[threadIdx.x] - t;
+	for (int i = b0; i < b0 + per && i < n; i++) {
+		const u32 v = a[i];
+		a[i] = run;
+		run += v;
+	}
+	const u32 total = wsum[blockDim.x - 1];
+	__syncthreads();
+	return total;
+}
+
+__global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPlan* __restrict__ Bp)
+{
+	const BuildPlan& B = *Bp;
+	__shared__ u32 hist[kMaxParts];
+	__shared__ u64 staged[kPartChunk];
+	__shared__ u32 wsum[kPartBlock];
+	const int P = B.nparts;
+	for (int p = threadIdx.x; p < P; p += blockDim.x)
+		hist[p] = 0;
+	__syncthreads();
+	const i64 r0 = (i64)blockIdx.x * kPartChunk;
+	u32 off[kU], pay[kU], rank[kU];
+	bool ok[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
+		ok[u] = r < B.rows && build_sel(B, r, off[u]);
+		pay[u] = (ok[u] && B.npay) ? (u32)(load1(B.pay, B.pay_type, r) - B.pmin) : 0u;
+	}
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		rank[u] = ok[u] ? atomicAdd(&hist[off[u] >> B.pshift], 1u) : 0u;
+	__syncthreads();
+	u32* cnt = B.pcnt + (u64)blockIdx.x * P;
+	for (int p = threadIdx.x; p < P; p += blockDim.x)
+		cnt[p] = hist[p];
+	__syncthreads();
+	const u32 n = block_exclusive_scan(hist, P, wsum);
+	u32* start = B.poff + (u64)blockIdx.x * P;
+	for (int p = threadIdx.x; p < P; p += blockDim.x)
+		start[p] = hist[p];
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		if (ok[u])
+			staged[hist[off[u] >> B.pshift] + rank[u]] = (u64)off[u] | ((u64)pay[u] << 32);
+	__syncthreads();
+	u64* dst = B.pbuf + r0;
+	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+		dst[i] = staged[i];
+}
+
+__global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan* __restrict__ Bp)
+{
+	const BuildPlan& B = *Bp;
+	extern __shared__ u64 lds_words[];
+	u32* slice = (u32*)lds_words;
+	__shared__ u32 pre[kMaxChunks];
+	__shared__ u32 base[kMaxChunks];
+	__shared__ u32 wsum[kPartBlock];
+	const int p = blockIdx.x;
+	const int P = B.nparts;
+	const int G = B.nchunks;
+	const int w = B.dir_w;
+	const u64 k0 = (u64)p << B.pshift;
+	const u64 range = (u64)(B.kmax - B.kmin) + 1;
+	const u64 nkeys = ((1ull << B.pshift) < range - k0) ? (1ull << B.pshift) : range - k0;
+	const u32 words = (u32)((nkeys * (u64)w + 3) / 4);
+	for (u32 i = threadIdx.x; i < words; i += blockDim.x)
+		slice[i] = 0xFFFFFFFFu;
+	for (int b = threadIdx.x; b < G; b += blockDim.x)
+		pre[b] = B.pcnt[(u64)b * P + p];
+	__syncthreads();
+	const u32 total = block_exclusive_scan(pre, G, wsum);
+	// row i of this partition (0 <= i < total) is pbuf[base[b] + i] for the
+	// last chunk b with pre[b] <= i
+	for (int b = threadIdx.x; b < G; b += blockDim.x)
+		base[b] = (u32)b * kPartChunk + B.poff[(u64)b * P + p] - pre[b];
+	__syncthreads();
+	for (u32 i0 = 0; i0 < total; i0 += kU * blockDim.x) {
+		u64 v[kU];
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			const u32 i = i0 + u * blockDim.x + threadIdx.x;
+			v[u] = kEmpty;
+			if (i < total) {
+				int lo = 0, hi = G - 1;
+				while (lo < hi) {
+					const int mid = (lo + hi + 1) >> 1;
+					if (pre[mid] <= i)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				v[u] = B.pbuf[base[lo] + i];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			if (v[u] == kEmpty)
+				continue;
+			const u32 off = (u32)v[u];
+			u32 mask, shift;
+			u32* word = dir_word(slice, w, off - (u32)k0, mask, shift);
+			const u32 bits = ((u32)(v[u] >> 32) << shift) & mask;
+			const u32 seen = atomicAnd(word, bits | ~mask);
+			if ((seen & mask) != mask) {
+				atomicOr(word, seen & mask);
+				*B.dups = 1u;
+				overflow_insert(B, off, v[u]);
+			}
+		}
+	}
+	__syncthreads();
+	u32* dst = (u32*)((char*)B.dir + k0 * (u64)w);
+	for (u32 i = threadIdx.x; i < words; i += blockDim.x)
+		dst[i] = slice[i];
+}
+
 __global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v, const u32* only_if)
 {
 	if (only_if && *only_if == 0u)
@@ -125,6 +315,15 @@ hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s, const u32* only_if)
 	if (blocks > 4096)
 		blocks = 4096;
 	hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v, only_if);
+	return hipGetLastError();
+}
+
+hipError_t launch_part_build(const BuildPlan& B, const BuildPlan* dB, hipStream_t s)
+{
+	if (B.nchunks > 0)
+		hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)B.nchunks), dim3(kPartBlock), 0, s, dB);
+	const size_t slice = (((size_t)1 << B.pshift) * (size_t)B.dir_w + 3) / 4 * 4;
+	hipLaunchKernelGGL(part_build_kernel, dim3((unsigned)B.nparts), dim3(kPartBlock), slice, s, dB);
 	return hipGetLastError();
 }
 
@@ -166,10 +365,7 @@ hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t 
 	default: VH_LAUNCH(GB_HASH); break;
 	}
 #undef VH_LAUNCH
-	hipError_t e = hipGetLastError();
-	if (e != hipSuccess)
-		return e;
-	return launch_merge(P, dP, grid, s);
+	return hipGetLastError();
 }
 
 } // namespace vh

@@ -141,6 +141,19 @@ struct BuildPlan {
 	int pad0;
 	u32* dups;
 	u32* error;          // bit 2: key outside [kmin, kmax]
+	// partitioned build of a direct table (nparts > 0): partition p holds
+	// the keys with (key - kmin) >> pshift == p.  Chunk b (rows
+	// [b * 4096, (b + 1) * 4096)) leaves its selected rows sorted by
+	// partition in pbuf[b * 4096 ...] as (key - kmin) | payload << 32, the
+	// run of partition p being pcnt[b * nparts + p] rows from offset
+	// poff[b * nparts + p].
+	u64* pbuf;
+	u32* pcnt;
+	u32* poff;
+	int nchunks;
+	int nparts;
+	int pshift;
+	int pad1;
 };
 
 } // namespace vh
