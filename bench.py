@@ -113,11 +113,8 @@ def main():
     # (fk 4 + date 4 + price 8 B per row) + one join-table entry per row
     # that passes the date predicate (1 byte in the direct layout, an
     # 8-byte slot in the hash layout)
-    kernel, src = queries[0].kernel()
-    entry = 8
-    for w, t in ((1, "uint8_t"), (2, "uint16_t"), (4, "uint32_t")):
-        if f"(const {t}*)J.dir" in src:
-            entry = w
+    kernel, _ = queries[0].kernel()
+    entry, _ = queries[0].join(0)
     sel = int((probe["date"] <= workloads.JOIN_DATE_CUTOFF).sum())
     probe_bytes = PROBE_ROWS * 16 + sel * entry
     probe_avg_ms = float(np.mean(probe_ms))

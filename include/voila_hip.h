@@ -183,8 +183,10 @@ enum vh_kernel {
 	VH_KERNEL_AUTO = 0,      /* generated, or generic if hipRTC fails       */
 	VH_KERNEL_GENERIC = 1,
 	VH_KERNEL_JIT = 2,       /* generated; vh_query_new fails without it    */
-	VH_KERNEL_JIT_LAZY = 3   /* generated, columns after the predicates are
+	VH_KERNEL_JIT_LAZY = 3,  /* generated, columns after the predicates are
 	                            loaded only for qualifying rows             */
+	VH_KERNEL_JIT_PIPE = 4   /* generated, the next chunk's columns are
+	                            loaded while this chunk's joins resolve     */
 };
 
 typedef struct vh_config {
@@ -195,7 +197,11 @@ typedef struct vh_config {
 	int32_t join_layout;   /* enum vh_join_layout                              */
 	int32_t kernel;        /* enum vh_kernel (single queries; batches run the
 	                          generic kernel)                                  */
-	int32_t reserved[2];
+	int32_t unroll;        /* rows per lane per step of the generated kernel,
+	                          0 = 4; 1..16                                     */
+	int32_t merge;         /* dense LDS group-by, how workgroups combine: 0 =
+	                          auto, 1 = partials + a merge launch, 2 = each
+	                          workgroup adds into the result with atomics     */
 } vh_config;
 
 /* ------------------------------------------------------------------------ */
@@ -255,6 +261,9 @@ int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t
  * generates C++ per query and compiles it with g++ -O3 -shared
  * (codegen.cpp:211-395, compiler.cpp:19-55). */
 int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, size_t* len);
+/* Layout of join j's table: entry_bytes 1, 2 or 4 for a direct table
+ * (entries = key range), 8 for hash slots (entries = slots). */
+int vh_query_join(const vh_query* q, int32_t j, int32_t* entry_bytes, int64_t* entries);
 /* The generated kernel source for a pipeline without creating a query (no
  * device needed): lowering only, same buffer convention. */
 int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap,

@@ -76,7 +76,7 @@ def pipelines():
 def test_generated_source_compiles(name, desc):
     for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
         for layout in (abi.VH_LAYOUT_AUTO, abi.VH_LAYOUT_HASH):
-            for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY):
+            for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY, abi.VH_KERNEL_JIT_PIPE):
                 src = source(desc, group_mode=mode, join_layout=layout, kernel=kernel)
                 assert 'extern "C" __global__' in src
                 rc, log, size = compile_for_gfx950(src)
@@ -88,8 +88,8 @@ def test_source_reflects_the_plan():
     d = workloads.join_aggregate_pipeline(b, p).desc()
     direct = source(d)
     hashed = source(d, join_layout=abi.VH_LAYOUT_HASH)
-    assert "(const uint8_t*)J.dir" in direct and "J.slots[h[u]]" not in direct
-    assert "J.slots[h[u]]" in hashed and "J.dir" not in hashed
+    assert "uint8_t*)P.joins[0].dir" in direct
+    assert "P.joins[0].slots" in hashed and ".dir" not in hashed
     # predicate column first, all columns loaded before the predicate (eager)
     eager = source(d)
     lazy = source(d, kernel=abi.VH_KERNEL_JIT_LAZY)

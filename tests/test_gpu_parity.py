@@ -29,7 +29,7 @@ def device():
     runtime.init(0)
 
 
-@pytest.mark.parametrize("kernel", [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT])
+@pytest.mark.parametrize("kernel", [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_PIPE])
 @pytest.mark.parametrize("mode", [abi.VH_GROUP_AUTO, abi.VH_GROUP_REG] + MODES[1:])
 def test_filter_groupby_small(mode, kernel):
     print(f"filter_groupby 300k mode={mode} kernel={kernel}", flush=True)
@@ -40,7 +40,7 @@ def test_filter_groupby_small(mode, kernel):
     assert got == want
 
 
-KERNELS = [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY]
+KERNELS = [abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY, abi.VH_KERNEL_JIT_PIPE]
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -63,7 +63,7 @@ def test_generated_kernel_is_the_default():
     q = runtime.Query(p.desc())
     kernel, src = q.kernel()
     assert kernel == abi.VH_KERNEL_JIT
-    assert "vh_generated" in src and "(const uint8_t*)J.dir" in src
+    assert "vh_generated" in src and "uint8_t*)P.joins[0].dir" in src
     q.run()
     assert oracle.sorted_rows(q.result()) == oracle.sorted_rows(
         oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
@@ -289,3 +289,37 @@ def test_direct_overflow_cleared_between_runs():
         bdev["k"].copy_from(keys)
         q.run()
         assert oracle.sorted_rows(q.result()) == oracle.sorted_rows(oracle.bruteforce(make(keys).desc()))
+
+
+@pytest.mark.parametrize("merge", [1, 2])
+def test_dense_group_by_merge_modes(merge):
+    """Workgroups of the dense LDS group-by either leave partials for a merge
+    launch (1) or add into the result with 128-bit atomics (2); both are
+    exact, for both kernels and the lane-private and shared layouts."""
+    print(f"dense group-by merge={merge}", flush=True)
+    t = datagen.gen_filter_groupby(200_000, seed=31)
+    b, pr = datagen.gen_join_aggregate(20_000, 100_000, seed=32)
+    cases = [(workloads.filter_groupby_pipeline(t).desc(), [t],
+              lambda a: workloads.filter_groupby_pipeline(t, a)),
+             (workloads.join_aggregate_pipeline(b, pr).desc(), [b, pr],
+              lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))]
+    for desc, arrays, make in cases:
+        want = oracle.sorted_rows(oracle.bruteforce(desc))
+        p, devs = on_gpu(arrays, make)
+        for mode in (abi.VH_GROUP_REG, abi.VH_GROUP_LDS):
+            for kernel in (abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT):
+                for grid in (0, 37):
+                    cfg = runtime.make_config(group_mode=mode, kernel=kernel, merge=merge, grid=grid)
+                    got = oracle.sorted_rows(runtime.run_pipeline(p, cfg))
+                    assert got == want, (mode, kernel, grid)
+
+
+def test_unroll_variants_of_the_generated_kernel():
+    print("generated kernel unroll 1..16", flush=True)
+    b, pr = datagen.gen_join_aggregate(20_000, 100_003, seed=33)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    for unroll in (1, 2, 8, 16):
+        for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_PIPE):
+            cfg = runtime.make_config(kernel=kernel, unroll=unroll)
+            assert oracle.sorted_rows(runtime.run_pipeline(p, cfg)) == want, (unroll, kernel)

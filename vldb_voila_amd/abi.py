@@ -91,9 +91,10 @@ VH_KERNEL_AUTO = 0
 VH_KERNEL_GENERIC = 1
 VH_KERNEL_JIT = 2
 VH_KERNEL_JIT_LAZY = 3
+VH_KERNEL_JIT_PIPE = 4
 
 
 class vh_config(C.Structure):
     _fields_ = [("group_mode", C.c_int32), ("grid", C.c_int32), ("table_shift", C.c_int32),
                 ("tile_shift", C.c_int32), ("join_layout", C.c_int32), ("kernel", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("unroll", C.c_int32), ("merge", C.c_int32)]

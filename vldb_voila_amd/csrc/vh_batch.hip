@@ -79,6 +79,8 @@ __global__ __launch_bounds__(256) void multi_fill_kernel(const FillSeg* __restri
 {
 	for (int s = blockIdx.y; s < nseg; s += gridDim.y) {
 		const FillSeg S = segs[s];
+		if (S.only_if && *S.only_if == 0u)
+			continue;
 		for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < S.n; i += (u64)gridDim.x * blockDim.x)
 			S.p[i] = S.v;
 	}

@@ -16,12 +16,15 @@ struct BatchItem {
 	i64 r1;
 };
 
+// p[0, n) = v; with only_if, only when *only_if != 0 when the launch reads it
 struct FillSeg {
 	u64* p;
 	u64 n;
 	u64 v;
+	const u32* only_if;
 };
 
+// One launch filling every segment (the batch's and a query's clear pass).
 hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s);
 // Items [first, first + count) with the launch's own queue counter.
 hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 first,

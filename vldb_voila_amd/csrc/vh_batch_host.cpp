@@ -137,7 +137,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 			probe_items.push_back({i, 1, -1, 0, r0, std::min<i64>(r0 + T, P.rows)});
 		const u64 groups = q->L.groups;
 		const u64 ns = (u64)std::max(P.nsums, 1);
-		segs.push_back({(u64*)q->flags, 4, 0});
+		segs.push_back({(u64*)q->flags, (u64)kFlagWords / 2, 0, nullptr});
 		segs.push_back({P.out_lo, groups * ns, 0});
 		segs.push_back({(u64*)P.out_hi, groups * ns, 0});
 		segs.push_back({P.out_cnt, groups, 0});

@@ -202,6 +202,19 @@ int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t
 	return VH_OK;
 }
 
+int vh_query_join(const vh_query* q, int32_t j, int32_t* entry_bytes, int64_t* entries)
+{
+	if (!q || j < 0 || j >= q->q.L.plan.njoins)
+		return fail(VH_ERR_INVALID, "vh_query_join arguments");
+	const BuildPlan& B = q->q.L.builds[j];
+	const bool direct = B.dir_w != 0;
+	if (entry_bytes)
+		*entry_bytes = direct ? B.dir_w : 8;
+	if (entries)
+		*entries = direct ? (int64_t)(B.kmax - B.kmin + 1) : (int64_t)q->q.L.capacity[j];
+	return VH_OK;
+}
+
 int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap, size_t* len)
 {
 	if (!desc || !len)
@@ -210,7 +223,8 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 	int rc = lower(desc, cfg, L);
 	if (rc)
 		return rc;
-	const std::string src = jit_source(L.plan, !cfg || cfg->kernel != VH_KERNEL_JIT_LAZY, kJitUnroll);
+	const int kernel = cfg ? cfg->kernel : VH_KERNEL_AUTO;
+	const std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel));
 	*len = src.size();
 	if (buf) {
 		if (cap <= src.size())

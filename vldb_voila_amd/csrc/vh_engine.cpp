@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/voila_hip.h"
+#include "vh_batch.hpp"
 #include "vh_engine.hpp"
 #include "vh_jit.hpp"
 #include "vh_kernels.hpp"
@@ -545,6 +546,15 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4 * device_cus()));
 	}
 	L.grid = grid;
+	// dense LDS group-by: workgroups leave partials for merge_kernel, or add
+	// into the result with atomics.  The atomics all land on a few words at
+	// the end of the launch and cost more than the merge launch unless there
+	// are only a few thousand of them.
+	const int merge = cfg ? cfg->merge : 0;
+	if (merge < 0 || merge > 2)
+		return fail(VH_ERR_INVALID, "merge %d", merge);
+	if (P.mode == GB_REG || P.mode == GB_LDS)
+		P.flush = merge == 2 || (merge == 0 && (double)grid * domain * (P.nsums + 1) <= 4096.0);
 	L.build_grid.assign(d->njoins, 0);
 	for (int j = 0; j < d->njoins; j++)
 		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 1023) / 1024, 4096));
@@ -579,10 +589,12 @@ int query_create(Query* q, bool single)
 	Lowered& L = q->L;
 	Plan& P = L.plan;
 	int rc;
-	// flags: [0] error, [1 + j] dups of join j
-	if ((rc = q->alloc((void**)&q->flags, 16 * sizeof(u32))))
+	// flags (kFlagWords u32): [0] error bits, [2 + 2j] repeated keys in
+	// join j's build, each in a 64-bit word of its own so the clear pass can
+	// zero them separately
+	if ((rc = q->alloc((void**)&q->flags, kFlagWords * sizeof(u32))))
 		return rc;
-	HIP_TRY(hipMemset(q->flags, 0, 16 * sizeof(u32)));
+	HIP_TRY(hipMemset(q->flags, 0, kFlagWords * sizeof(u32)));
 	P.error = q->flags;
 	for (int j = 0; j < P.njoins; j++) {
 		if ((rc = q->alloc((void**)&q->slots[j], L.capacity[j] * sizeof(u64))))
@@ -591,7 +603,7 @@ int query_create(Query* q, bool single)
 		// them, so they start out empty here
 		HIP_TRY(hipMemset(q->slots[j], 0xFF, L.capacity[j] * sizeof(u64)));
 		P.joins[j].slots = q->slots[j];
-		P.joins[j].dups = q->flags + 1 + j;
+		P.joins[j].dups = dups_flag(q, j);
 		if (L.dir_words[j]) {
 			if ((rc = q->alloc((void**)&q->dir[j], L.dir_words[j] * sizeof(u64))))
 				return rc;
@@ -610,7 +622,7 @@ int query_create(Query* q, bool single)
 				return rc;
 		}
 		L.builds[j].slots = q->slots[j];
-		L.builds[j].dups = q->flags + 1 + j;
+		L.builds[j].dups = dups_flag(q, j);
 		L.builds[j].error = q->flags;
 	}
 	const u64 groups = L.groups;
@@ -630,6 +642,35 @@ int query_create(Query* q, bool single)
 		if ((rc = q->alloc((void**)&P.hkeys, groups * sizeof(u64))))
 			return rc;
 	}
+	// the clear pass of every run, one launch: join tables, atomically
+	// accumulated outputs, error bits.  A partitioned build rewrites its
+	// whole direct table and clears its own repeated-key flag
+	// (part_scatter_kernel), so only its overflow slots are cleared here, and
+	// only after a run that used them; other joins clear their flag here.
+	std::vector<FillSeg> segs;
+	segs.push_back({(u64*)q->flags, 1, 0, nullptr});
+	for (int j = 0; j < P.njoins; j++) {
+		if (L.builds[j].nparts) {
+			segs.push_back({q->slots[j], L.capacity[j], ~0ull, dups_flag(q, j)});
+		} else {
+			segs.push_back({(u64*)dups_flag(q, j), 1, 0, nullptr});
+			segs.push_back({q->slots[j], L.capacity[j], ~0ull, nullptr});
+			if (q->dir[j])
+				segs.push_back({q->dir[j], L.dir_words[j], ~0ull, nullptr});
+		}
+	}
+	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH || P.flush) {
+		segs.push_back({P.out_lo, groups * nsums, 0, nullptr});
+		segs.push_back({(u64*)P.out_hi, groups * nsums, 0, nullptr});
+		segs.push_back({P.out_cnt, groups, 0, nullptr});
+		if (P.mode == GB_HASH)
+			segs.push_back({P.hkeys, groups, ~0ull, nullptr});
+	}
+	q->nclear = (int)segs.size();
+	if ((rc = q->alloc((void**)&q->dclear, segs.size() * sizeof(FillSeg))))
+		return rc;
+	HIP_TRY(hipMemcpy(q->dclear, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
+
 	// device copies of the plans
 	if ((rc = q->alloc((void**)&q->dplan, sizeof(Plan))))
 		return rc;
@@ -641,10 +682,10 @@ int query_create(Query* q, bool single)
 	}
 	// the generated probe kernel, unless the caller asked for the generic one
 	const int kernel = q->cfg.kernel;
-	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_LAZY)
+	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_PIPE)
 		return fail(VH_ERR_INVALID, "kernel %d", kernel);
 	if (single && kernel != VH_KERNEL_GENERIC) {
-		q->jit_src = jit_source(P, kernel != VH_KERNEL_JIT_LAZY, kJitUnroll);
+		q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel));
 		std::string log;
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
 		if (rc && kernel != VH_KERNEL_AUTO)
@@ -663,31 +704,8 @@ int query_run(Query* q, hipStream_t s, int phase)
 	Plan& P = L.plan;
 	if (phase < -1 || phase > 3)
 		return fail(VH_ERR_INVALID, "phase %d", phase);
-	if (phase == -1 || phase == 0) {
-	// clear: join tables, atomically accumulated outputs, then the
-	// error/dup flags.  A direct table's overflow slots are refilled only
-	// when the last build spilled into them (its dups flag, still set here).
-	for (int j = 0; j < P.njoins; j++) {
-		if (q->dir[j]) {
-			// a partitioned build writes every entry of the table itself
-			if (!L.builds[j].nparts)
-				HIP_TRY(launch_fill(q->dir[j], L.dir_words[j], ~0ull, s));
-			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s, q->flags + 1 + j));
-		} else {
-			HIP_TRY(launch_fill(q->slots[j], L.capacity[j], ~0ull, s));
-		}
-	}
-	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH) {
-		const u64 groups = L.groups;
-		const int nsums = std::max(P.nsums, 1);
-		HIP_TRY(launch_fill(P.out_lo, groups * nsums, 0, s));
-		HIP_TRY(launch_fill((u64*)P.out_hi, groups * nsums, 0, s));
-		HIP_TRY(launch_fill(P.out_cnt, groups, 0, s));
-		if (P.mode == GB_HASH)
-			HIP_TRY(launch_fill(P.hkeys, groups, ~0ull, s));
-	}
-	HIP_TRY(launch_fill((u64*)q->flags, 4, 0, s));
-	}
+	if (phase == -1 || phase == 0)
+		HIP_TRY(launch_multi_fill(q->dclear, q->nclear, s));
 	if (phase == -1 || phase == 1) {
 		for (int j = 0; j < P.njoins; j++) {
 			if (L.builds[j].nparts)
@@ -742,7 +760,7 @@ int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
 	Plan& P = L.plan;
 	const vh_pipeline_desc& d = q->desc;
 	HIP_TRY(hipStreamSynchronize(s));
-	u32 flags[16];
+	u32 flags[kFlagWords];
 	HIP_TRY(hipMemcpy(flags, q->flags, sizeof(flags), hipMemcpyDeviceToHost));
 	if (flags[0] & 4u)
 		return fail(VH_ERR_INVALID, "a build key lies outside its column's [min, max]");
@@ -755,7 +773,7 @@ int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
 	if (flags[0] & 16u)
 		return fail(VH_ERR_UNSUPPORTED, "a JoinN before the last join matched duplicate build keys");
 	for (int j = 0; j < P.njoins - 1; j++)
-		if (d.joins[j].variant == VH_JOINN && flags[1 + j])
+		if (d.joins[j].variant == VH_JOINN && flags[2 + 2 * j])
 			return fail(VH_ERR_UNSUPPORTED,
 				"join %d: JoinN with duplicate build keys before the last join", j);
 	const u64 groups = L.groups;

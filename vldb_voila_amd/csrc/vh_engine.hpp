@@ -46,6 +46,8 @@ struct Query {
 	u32* flags = nullptr;
 	u64* slots[VH_MAX_JOINS] = {};
 	u64* dir[VH_MAX_JOINS] = {};
+	struct FillSeg* dclear = nullptr;   // segments of the per-run clear pass
+	int nclear = 0;
 	Plan* dplan = nullptr;
 	BuildPlan* dbuild[VH_MAX_JOINS] = {};
 	std::vector<void*> allocs;
@@ -56,6 +58,9 @@ struct Query {
 	~Query();
 	int alloc(void** p, size_t bytes);
 };
+
+static constexpr int kFlagWords = 2 + 2 * VH_MAX_JOINS;   // error word, then a dups word per join
+inline u32* dups_flag(Query* q, int j) { return q->flags + 2 + 2 * j; }
 
 int fail(int code, const char* fmt, ...);
 const char* last_error();

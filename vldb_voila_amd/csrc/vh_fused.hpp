@@ -561,7 +561,8 @@ __device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
 
 // End of a one-pipeline launch: GB_LDSHASH adds its LDS groups to the
 // global arrays; GB_REG / GB_LDS write the block's exact partials
-// (element-major, see merge_kernel) for the merge launch that follows.
+// (element-major, see merge_kernel) for the merge launch that follows, or
+// with P.flush add them into the global arrays with 128-bit atomics.
 template <int MODE>
 __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 {
@@ -601,8 +602,13 @@ __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 				}
 				wave_sum_i128(lo, hi);
 				if (lane == 0) {
-					mine[(2 * e) * nb] = lo;
-					mine[(2 * e + 1) * nb] = (u64)hi;
+					if (P.flush) {
+						if (lo || hi)
+							atomic_add_i128(&P.out_lo[e], &P.out_hi[e], lo, hi);
+					} else {
+						mine[(2 * e) * nb] = lo;
+						mine[(2 * e + 1) * nb] = (u64)hi;
+					}
 				}
 			} else {
 				const u64 g = (u64)e - D * P.nsums;
@@ -610,8 +616,12 @@ __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 				for (u32 t = lane; t < bd; t += kWave)
 					c += L.cnt[g * bd + t];
 				c = wave_sum_u64(c);
-				if (lane == 0)
-					mine[(2 * D * P.nsums + g) * nb] = c;
+				if (lane == 0) {
+					if (!P.flush)
+						mine[(2 * D * P.nsums + g) * nb] = c;
+					else if (c)
+						atomicAdd(&P.out_cnt[g], c);
+				}
 			}
 		}
 	} else {
@@ -619,12 +629,21 @@ __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 		for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
 			const SumArg& S = P.sums[i / D];
 			const u64 lo = L.lo[i];
-			mine[(2 * i) * nb] = lo;
-			mine[(2 * i + 1) * nb] = (P.wide && S.wide) ? (u64)(i64)L.hi[i]
-				: (S.arg_signed ? (u64)((i64)lo >> 63) : 0ull);
+			const i64 hi = (P.wide && S.wide) ? (i64)L.hi[i] : (S.arg_signed ? (i64)lo >> 63 : 0);
+			if (P.flush) {
+				if (lo || hi)
+					atomic_add_i128(&P.out_lo[i], &P.out_hi[i], lo, hi);
+			} else {
+				mine[(2 * i) * nb] = lo;
+				mine[(2 * i + 1) * nb] = (u64)hi;
+			}
 		}
-		for (u64 i = threadIdx.x; i < D; i += blockDim.x)
-			mine[(2 * n + i) * nb] = L.cnt[i];
+		for (u64 i = threadIdx.x; i < D; i += blockDim.x) {
+			if (!P.flush)
+				mine[(2 * n + i) * nb] = L.cnt[i];
+			else if (L.cnt[i])
+				atomicAdd(&P.out_cnt[i], (u64)L.cnt[i]);
+		}
 	}
 }
 

@@ -60,13 +60,38 @@ struct Col {
 struct Gen {
 	const Plan& P;
 	bool eager;
+	bool pipe;          // load the next chunk's columns while this one's joins resolve
 	int U;
 	std::vector<Col> cols;
+	// plan fields the kernel reads, copied into locals before the row loop
+	// (the loop's atomics would otherwise make the compiler reload them)
+	std::map<std::string, std::string> uni;
+	std::string decls;
 	std::string out;
 
-	Gen(const Plan& p, bool e, int u) : P(p), eager(e), U(u) {}
+	Gen(const Plan& p, bool e, bool pp, int u) : P(p), eager(e || pp), pipe(pp), U(u) {}
+	bool next_emitted = false;
 
 	void line(const std::string& s) { out += s; out += '\n'; }
+
+	// Read-only pointer into global memory.  The plan stores generic
+	// pointers; loads through them would be flat loads, which also count
+	// against the LDS counter and so can only be waited for all together.
+	static std::string gptr(const std::string& t)
+	{
+		return "const __attribute__((address_space(1))) " + t + "*";
+	}
+
+	std::string uniform(const std::string& type, const std::string& expr)
+	{
+		auto it = uni.find(expr);
+		if (it != uni.end())
+			return it->second;
+		const std::string name = "k" + std::to_string(uni.size());
+		uni[expr] = name;
+		decls += "\t" + type + " const " + name + " = " + expr + ";\n";
+		return name;
+	}
 
 	int col_index(const Role& r, const std::string& path, bool pred)
 	{
@@ -95,52 +120,19 @@ struct Gen {
 		case ROLE_COL:
 			snprintf(b, sizeof b, "((i64)c%d[u])", col_index(r, path, false));
 			return b;
-		case ROLE_PAY:
+		case ROLE_PAY: {
+			snprintf(b, sizeof b, "P.joins[%d].pmin", r.slot);
+			const std::string pmin = uniform("i64", b);
 			if (r.slot == tail_join)
-				snprintf(b, sizeof b, "((i64)%s + P.joins[%d].pmin)", tail_pay, r.slot);
-			else
-				snprintf(b, sizeof b, "((i64)pay%d[u] + P.joins[%d].pmin)", r.slot, r.slot);
+				return "((i64)" + std::string(tail_pay) + " + " + pmin + ")";
+			snprintf(b, sizeof b, "((i64)pay%d[u] + %s)", r.slot, pmin.c_str());
 			return b;
+		}
 		case ROLE_CONST:
-			return "(" + path + ".c)";
+			return "(" + uniform("i64", path + ".c") + ")";
 		default:
 			return "((i64)0)";
 		}
-	}
-
-	std::string consume_body(int mode, int tail_join, const char* tail_pay)
-	{
-		std::string s;
-		char b[512];
-		s += "\t\t\tu64 g = 0;\n\t\t\tbool ok = true;\n";
-		for (int k = 0; k < P.nkeys; k++) {
-			snprintf(b, sizeof b, "P.keys[%d]", k);
-			const std::string kp = b;
-			s += "\t\t\t{\n\t\t\t\tconst u64 o = (u64)(" + role(P.keys[k].r, kp + ".r", tail_join, tail_pay) +
-				" - " + kp + ".min);\n";
-			s += "\t\t\t\tok = ok && o <= " + kp + ".range;\n";
-			if (mode == GB_HASH)
-				s += "\t\t\t\tg += o << " + kp + ".shift;\n";
-			else
-				s += "\t\t\t\tg += o * " + kp + ".mult;\n";
-			s += "\t\t\t}\n";
-		}
-		if (mode == GB_HASH)
-			s += "\t\t\tif (!ok) { atomicOr(P.error, 8u); return; }\n";
-		else
-			s += "\t\t\tif (!ok || g >= P.domain) { atomicOr(P.error, 8u); return; }\n";
-		snprintf(b, sizeof b, "\t\t\tconst i64 slot = locate<%d>(P, L, g);\n", mode);
-		s += b;
-		if (mode == GB_HASH)
-			s += "\t\t\tif (slot < 0) return;\n";
-		snprintf(b, sizeof b, "\t\t\tadd_count<%d>(P, L, g, slot);\n", mode);
-		s += b;
-		for (int si = 0; si < P.nsums; si++) {
-			snprintf(b, sizeof b, "\t\t\tadd_sum<%d>(P, L, %d, g, slot, %s);\n", mode, si,
-				sum_expr(si, tail_join, tail_pay).c_str());
-			s += b;
-		}
-		return s;
 	}
 
 	// Argument of sum si for row u: the product of its linear terms (u64
@@ -154,7 +146,7 @@ struct Gen {
 			const Term& T = S.t[t];
 			snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
 			const std::string tp = b;
-			std::string term = "(" + tp + ".c";
+			std::string term = "(" + uniform("i64", tp + ".c");
 			if (T.sa)
 				term += (T.sa > 0 ? " + " : " - ") + role(T.a, tp + ".a", tail_join, tail_pay);
 			if (T.sb)
@@ -165,9 +157,84 @@ struct Gen {
 		return prod.empty() ? "(u64)0" : prod;
 	}
 
+	// Aggregate one row (u, with the last join's payload tp): group index,
+	// count, sums; add_count / add_sum of vh_fused.hpp with the plan's
+	// flags as literals for the dense modes.
+	std::string consume_body(int mode, int tail_join, const char* tail_pay)
+	{
+		std::string s;
+		char b[1024];
+		const std::string err = uniform("u32*", "P.error");
+		const std::string dom = uniform("u64", "P.domain");
+		s += "\t\t\tu64 g = 0;\n\t\t\tbool ok = true;\n";
+		for (int k = 0; k < P.nkeys; k++) {
+			snprintf(b, sizeof b, "P.keys[%d]", k);
+			const std::string kp = b;
+			s += "\t\t\t{\n\t\t\t\tconst u64 o = (u64)(" + role(P.keys[k].r, kp + ".r", tail_join, tail_pay) + " - " +
+				uniform("i64", kp + ".min") + ");\n";
+			s += "\t\t\t\tok = ok && o <= " + uniform("u64", kp + ".range") + ";\n";
+			if (mode == GB_HASH)
+				s += "\t\t\t\tg += o << " + uniform("int", kp + ".shift") + ";\n";
+			else
+				s += "\t\t\t\tg += o * " + uniform("u64", kp + ".mult") + ";\n";
+			s += "\t\t\t}\n";
+		}
+		if (mode == GB_HASH)
+			s += "\t\t\tif (!ok) { atomicOr(" + err + ", 8u); return; }\n";
+		else
+			s += "\t\t\tif (!ok || g >= " + dom + ") { atomicOr(" + err + ", 8u); return; }\n";
+		const bool dense = mode == GB_REG || mode == GB_LDS || mode == GB_GLOBAL;
+		if (!dense) {
+			snprintf(b, sizeof b, "\t\t\tconst i64 slot = locate<%d>(P, L, g);\n", mode);
+			s += b;
+			if (mode == GB_HASH)
+				s += "\t\t\tif (slot < 0) return;\n";
+			snprintf(b, sizeof b, "\t\t\tadd_count<%d>(P, L, g, slot);\n", mode);
+			s += b;
+		} else if (mode == GB_REG) {
+			s += "\t\t\tL.cnt[g * blockDim.x + threadIdx.x] += 1u;\n";
+		} else if (mode == GB_LDS) {
+			s += "\t\t\tatomicAdd(&L.cnt[g], 1u);\n";
+		} else {
+			s += "\t\t\tatomicAdd(&" + uniform("u64*", "P.out_cnt") + "[g], 1ull);\n";
+		}
+		for (int si = 0; si < P.nsums; si++) {
+			const SumArg& S = P.sums[si];
+			const std::string v = sum_expr(si, tail_join, tail_pay);
+			const bool track = P.wide && S.wide;
+			const bool exact = P.wide && S.res128;
+			const char* sgn = S.arg_signed ? "true" : "false";
+			if (!dense) {
+				snprintf(b, sizeof b, "\t\t\tadd_sum<%d>(P, L, %d, g, slot, %s);\n", mode, si, v.c_str());
+				s += b;
+			} else if (mode == GB_REG) {
+				snprintf(b, sizeof b,
+					"\t\t\t{\n\t\t\t\tconst u64 v = %s;\n\t\t\t\tconst u64 idx = (g * %d + %d) * blockDim.x + threadIdx.x;\n"
+					"\t\t\t\tconst u64 o = L.lo[idx];\n\t\t\t\tconst u64 n = o + v;\n\t\t\t\tL.lo[idx] = n;\n",
+					v.c_str(), P.nsums, si);
+				s += b;
+				if (track)
+					s += std::string("\t\t\t\tL.hi[idx] += (int)(n < o) - (int)(") + sgn + " && (i64)v < 0);\n";
+				s += "\t\t\t}\n";
+			} else if (mode == GB_LDS) {
+				snprintf(b, sizeof b, "\t\t\t{\n\t\t\t\tconst u64 idx = (u64)%d * %s + g;\n"
+					"\t\t\t\tadd_lds_exact(&L.lo[idx], &L.hi[idx], %s, %s, %s);\n\t\t\t}\n",
+					si, dom.c_str(), v.c_str(), track ? "true" : "false", sgn);
+				s += b;
+			} else {
+				snprintf(b, sizeof b, "\t\t\t{\n\t\t\t\tconst u64 idx = (u64)%d * %s + g;\n"
+					"\t\t\t\tadd_global(&%s[idx], &%s[idx], %s, %s, %s);\n\t\t\t}\n",
+					si, dom.c_str(), uniform("u64*", "P.out_lo").c_str(), uniform("i64*", "P.out_hi").c_str(),
+					v.c_str(), exact ? "true" : "false", sgn);
+				s += b;
+			}
+		}
+		return s;
+	}
+
 	std::string source(int mode)
 	{
-		char b[512];
+		char b[1024];
 		// register every probe column: predicates first, then join keys,
 		// group keys and sum arguments
 		for (int p = 0; p < P.npreds; p++) {
@@ -194,40 +261,54 @@ struct Gen {
 
 		const int last = P.njoins - 1;
 		const bool tail = P.njoins > 0 && P.joins[last].variant == VH_JOINN;
+		const std::string err = uniform("u32*", "P.error");
 
-		line("// generated probe kernel");
-		line("typedef __hip_internal::int8_t int8_t;");
-		line("typedef __hip_internal::uint8_t uint8_t;");
-		line("typedef __hip_internal::int16_t int16_t;");
-		line("typedef __hip_internal::uint16_t uint16_t;");
-		line("typedef __hip_internal::int32_t int32_t;");
-		line("typedef __hip_internal::uint32_t uint32_t;");
-		line("typedef __hip_internal::int64_t int64_t;");
-		line("typedef __hip_internal::uint64_t uint64_t;");
-		line("#include \"vh_fused.hpp\"");
-		line("using namespace vh;");
-		snprintf(b, sizeof b, "static constexpr int U = %d;", U);
-		line(b);
-		line("extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)");
-		line("{");
-		line("\tconst Plan& P = *Pp;");
-		line("\textern __shared__ u64 lds[];");
-		const int lmode = mode;
-		snprintf(b, sizeof b, "\tconst Lds L = lds_layout<%d>(P, lds);", lmode);
-		line(b);
-		snprintf(b, sizeof b, "\tlds_init<%d>(P, L);", lmode);
-		line(b);
-		line("\tu32 dups_mask = 0;");
-		for (int j = 0; j < P.njoins; j++)
-			if (P.joins[j].variant == VH_JOINN) {
-				snprintf(b, sizeof b, "\tif (*P.joins[%d].dups) dups_mask |= %uu;", j, 1u << j);
+		// ---- the row loop (declarations of hoisted plan fields go first) ----
+		// Loads are unconditional so they issue back to back: rows that do
+		// not qualify read row 0 (a line every such lane shares) and their
+		// value is discarded.  A load under a per-lane branch would be
+		// followed by a wait before the branch rejoins.
+		auto col_ptr = [&](int i) {
+			const std::string ty = ctype_of(cols[i].type);
+			return uniform(gptr(ty), "(" + gptr(ty) + ")" + cols[i].path);
+		};
+		if (pipe) {
+			// software pipeline: chunk i+1's columns are requested after chunk
+			// i's first join lookups, so their latency overlaps that lookup and
+			// the aggregation (loads retire in order: the lookup is waited for
+			// without waiting for them)
+			line("\ti64 base = (i64)blockIdx.x * tile;");
+			for (size_t i = 0; i < cols.size(); i++) {
+				snprintf(b, sizeof b, "\t%s c%zu[U];", ctype_of(cols[i].type), i);
+				line(b);
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\tfor (int u = 0; u < U; u++) { const i64 r = base + (i64)u * blockDim.x + "
+					"threadIdx.x; c%zu[u] = %s[r < rows ? r : 0]; }", i, col_ptr((int)i).c_str());
 				line(b);
 			}
-		line("\t(void)dups_mask;");
-		line("\t__syncthreads();");
-		line("\tconst i64 rows = P.rows;");
-		line("\tconst i64 tile = (i64)U * blockDim.x;");
-		line("\tfor (i64 base = (i64)blockIdx.x * tile; base < rows; base += (i64)gridDim.x * tile) {");
+			line("\tfor (; base < rows; base += (i64)gridDim.x * tile) {");
+		} else {
+			line("\tfor (i64 base = (i64)blockIdx.x * tile; base < rows; base += (i64)gridDim.x * tile) {");
+		}
+		auto emit_next = [&]() {
+			if (!pipe || next_emitted)
+				return;
+			next_emitted = true;
+			line("\t\t__builtin_amdgcn_sched_barrier(0);");
+			line("\t\tconst i64 nbase = base + (i64)gridDim.x * tile;");
+			for (size_t i = 0; i < cols.size(); i++) {
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) { const i64 r = nbase + (i64)u * blockDim.x + "
+					"threadIdx.x; n%zu[u] = %s[r < rows ? r : 0]; }", i, col_ptr((int)i).c_str());
+				line(b);
+			}
+			line("\t\t__builtin_amdgcn_sched_barrier(0);");
+		};
+		if (pipe)
+			for (size_t i = 0; i < cols.size(); i++) {
+				snprintf(b, sizeof b, "\t\t%s n%zu[U];", ctype_of(cols[i].type), i);
+				line(b);
+			}
 		line("\t\ti64 row[U];");
 		line("\t\tbool alive[U];");
 		line("#pragma unroll");
@@ -235,75 +316,81 @@ struct Gen {
 		line("\t\t\trow[u] = base + (i64)u * blockDim.x + threadIdx.x;");
 		line("\t\t\talive[u] = row[u] < rows;");
 		line("\t\t}");
-		// column loads: predicate columns, then (eager) the rest
-		auto load = [&](int i, const char* guard) {
-			const Col& c = cols[i];
-			snprintf(b, sizeof b, "\t\t%s c%d[U];", ctype_of(c.type), i);
+		auto load = [&](int i) {
+			if (pipe)
+				return;   // loaded by the previous iteration (or before the loop)
+			const std::string ty = ctype_of(cols[i].type);
+			snprintf(b, sizeof b, "\t\t%s c%d[U];", ty.c_str(), i);
 			line(b);
 			line("#pragma unroll");
-			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%d[u] = %s ? ((const %s*)%s)[row[u]] : (%s)0;", i,
-				guard, ctype_of(c.type), c.path.c_str(), ctype_of(c.type));
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%d[u] = %s[alive[u] ? row[u] : 0];", i,
+				col_ptr(i).c_str());
 			line(b);
 		};
 		for (size_t i = 0; i < cols.size(); i++)
 			if (cols[i].pred)
-				load((int)i, "alive[u]");
+				load((int)i);
 		if (eager)
 			for (size_t i = 0; i < cols.size(); i++)
 				if (!cols[i].pred)
-					load((int)i, "alive[u]");
+					load((int)i);
 		for (int p = 0; p < P.npreds; p++) {
 			snprintf(b, sizeof b, "P.preds[%d].r", p);
 			const std::string v = role(P.preds[p].r, b);
+			snprintf(b, sizeof b, "P.preds[%d].value", p);
+			const std::string val = uniform("i64", b);
 			line("#pragma unroll");
-			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) alive[u] = alive[u] && compare(%d, %d, %s, P.preds[%d].value);",
-				P.preds[p].cmp, P.preds[p].r.type, v.c_str(), p);
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) alive[u] = alive[u] && compare(%d, %d, %s, %s);",
+				P.preds[p].cmp, P.preds[p].r.type, v.c_str(), val.c_str());
 			line(b);
 		}
 		if (!eager)
 			for (size_t i = 0; i < cols.size(); i++)
 				if (!cols[i].pred)
-					load((int)i, "alive[u]");
+					load((int)i);
 
 		// joins
 		for (int j = 0; j < P.njoins; j++) {
 			const JoinT& J = P.joins[j];
+			snprintf(b, sizeof b, "P.joins[%d]", j);
+			const std::string jp = b;
+			const std::string kmin = uniform("i64", jp + ".kmin"), kmax = uniform("i64", jp + ".kmax");
+			const std::string mask = uniform("u64", jp + ".mask");
+			const std::string slots = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".slots");
 			snprintf(b, sizeof b, "\t\tu32 pay%d[U];", j);
 			line(b);
 			line("\t\t{");
-			snprintf(b, sizeof b, "\t\t\tconst JoinT& J = P.joins[%d];", j);
-			line(b);
 			line("\t\t\tu32 off[U];");
 			line("#pragma unroll");
 			line("\t\t\tfor (int u = 0; u < U; u++) {");
-			snprintf(b, sizeof b, "P.joins[%d].key", j);
-			line("\t\t\t\tconst i64 key = " + role(J.key, b) + ";");
-			line("\t\t\t\tif (key < J.kmin || key > J.kmax) alive[u] = false;");
-			line("\t\t\t\toff[u] = (u32)(key - J.kmin);");
+			line("\t\t\t\tconst i64 key = " + role(J.key, jp + ".key") + ";");
+			line("\t\t\t\tif (key < " + kmin + " || key > " + kmax + ") alive[u] = false;");
+			line("\t\t\t\toff[u] = alive[u] ? (u32)(key - " + kmin + ") : 0u;");
 			line("\t\t\t}");
 			if (J.dir_w) {
 				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
 				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
+				const std::string dir = uniform(gptr(et), "(" + gptr(et) + ")" + jp + ".dir");
 				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = alive[u] ? (u32)((const %s*)J.dir)[off[u]] : %s;",
-					j, et, ev);
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = (u32)%s[off[u]];", j, dir.c_str());
 				line(b);
+				if (j == 0)
+					emit_next();
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (pay%d[u] == %s) alive[u] = false;", j, ev);
 				line(b);
 				if (j == last && tail) {
 					// repeated keys sit in the overflow slots from voila_hash(off) on
 					line("\t\t\tu64 th[U];");
-					line("\t\t\tu32 toff[U];");
 					line("#pragma unroll");
-					line("\t\t\tfor (int u = 0; u < U; u++) { th[u] = voila_hash((u64)off[u]) & J.mask; toff[u] = off[u]; }");
+					line("\t\t\tfor (int u = 0; u < U; u++) th[u] = voila_hash((u64)off[u]) & " + mask + ";");
 				}
 			} else {
 				line("\t\t\tu64 h[U], s[U];");
 				line("#pragma unroll");
-				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & J.mask;");
+				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + mask + ";");
 				line("#pragma unroll");
-				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = alive[u] ? J.slots[h[u]] : kEmpty;");
+				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = " + slots + "[h[u]];");
 				line("#pragma unroll");
 				line("\t\t\tfor (int u = 0; u < U; u++) {");
 				snprintf(b, sizeof b, "\t\t\t\tpay%d[u] = 0;", j);
@@ -312,19 +399,18 @@ struct Gen {
 				line("\t\t\t\tu64 n = 0;");
 				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {");
 				line("\t\t\t\t\tn++;");
-				line("\t\t\t\t\th[u] = (h[u] + 1) & J.mask;");
-				line("\t\t\t\t\ts[u] = J.slots[h[u]];");
+				line("\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
+				line("\t\t\t\t\ts[u] = " + slots + "[h[u]];");
 				line("\t\t\t\t}");
-				line("\t\t\t\tif (n >= kMaxProbe) atomicOr(P.error, 32u);");
+				line("\t\t\t\tif (n >= kMaxProbe) atomicOr(" + err + ", 32u);");
 				line("\t\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) alive[u] = false;");
 				snprintf(b, sizeof b, "\t\t\t\telse pay%d[u] = (u32)(s[u] >> 32);", j);
 				line(b);
 				line("\t\t\t}");
 				if (j == last && tail) {
 					line("\t\t\tu64 th[U];");
-					line("\t\t\tu32 toff[U];");
 					line("#pragma unroll");
-					line("\t\t\tfor (int u = 0; u < U; u++) { th[u] = (h[u] + 1) & J.mask; toff[u] = off[u]; }");
+					line("\t\t\tfor (int u = 0; u < U; u++) th[u] = (h[u] + 1) & " + mask + ";");
 				}
 			}
 			if (j != last && J.variant == VH_JOINN) {
@@ -332,13 +418,13 @@ struct Gen {
 				snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u)", j);
 				line(b);
 				line("#pragma unroll");
-				line("\t\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) atomicOr(P.error, 16u);");
+				line("\t\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) atomicOr(" + err + ", 16u);");
 			}
 			if (j == last) {
-				// aggregate inside the last join's scope (its tail needs th / toff)
-				const std::string tp = "tp";
+				emit_next();
+				// aggregate inside the last join's scope (its tail needs th / off)
 				line("\t\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
-				out += consume_body(lmode, last, "tp");
+				out += consume_body(mode, last, "tp");
 				line("\t\t\t};");
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u, pay%d[u]);", j);
@@ -351,11 +437,11 @@ struct Gen {
 					line("\t\t\t\t\tif (!alive[u]) continue;");
 					line("\t\t\t\t\tu64 hh = th[u];");
 					line("\t\t\t\t\tfor (u64 n = 0;; n++) {");
-					line("\t\t\t\t\t\tif (n >= kMaxProbe) { atomicOr(P.error, 32u); break; }");
-					line("\t\t\t\t\t\tconst u64 sv = J.slots[hh];");
+					line("\t\t\t\t\t\tif (n >= kMaxProbe) { atomicOr(" + err + ", 32u); break; }");
+					line("\t\t\t\t\t\tconst u64 sv = " + slots + "[hh];");
 					line("\t\t\t\t\t\tif (sv == kEmpty) break;");
-					line("\t\t\t\t\t\tif ((u32)sv == toff[u]) consume(u, (u32)(sv >> 32));");
-					line("\t\t\t\t\t\thh = (hh + 1) & J.mask;");
+					line("\t\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
+					line("\t\t\t\t\t\thh = (hh + 1) & " + mask + ";");
 					line("\t\t\t\t\t}");
 					line("\t\t\t\t}");
 					line("\t\t\t}");
@@ -364,6 +450,7 @@ struct Gen {
 			line("\t\t}");
 		}
 		if (P.njoins == 0) {
+			emit_next();
 			line("\t\t{");
 			line("\t\t\tauto consume = [&](int u) __attribute__((always_inline)) {");
 			out += consume_body(mode, -1, nullptr);
@@ -372,11 +459,36 @@ struct Gen {
 			line("\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u);");
 			line("\t\t}");
 		}
+		if (pipe)
+			for (size_t i = 0; i < cols.size(); i++) {
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%zu[u] = n%zu[u];", i, i);
+				line(b);
+			}
 		line("\t}");
-		snprintf(b, sizeof b, "\tblock_epilogue<%d>(P, L);", lmode);
-		line(b);
-		line("}");
-		return out;
+		const std::string loop = out;
+
+		// ---- assemble ----
+		std::string k;
+		k += "// generated probe kernel\n";
+		for (const char* t : {"int8_t", "uint8_t", "int16_t", "uint16_t", "int32_t", "uint32_t", "int64_t", "uint64_t"})
+			k += std::string("typedef __hip_internal::") + t + " " + t + ";\n";
+		k += "#include \"vh_fused.hpp\"\nusing namespace vh;\n";
+		k += "static constexpr int U = " + std::to_string(U) + ";\n";
+		k += "extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)\n{\n";
+		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
+		k += "\tconst Lds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
+		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
+		k += "\tu32 dups_mask = 0;\n";
+		for (int j = 0; j < P.njoins; j++)
+			if (P.joins[j].variant == VH_JOINN)
+				k += "\tif (*P.joins[" + std::to_string(j) + "].dups) dups_mask |= " + std::to_string(1u << j) + "u;\n";
+		k += "\t(void)dups_mask;\n";
+		k += decls;
+		k += "\t__syncthreads();\n\tconst i64 rows = P.rows;\n\tconst i64 tile = (i64)U * blockDim.x;\n";
+		k += loop;
+		k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
+		return k;
 	}
 };
 
@@ -401,9 +513,9 @@ std::string device_arch()
 
 } // namespace
 
-std::string jit_source(const Plan& P, bool eager, int unroll)
+std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline)
 {
-	Gen g(P, eager, unroll);
+	Gen g(P, eager, pipeline, unroll);
 	return g.source(P.mode);
 }
 

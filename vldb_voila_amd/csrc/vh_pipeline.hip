@@ -128,6 +128,7 @@ static constexpr int kMaxParts = 4096;
 static constexpr int kPartBlock = 1024;               // workgroup size of both launches
 static constexpr int kPartChunk = kPartBlock * kU;    // rows per scatter workgroup
 static constexpr int kMaxChunks = 2048;
+static_assert(kMaxParts <= kWave * 64 && kMaxChunks <= kWave * 64, "block_exclusive_scan bound");
 static_assert(kPartChunk == kBuildChunkRows && kMaxChunks == kMaxBuildChunks && kMaxParts == kMaxBuildParts,
 	"partitioned build limits shared with the host");
 
@@ -160,32 +161,36 @@ __device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64
 	}
 }
 
-// In-place exclusive prefix sum of a[0, n) in LDS; every thread gets the
-// total.  Each thread scans a contiguous segment, then the segment sums.
-__device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* wsum)
+// In-place exclusive prefix sum of a[0, n) in LDS (n <= kWave * 64) by the
+// first wave: each lane sums a contiguous segment, a wave scan of the
+// segment sums, then each lane rewrites its segment.  One barrier; every
+// thread gets the total (through *total, in LDS).
+__device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* total)
 {
-	const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
-	const int b0 = (int)threadIdx.x * per;
-	u32 t = 0;
-	for (int i = b0; i < b0 + per && i < n; i++)
-		t += a[i];
-	wsum[threadIdx.x] = t;
-	__syncthreads();
-	for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-		const u32 v = threadIdx.x >= (unsigned)d ? wsum[threadIdx.x - d] : 0u;
-		__syncthreads();
-		wsum[threadIdx.x] += v;
-		__syncthreads();
+	if (threadIdx.x < (unsigned)kWave) {
+		const int lane = threadIdx.x;
+		const int per = (n + kWave - 1) / kWave;
+		const int b0 = lane * per;
+		u32 t = 0;
+		for (int i = b0; i < b0 + per && i < n; i++)
+			t += a[i];
+		u32 x = t;
+		for (int d = 1; d < kWave; d <<= 1) {
+			const u32 y = __shfl_up(x, d, kWave);
+			if (lane >= d)
+				x += y;
+		}
+		u32 run = x - t;
+		for (int i = b0; i < b0 + per && i < n; i++) {
+			const u32 v = a[i];
+			a[i] = run;
+			run += v;
+		}
+		if (lane == kWave - 1)
+			*total = x;
 	}
-	u32 run = wsum[threadIdx.x] - t;
-	for (int i = b0; i < b0 + per && i < n; i++) {
-		const u32 v = a[i];
-		a[i] = run;
-		run += v;
-	}
-	const u32 total = wsum[blockDim.x - 1];
 	__syncthreads();
-	return total;
+	return *total;
 }
 
 __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPlan* __restrict__ Bp)
@@ -193,8 +198,12 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	const BuildPlan& B = *Bp;
 	__shared__ u32 hist[kMaxParts];
 	__shared__ u64 staged[kPartChunk];
-	__shared__ u32 wsum[kPartBlock];
+	__shared__ u32 total;
 	const int P = B.nparts;
+	// the repeated-key flag of this build: set (if at all) by
+	// part_build_kernel, which runs after every workgroup of this launch
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		*B.dups = 0u;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
 		hist[p] = 0;
 	__syncthreads();
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
 		cnt[p] = hist[p];
 	__syncthreads();
-	const u32 n = block_exclusive_scan(hist, P, wsum);
+	const u32 n = block_exclusive_scan(hist, P, &total);
 	u32* start = B.poff + (u64)blockIdx.x * P;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
 		start[p] = hist[p];
@@ -236,7 +245,8 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 	u32* slice = (u32*)lds_words;
 	__shared__ u32 pre[kMaxChunks];
 	__shared__ u32 base[kMaxChunks];
-	__shared__ u32 wsum[kPartBlock];
+	__shared__ u32 total_rows;
+	constexpr int V = 2 * kU;   // rows gathered per thread per step
 	const int p = blockIdx.x;
 	const int P = B.nparts;
 	const int G = B.nchunks;
@@ -247,19 +257,21 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 	const u32 words = (u32)((nkeys * (u64)w + 3) / 4);
 	for (u32 i = threadIdx.x; i < words; i += blockDim.x)
 		slice[i] = 0xFFFFFFFFu;
-	for (int b = threadIdx.x; b < G; b += blockDim.x)
+	for (int b = threadIdx.x; b < G; b += blockDim.x) {
 		pre[b] = B.pcnt[(u64)b * P + p];
+		base[b] = B.poff[(u64)b * P + p];
+	}
 	__syncthreads();
-	const u32 total = block_exclusive_scan(pre, G, wsum);
+	const u32 total = block_exclusive_scan(pre, G, &total_rows);
 	// row i of this partition (0 <= i < total) is pbuf[base[b] + i] for the
 	// last chunk b with pre[b] <= i
 	for (int b = threadIdx.x; b < G; b += blockDim.x)
-		base[b] = (u32)b * kPartChunk + B.poff[(u64)b * P + p] - pre[b];
+		base[b] = (u32)b * kPartChunk + base[b] - pre[b];
 	__syncthreads();
-	for (u32 i0 = 0; i0 < total; i0 += kU * blockDim.x) {
-		u64 v[kU];
+	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
+		u64 v[V];
 #pragma unroll
-		for (int u = 0; u < kU; u++) {
+		for (int u = 0; u < V; u++) {
 			const u32 i = i0 + u * blockDim.x + threadIdx.x;
 			v[u] = kEmpty;
 			if (i < total) {
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 			}
 		}
 #pragma unroll
-		for (int u = 0; u < kU; u++) {
+		for (int u = 0; u < V; u++) {
 			if (v[u] == kEmpty)
 				continue;
 			const u32 off = (u32)v[u];
@@ -346,7 +358,7 @@ size_t pipeline_lds_bytes(const Plan& P)
 
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s)
 {
-	if (P.mode != GB_REG && P.mode != GB_LDS)
+	if ((P.mode != GB_REG && P.mode != GB_LDS) || P.flush)
 		return hipSuccess;
 	const u64 elems = P.domain * (u64)P.nsums + P.domain;
 	hipLaunchKernelGGL(merge_kernel, dim3((unsigned)elems), dim3(kBlock), 0, s, dP, (u32)grid);

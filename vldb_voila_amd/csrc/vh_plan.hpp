@@ -105,6 +105,9 @@ struct Plan {
 	int nsums;
 	int wide;            // some sum needs exact 128-bit bookkeeping (see SumArg)
 	int mode;            // enum GroupMode
+	int flush;           // GB_REG / GB_LDS: workgroups add into out_* with
+	                     // atomics instead of leaving partials to merge
+	int pad_flush;
 	Pred preds[VH_MAX_PREDS];
 	JoinT joins[VH_MAX_JOINS];
 	KeyT keys[VH_MAX_KEYS];

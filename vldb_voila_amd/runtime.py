@@ -48,6 +48,7 @@ def lib():
         L.vh_query_free.argtypes = [vp]
         L.vh_query_info.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
         L.vh_query_kernel.argtypes = [vp, C.POINTER(i32), C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.vh_query_join.argtypes = [vp, i32, C.POINTER(i32), C.POINTER(i64)]
         L.vh_type_from_minmax.argtypes = [C.c_double, C.c_double]
         _LIB = L
     return _LIB
@@ -161,7 +162,7 @@ class Event:
             pass
 
 
-def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0):
+def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0, unroll=0, merge=0):
     c = abi.vh_config()
     c.group_mode = group_mode
     c.grid = grid
@@ -169,6 +170,8 @@ def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=
     c.tile_shift = tile_shift
     c.join_layout = join_layout
     c.kernel = kernel
+    c.unroll = unroll
+    c.merge = merge
     return c
 
 
@@ -204,6 +207,12 @@ class Query:
         m, g, n = C.c_int32(), C.c_int32(), C.c_int64()
         check(lib().vh_query_info(self.ptr, C.byref(m), C.byref(g), C.byref(n)))
         return {"group_mode": m.value, "grid": g.value, "groups": n.value}
+
+    def join(self, j=0):
+        """(entry bytes, entries) of join j's table: 1/2/4 direct, 8 hash."""
+        w, n = C.c_int32(), C.c_int64()
+        check(lib().vh_query_join(self.ptr, j, C.byref(w), C.byref(n)))
+        return w.value, n.value
 
     def kernel(self):
         """(enum vh_kernel in use, generated HIP source or '')."""
