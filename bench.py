@@ -1,19 +1,32 @@
-"""Benchmark: single hash join + 5-group aggregate on one MI355X.
+"""Benchmark of the VOILA operator path on one MI355X.
 
-Workload (seed 42): build 1,500,000 rows (unique int32 keys 1..1.5M permuted,
-int32 nation U[0,25), int32 segment U[0,5)); probe 6,000,000 rows (int32 fk
-uniform over the build keys, int32 date over 2,526 days with a predicate
-keeping 50%, int64 price U[90000, 10500000)).  Output: sum(price) grouped by
-the build side's segment, int64, checked bit-exact against the CPU oracle.
+    python bench.py [--workload join|filter|star|batch] [--steps K] [--warmup W]
 
-A step is one full run of the pipeline on device-resident inputs: clear the
-join table, build it, then the fused select + probe + aggregate kernel.  The
-inputs rotate over enough copies that the working set exceeds the 256 MB
+join (default, the headline configuration): build 1,500,000 rows (unique
+  int32 keys 1..1.5M permuted, int32 nation U[0,25), int32 segment U[0,5));
+  probe 6,000,000 rows (int32 fk uniform over the build keys, int32 date over
+  2,526 days with a predicate keeping 50%, int64 price U[90000, 10500000));
+  sum(price) grouped by the build side's segment.
+filter: 6,000,000 rows, date predicate keeping ~96%, group by (flag,
+  status): sum(qty), sum(price), sum(price * (100 - disc)) as int128, count.
+star: fact 60,000,000 rows x date / customer (region 2) / supplier
+  (region 2) / part, FKs uniform or (--zipf 1.0) Zipf on part; group by
+  (c_nation, s_nation, year): sum(revenue - cost), count.
+batch: 256 uneven select -> join -> group-by pipelines in one persistent
+  launch replayed from a hipGraph (the same work as plain launches one
+  pipeline after another is timed beside it).
+
+All inputs are synthetic (seed 42, counter-based splitmix64).  A step is one
+full run on device-resident inputs: clear, build every join table, the fused
+select + probe + aggregate kernel, the merge of per-workgroup partials.
+Inputs rotate over enough copies that the working set exceeds the 256 MB
 Infinity Cache, so the figures are HBM figures.
 
-Prints one JSON line (the driver's contract) with `roofline` (probe kernel,
-HIP events on its stream) and `cpu_baseline` (the oracle's vector(1024)
-restatement on the host cores, a bounded sample).
+Prints one JSON line: `roofline` is the probe kernel (HIP events on its
+stream around its launch inside the timed region); `cpu_baseline` times the
+oracle's vector(1024) restatement on the host cores over a bounded sample,
+and its rows double as the check of the GPU result (the run fails on a
+difference).
 """
 from __future__ import annotations
 
@@ -27,8 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BUILD_ROWS = 1_500_000
-PROBE_ROWS = 6_000_000
+METRIC = "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend"
 
 
 def parse():
@@ -36,12 +48,64 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["join", "filter", "star", "batch"], default="join")
+    ap.add_argument("--zipf", type=float, default=0.0, help="star: Zipf theta of the part FKs")
     ap.add_argument("--copies", type=int, default=0, help="input copies rotated (0 = auto)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="budget of the CPU baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores, at most 16")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and so the result check)")
     return ap.parse_args()
+
+
+# ---------------------------------------------------------------------------
+# workloads: host arrays, a pipeline factory over device addresses, and the
+# probe kernel's algorithmic bytes per launch
+# ---------------------------------------------------------------------------
+
+def workload_join(datagen, workloads):
+    build, probe = datagen.gen_join_aggregate(1_500_000, 6_000_000, seed=42)
+    sel = int((probe["date"] <= workloads.JOIN_DATE_CUTOFF).sum())
+
+    def make(addrs=None):
+        a = addrs or [None, None]
+        return workloads.join_aggregate_pipeline(build, probe, a[0], a[1])
+
+    def probe_bytes(q):
+        # fk 4 + date 4 + price 8 B per probe row, one join-table entry per
+        # row that passes the date predicate
+        entry, _ = q.join(0)
+        return 6_000_000 * 16 + sel * entry
+    return {"name": "hash join 1.5M build x 6M probe (50% date filter) + sum(price) by segment (5 groups)",
+            "tables": [build, probe], "make": make, "tuples": 7_500_000, "probe_bytes": probe_bytes,
+            "config": {"build_rows": 1_500_000, "probe_rows": 6_000_000}}
+
+
+def workload_filter(datagen, workloads):
+    t = datagen.gen_filter_groupby(6_000_000, seed=42)
+
+    def make(addrs=None):
+        return workloads.filter_groupby_pipeline(t, addrs[0] if addrs else None)
+    # flag 4 + status 4 + qty 8 + price 8 + disc 8 + date 4 B per row
+    return {"name": "filter (96%) + group by (flag, status): 3 sums (one int128) + count, 6M rows",
+            "tables": [t], "make": make, "tuples": 6_000_000,
+            "probe_bytes": lambda q: 6_000_000 * 36, "config": {"rows": 6_000_000}}
+
+
+def workload_star(datagen, workloads, theta):
+    fact, date, cust, supp, part = datagen.gen_star(60_000_000, zipf_part=theta, seed=42)
+    dims = sum(len(next(iter(t.values()))) for t in (date, cust, supp, part))
+
+    def make(addrs=None):
+        a = addrs or [None] * 5
+        return workloads.star_pipeline(fact, date, cust, supp, part,
+                                       {"fact": a[0], "date": a[1], "customer": a[2], "supplier": a[3],
+                                        "part": a[4]})
+    # the four int32 FKs and two int64 measures of every fact row
+    return {"name": f"star join: fact 60M x date, customer, supplier (region filters), part; "
+                    f"175 groups; part FKs Zipf theta={theta}",
+            "tables": [fact, date, cust, supp, part], "make": make, "tuples": 60_000_000 + dims,
+            "probe_bytes": lambda q: 60_000_000 * 32,
+            "config": {"fact_rows": 60_000_000, "dimension_rows": dims, "zipf_theta": theta}}
 
 
 def main():
@@ -50,42 +114,41 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every GPU box has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import numpy as np
 
     from vldb_voila_amd import datagen, runtime, workloads
 
-    build, probe = datagen.gen_join_aggregate(BUILD_ROWS, PROBE_ROWS, seed=42)
-    bytes_per_copy = sum(a.nbytes for a in build.values()) + sum(a.nbytes for a in probe.values())
-    copies = args.copies or max(2, -(-768 * 2**20 // bytes_per_copy))
+    if args.workload == "join":
+        W = workload_join(datagen, workloads)
+    elif args.workload == "filter":
+        W = workload_filter(datagen, workloads)
+    else:
+        W = workload_star(datagen, workloads, args.zipf)
+    bytes_per_copy = sum(a.nbytes for t in W["tables"] for a in t.values())
+    copies = args.copies or max(1, -(-768 * 2**20 // bytes_per_copy))
 
     runtime.init(0)
     stream = runtime.Stream()
     queries, bufs = [], []
     for c in range(copies):
-        db, dp = runtime.upload(build), runtime.upload(probe)
-        bufs.append((db, dp))
-        pipe = workloads.join_aggregate_pipeline(build, probe, runtime.addrs(db), runtime.addrs(dp))
-        queries.append(runtime.Query(pipe.desc()))
-    info = queries[0].info()
-
-    # correctness of the measured path (every copy gives the same rows)
-    if not args.no_check:
-        import oracle
-        want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(build, probe).desc()))
-        for q in queries[:2]:
-            q.run(stream)
-            got = oracle.sorted_rows(q.result(stream))
-            if got != want:
-                print(json.dumps({"error": "GPU result differs from the oracle",
-                                  "got": got, "want": want}))
-                sys.exit(1)
+        dev = [runtime.upload(t) for t in W["tables"]]
+        bufs.append(dev)
+        queries.append(runtime.Query(W["make"]([runtime.addrs(d) for d in dev]).desc()))
+    q0 = queries[0]
+    info = q0.info()
+    kernel, _ = q0.kernel()
+    probe_bytes = W["probe_bytes"](q0)
+    q0.run(stream)
+    gpu_rows = sorted(q0.result(stream).splitlines())
 
     for i in range(args.warmup):
         queries[i % copies].run(stream)
     stream.synchronize()
 
-    # timed region: K steps, probe kernel bracketed by events on its stream
+    # timed region: K steps; the probe kernel bracketed by events on its stream
     K = args.steps
     ev = [(runtime.Event(), runtime.Event()) for _ in range(K)]
     t_start, t_stop = runtime.Event(), runtime.Event()
@@ -103,26 +166,13 @@ def main():
     t_stop.record(stream)
     stream.synchronize()
     wall = time.perf_counter() - wall0
-    total_ms = t_start.elapsed_ms(t_stop)
-    probe_ms = [a.elapsed_ms(b) for a, b in ev]
-    ms_per_step = total_ms / K
-    tuples = BUILD_ROWS + PROBE_ROWS
-    value = tuples / (ms_per_step / 1e3)
-
-    # algorithmic bytes of the probe kernel per launch: the probe columns
-    # (fk 4 + date 4 + price 8 B per row) + one join-table entry per row
-    # that passes the date predicate (1 byte in the direct layout, an
-    # 8-byte slot in the hash layout)
-    kernel, _ = queries[0].kernel()
-    entry, _ = queries[0].join(0)
-    sel = int((probe["date"] <= workloads.JOIN_DATE_CUTOFF).sum())
-    probe_bytes = PROBE_ROWS * 16 + sel * entry
-    probe_avg_ms = float(np.mean(probe_ms))
+    ms_per_step = t_start.elapsed_ms(t_stop) / K
+    probe_avg_ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
     achieved = probe_bytes / (probe_avg_ms / 1e3) / 1e9
 
     line = {
-        "metric": "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend",
-        "value": value,
+        "metric": METRIC,
+        "value": W["tuples"] / (ms_per_step / 1e3),
         "unit": "input tuples/s",
         "n_gpus": 1,
         "steps": K,
@@ -133,17 +183,13 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seed 42, counter-based splitmix64)",
-        "config": {
-            "workload": "hash join 1.5M build x 6M probe (50% date filter) + sum(price) by segment (5 groups)",
-            "build_rows": BUILD_ROWS, "probe_rows": PROBE_ROWS, "input_copies": copies,
-            "group_mode": info["group_mode"], "grid": info["grid"],
-            "probe_kernel": "generated (hipRTC)" if kernel == 2 else "generic",
-            "join_table_entry_bytes": entry,
-        },
+        "config": dict({"workload": W["name"], "input_copies": copies, "group_mode": info["group_mode"],
+                        "grid": info["grid"],
+                        "probe_kernel": "generated (hipRTC)" if kernel == 2 else "generic"}, **W["config"]),
         "roofline": {
             "bound": "hbm",
             "kernel": ("vh_generated" if kernel == 2 else "vh::pipeline_kernel") +
-                      " (select + join probe + group-by, one launch)",
+                      " (select + join probes + group-by, one launch)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -154,35 +200,121 @@ def main():
         },
         "wall_s": wall,
     }
-
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(build, probe, args.cpu_seconds)
+        base, cpu_rows = cpu_baseline(W["make"]().desc(), W["tuples"], args)
+        line["cpu_baseline"] = base
+        if cpu_rows != gpu_rows:
+            print(json.dumps({"error": "GPU result differs from the CPU oracle", "gpu": gpu_rows[:8],
+                              "cpu": cpu_rows[:8]}), flush=True)
+            sys.exit(1)
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(build, probe, budget_s):
+def cpu_baseline(desc, tuples, args):
     """The oracle's vector(1024) restatement on the host cores: the full
-    workload, hot runs until the budget is spent (at least 2, at most 5)."""
+    workload, hot runs until the budget is spent (at least 2, at most 5).
+    Returns (baseline record, its sorted result rows)."""
     import oracle
-    from vldb_voila_amd import workloads
 
-    threads = min(os.cpu_count() or 1, 16)
-    d = workloads.join_aggregate_pipeline(build, probe).desc()
-    times = []
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    times, text = [], ""
     t0 = time.perf_counter()
-    while len(times) < 5 and (len(times) < 2 or time.perf_counter() - t0 < budget_s):
-        _, ns = oracle.run_pipeline(d, threads=threads)
+    while len(times) < 5 and (len(times) < 2 or time.perf_counter() - t0 < args.cpu_seconds):
+        text, ns = oracle.run_pipeline(desc, threads=threads)
         times.append(ns / 1e9)
     best = min(times)
-    tuples = BUILD_ROWS + PROBE_ROWS
-    return {
-        "value": tuples / best,
-        "unit": "input tuples/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"full workload, {len(times)} hot runs of the vector(1024) restatement, "
-                  f"min {best * 1e3:.1f} ms, avg {sum(times) / len(times) * 1e3:.1f} ms",
+    return ({"value": tuples / best, "unit": "input tuples/s", "cores": threads, "kind": "port",
+             "sample": f"full workload, {len(times)} hot runs of the vector(1024) restatement, "
+                       f"min {best * 1e3:.1f} ms, avg {sum(times) / len(times) * 1e3:.1f} ms"},
+            sorted(text.splitlines()))
+
+
+def bench_batch(args):
+    """256 uneven pipelines: one hipGraph replay of the persistent launches
+    against plain launches one pipeline after another."""
+    from vldb_voila_amd import datagen, runtime, workloads
+
+    runtime.init(0)
+    stream = runtime.Stream()
+    params = datagen.batch_params(256, seed=42)
+    descs, devs, hosts = [], [], []
+    tuples = 0
+    for i, prm in enumerate(params):
+        b, p = datagen.gen_batch_pipeline(prm, seed=42, index=i)
+        db, dp = runtime.upload(b), runtime.upload(p)
+        devs.append((db, dp))
+        descs.append(workloads.batch_pipeline(prm, b, p, runtime.addrs(db), runtime.addrs(dp)).desc())
+        hosts.append((prm, b, p))
+        tuples += prm["probe_rows"] + prm["build_rows"]
+    queries = [runtime.Query(d, runtime.make_config(kernel=1)) for d in descs]
+    batch = runtime.Batch(descs)
+    for _ in range(max(args.warmup, 1)):
+        batch.run_graph(stream)
+        for q in queries:
+            q.run(stream)
+    stream.synchronize()
+    gpu_rows = [sorted(batch.result(i, stream).splitlines()) for i in range(len(descs))]
+    seq_rows = [sorted(q.result(stream).splitlines()) for q in queries]
+    if gpu_rows != seq_rows:
+        print(json.dumps({"error": "batched and sequential results differ"}), flush=True)
+        sys.exit(1)
+    e0, e1, e2 = runtime.Event(), runtime.Event(), runtime.Event()
+    K = args.steps
+    wall0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(K):
+        batch.run_graph(stream)
+    e1.record(stream)
+    for _ in range(K):
+        for q in queries:
+            q.run(stream)
+    e2.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - wall0
+    batched_ms = e0.elapsed_ms(e1) / K
+    seq_ms = e1.elapsed_ms(e2) / K
+    binfo = batch.info()
+    # every probe row's columns (fk 4 + a 4 + g 4 + v 8 + w 8 B) and every
+    # build row's key and payload (8 B)
+    alg = sum(prm["probe_rows"] * 28 + prm["build_rows"] * 8 for prm, _, _ in hosts)
+    line = {
+        "metric": METRIC, "value": tuples / (batched_ms / 1e3), "unit": "input tuples/s", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": batched_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seed 42, counter-based splitmix64)",
+        "config": {"workload": "256 uneven select-join-group-by pipelines, one hipGraph of persistent launches",
+                   "input_tuples": tuples, "batch": binfo, "sequential_ms": seq_ms,
+                   "speedup_vs_sequential": seq_ms / batched_ms},
+        "roofline": {"bound": "hbm", "kernel": "whole batch (clear + builds + probes)",
+                     "achieved": alg / (batched_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "avg_kernel_ms": batched_ms, "algorithmic_bytes": alg},
+        "wall_s": wall,
     }
+    if not args.no_cpu:
+        # a bounded sample: the smallest pipelines until the budget is spent;
+        # each one's rows are also the check of the batched GPU result
+        import oracle
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        order = sorted(range(len(hosts)), key=lambda i: hosts[i][0]["probe_rows"])
+        done, ns_total, t_sample = 0, 0, 0
+        t0 = time.perf_counter()
+        for i in order:
+            if done and time.perf_counter() - t0 > args.cpu_seconds:
+                break
+            prm, b, p = hosts[i]
+            text, ns = oracle.run_pipeline(workloads.batch_pipeline(prm, b, p).desc(), threads=threads)
+            if sorted(text.splitlines()) != gpu_rows[i]:
+                print(json.dumps({"error": f"pipeline {i}: GPU result differs from the CPU oracle"}), flush=True)
+                sys.exit(1)
+            ns_total += ns
+            t_sample += prm["probe_rows"] + prm["build_rows"]
+            done += 1
+        line["cpu_baseline"] = {"value": t_sample / (ns_total / 1e9), "unit": "input tuples/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"the {done} smallest of the 256 pipelines ({t_sample} tuples), "
+                                          f"one run each of the vector(1024) restatement"}
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

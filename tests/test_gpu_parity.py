@@ -323,3 +323,22 @@ def test_unroll_variants_of_the_generated_kernel():
         for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_PIPE):
             cfg = runtime.make_config(kernel=kernel, unroll=unroll)
             assert oracle.sorted_rows(runtime.run_pipeline(p, cfg)) == want, (unroll, kernel)
+
+
+@pytest.mark.parametrize("theta", [0.0, 1.0])
+def test_star_schema(theta):
+    """Four joins (two dimensions filtered on region, part a pure semi-join),
+    175 groups; uniform and Zipf(1.0) part keys; every generated-kernel
+    variant and the generic kernel."""
+    print(f"star schema 2M fact rows theta={theta}", flush=True)
+    fact, date, cust, supp, part = datagen.gen_star(2_000_000, zipf_part=theta, seed=7, dims_scale=0.2)
+    want = oracle.sorted_rows(oracle.run_pipeline(
+        workloads.star_pipeline(fact, date, cust, supp, part).desc(), threads=8)[0])
+    assert len(want) == 175
+    dev = {k: runtime.upload(v) for k, v in
+           (("fact", fact), ("date", date), ("customer", cust), ("supplier", supp), ("part", part))}
+    addrs = {k: runtime.addrs(v) for k, v in dev.items()}
+    p = workloads.star_pipeline(fact, date, cust, supp, part, addrs)
+    for kernel in KERNELS:
+        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(kernel=kernel)))
+        assert got == want, kernel
