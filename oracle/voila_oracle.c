@@ -22,7 +22,7 @@
  * joined by '|', numbers printed by voila_cast<T>::to_cstr
  * (runtime.hpp:348-387).  Rows are compared sorted (runtime_framework.cpp:238-247).
  *
- * Parity pins (checked in tests/test_oracle.py and tests/test_oracle_ref.py):
+ * Parity pins (checked in tests/test_oracle.py):
  *   hash/rehash     against the reference's own runtime.hpp (oracle/_ref) and
  *                   exact Python integers;
  *   seltrue/selfalse/selunion against runtime.cpp (oracle/_ref) on random,

@@ -101,8 +101,10 @@ def main(argv=None):
         Xtr, Ttr, Xho, Tho = z["Xtr"], z["Ttr"], z["Xho"], z["Tho"]
         pho = datagen.batch_params(len(Xho), seed=4242, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
                                    group_range=GROUP_RANGE)
+        ptr = datagen.batch_params(len(Xtr), seed=42, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
+                                   group_range=GROUP_RANGE)
     else:
-        Xtr, Ttr, _ = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"))
+        Xtr, Ttr, ptr = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"))
         Xho, Tho, pho = explore(a.heldout, seed=4242, reps=a.reps, out=os.path.join(a.out, "partial_heldout.npz"))
         np.savez(os.path.join(a.out, "timings.npz"), Xtr=Xtr, Ttr=Ttr, Xho=Xho, Tho=Tho)
     t0 = time.time()
@@ -113,7 +115,7 @@ def main(argv=None):
     rep = evaluate(net, mu, sd, Xho, Tho, pho)
     rep["train"] = {"pipelines": int(len(Xtr)), "epochs": a.epochs, "seconds": train_s,
                     "loss_first": losses[0], "loss_last": losses[-1]}
-    rep["train_fit"] = evaluate(net, mu, sd, Xtr, Ttr, datagen.batch_params(len(Xtr), seed=42))
+    rep["train_fit"] = evaluate(net, mu, sd, Xtr, Ttr, ptr)
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(rep, f, indent=1)
     print(json.dumps(rep), flush=True)
