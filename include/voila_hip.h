@@ -160,7 +160,11 @@ enum vh_group_mode {
 	VH_GROUP_HASH = 3,     /* packed key, global CAS open addressing          */
 	VH_GROUP_LDSHASH = 4,  /* per-block LDS hash table over the dense index,
 	                          global atomics when it is full                  */
-	VH_GROUP_DENSE = 5     /* VH_GROUP_REG when it fits, else VH_GROUP_LDS    */
+	VH_GROUP_DENSE = 5,    /* VH_GROUP_REG when it fits, else VH_GROUP_LDS    */
+	VH_GROUP_PART = 6      /* dense domain too large for LDS: rows' (group,
+	                          values) tuples are partitioned by group range
+	                          and each partition aggregated in LDS; no
+	                          per-row global atomics                         */
 };
 
 /* Physical layout of a join table.  Both give the same matches. */

@@ -10,7 +10,7 @@ from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
 pytestmark = pytest.mark.gpu
 
 MODES = [abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH,
-         abi.VH_GROUP_LDSHASH]
+         abi.VH_GROUP_LDSHASH, abi.VH_GROUP_PART]
 
 
 def gpu_rows(builder_fn, mode=abi.VH_GROUP_AUTO, **cfg):
@@ -165,9 +165,39 @@ def test_many_groups_hash_and_global():
         return p
     want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
     p, devs = on_gpu([t], make)
-    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
-        got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode)))
-        assert got == want, mode
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH,
+                 abi.VH_GROUP_PART):
+        for kernel in (abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_PIPE):
+            got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode, kernel=kernel)))
+            assert got == want, (mode, kernel)
+
+
+@pytest.mark.parametrize("rows", [0, 1, 1023, 300_001])
+def test_partitioned_group_by_shapes(rows):
+    """GB_PART over a 1M-group domain: regions (workgroup slabs) of uneven
+    size, partitions with no tuples, JoinN fan-out past a region's tuple
+    space (added with atomics), signed i64 sums into i128, grids that leave
+    the last region short."""
+    print(f"partitioned group-by rows={rows}", flush=True)
+    rng = np.random.default_rng(rows + 5)
+    build = {"key": rng.integers(0, 3000, 9000).astype(np.int32),        # ~3 rows per key
+             "pay": rng.integers(0, 7, 9000).astype(np.int32)}
+    probe = {"fk": rng.integers(0, 3100, rows).astype(np.int32),
+             "g": rng.integers(0, 1_000_000, rows).astype(np.int32),
+             "v": rng.integers(-2**62, 2**62, rows).astype(np.int64),
+             "w": rng.integers(0, 2**20, rows).astype(np.int32)}
+
+    def make(bb, pp):
+        p = Pipeline(table_from_arrays("s", probe, pp))
+        p.join(table_from_arrays("r", build, bb), "key", "fk", payload=["pay"], variant=abi.VH_JOINN, names=["pay"])
+        p.groupby(["g"], [("sum", col("v")), ("sum", col("w") * col("pay")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None, None).desc()))
+    p, devs = on_gpu([build, probe], make)
+    for kernel in (abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_PIPE):
+        for grid in (0, 3, 64):
+            cfg = runtime.make_config(group_mode=abi.VH_GROUP_PART, kernel=kernel, grid=grid)
+            assert oracle.sorted_rows(runtime.run_pipeline(p, cfg)) == want, (kernel, grid)
 
 
 def test_global_aggregate_and_empty_inputs():

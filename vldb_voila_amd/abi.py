@@ -80,6 +80,7 @@ class vh_pipeline_desc(C.Structure):
 
 VH_GROUP_AUTO, VH_GROUP_REG, VH_GROUP_LDS, VH_GROUP_GLOBAL, VH_GROUP_HASH, VH_GROUP_LDSHASH = -1, 0, 1, 2, 3, 4
 VH_GROUP_DENSE = 5
+VH_GROUP_PART = 6
 
 # enum vh_join_layout
 VH_LAYOUT_AUTO = 0

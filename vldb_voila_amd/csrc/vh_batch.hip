@@ -21,7 +21,13 @@ namespace vh {
 template <int MODE>
 __device__ __forceinline__ void probe_item_m(const Plan& P, i64 r0, i64 r1, u64* lds, u32 dups_mask)
 {
-	const Lds L = lds_layout<MODE>(P, lds);
+	Lds L = lds_layout<MODE>(P, lds);
+	if (MODE == GB_PART) {
+		// an item is one region: its rows' tuple space and table row
+		L.rbase = r0;
+		L.rcap = (u32)(r1 - r0);
+		L.region = (u32)(r0 / P.gregion_rows);
+	}
 	lds_init<MODE>(P, L);
 	__syncthreads();
 	process_range<MODE>(P, L, r0, r1, (i64)kU * blockDim.x, dups_mask);
@@ -30,6 +36,8 @@ __device__ __forceinline__ void probe_item_m(const Plan& P, i64 r0, i64 r1, u64*
 		lds_dense_flush(P, L);
 	else if (MODE == GB_LDSHASH)
 		ldshash_flush(P, L);
+	else if (MODE == GB_PART)
+		part_finish(P, L);
 	__syncthreads();
 }
 
@@ -43,6 +51,7 @@ __device__ void probe_item(const Plan& P, i64 r0, i64 r1, u64* lds)
 	case GB_LDS: probe_item_m<GB_LDS>(P, r0, r1, lds, dups_mask); break;
 	case GB_LDSHASH: probe_item_m<GB_LDSHASH>(P, r0, r1, lds, dups_mask); break;
 	case GB_GLOBAL: probe_item_m<GB_GLOBAL>(P, r0, r1, lds, dups_mask); break;
+	case GB_PART: probe_item_m<GB_PART>(P, r0, r1, lds, dups_mask); break;
 	default: probe_item_m<GB_HASH>(P, r0, r1, lds, dups_mask); break;
 	}
 }
@@ -67,8 +76,17 @@ __global__ __launch_bounds__(256) void batch_kernel(const Plan* __restrict__ pla
 			const BuildPlan& B = builds[I.build];
 			for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
 				build_chunk(B, base, I.r1);
-		} else {
+		} else if (I.phase == 1) {
 			probe_item(plans[I.plan], I.r0, I.r1, lds);
+		} else {
+			// GB_PART: partition r0 of the plan; LDS = slice, then two
+			// per-region arrays
+			const Plan& P = plans[I.plan];
+			const u64 slice_words = (((u64)1 << P.gshift) * ((u64)P.nsums * 12 + 4) + 7) / 8;
+			const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
+			u32* pre = (u32*)(lds + slice_words);
+			part_agg(P, (int)I.r0, lds, pre, pre + R, pre + 2 * R);
+			__syncthreads();
 		}
 	}
 }

@@ -33,6 +33,8 @@ struct vh_batch {
 	u32* d_ctl = nullptr;
 	u32 nitems = 0;
 	u32 nbuild = 0;
+	u32 nprobe = 0;
+	size_t agg_lds = 0;
 	int nsegs = 0;
 	int grid = 0;
 	size_t lds = 0;
@@ -70,15 +72,20 @@ static int tile_rows_of(const vh_config& c)
 	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
 }
 
-// clear, then every build item, then every probe item
+// clear, then every build item, then every probe item, then the partition
+// aggregation items of partitioned group-bys
 static hipError_t enqueue(vh_batch* b, hipStream_t s)
 {
 	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
 	if (e == hipSuccess)
 		e = launch_batch(b->d_plans, b->d_builds, b->d_items, 0, b->nbuild, b->d_ctl, b->grid, 0, s);
 	if (e == hipSuccess)
-		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild, b->nitems - b->nbuild, b->d_ctl + 1,
-			b->grid, b->lds, s);
+		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild, b->nprobe, b->d_ctl + 1, b->grid, b->lds,
+			s);
+	const u32 nagg = b->nitems - b->nbuild - b->nprobe;
+	if (e == hipSuccess && nagg)
+		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild + b->nprobe, nagg, b->d_ctl + 2, b->grid,
+			b->agg_lds, s);
 	return e;
 }
 
@@ -94,7 +101,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		return fail(VH_ERR_NOMEM, "host allocation");
 	std::vector<Plan> plans;
 	std::vector<BuildPlan> builds;
-	std::vector<BatchItem> build_items, probe_items;
+	std::vector<BatchItem> build_items, probe_items, agg_items;
 	std::vector<FillSeg> segs;
 	int rc = VH_OK;
 	for (int i = 0; i < n && !rc; i++) {
@@ -118,12 +125,21 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		Plan& P = q->L.plan;
 		if (P.mode == GB_REG)
 			P.mode = GB_LDS;            // register state cannot outlive an item
+		const int T = tile_rows_of(q->cfg);
+		if (P.mode == GB_PART) {
+			// an item is a region: the [region][partition] tables have a row per tile
+			P.gregion_rows = T;
+			if ((P.rows + T - 1) / T > kMaxGroupRegions) {
+				rc = fail(VH_ERR_UNSUPPORTED, "pipeline %d: more than %d items for a partitioned group-by", i,
+					kMaxGroupRegions);
+				break;
+			}
+		}
 		if ((rc = query_create(q, false))) {
 			fail(rc, "pipeline %d: %s", i, last_error());
 			break;
 		}
 		b->lds = std::max(b->lds, pipeline_lds_bytes(P));
-		const int T = tile_rows_of(q->cfg);
 		const int first_build = (int)builds.size();
 		for (int j = 0; j < P.njoins; j++) {
 			builds.push_back(q->L.builds[j]);
@@ -135,6 +151,12 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		}
 		for (i64 r0 = 0; r0 < P.rows; r0 += T)
 			probe_items.push_back({i, 1, -1, 0, r0, std::min<i64>(r0 + T, P.rows)});
+		if (P.mode == GB_PART) {
+			for (int p = 0; p < P.gparts; p++)
+				agg_items.push_back({i, 2, -1, 0, p, p + 1});
+			const size_t R = (size_t)((P.rows + T - 1) / T);
+			b->agg_lds = std::max(b->agg_lds, (part_slice_bytes(P) + 7) / 8 * 8 + (2 * R + 1) * sizeof(u32));
+		}
 		const u64 groups = q->L.groups;
 		const u64 ns = (u64)std::max(P.nsums, 1);
 		segs.push_back({(u64*)q->flags, (u64)kFlagWords / 2, 0, nullptr});
@@ -151,9 +173,11 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	}
 	std::vector<BatchItem> items = build_items;
 	items.insert(items.end(), probe_items.begin(), probe_items.end());
+	items.insert(items.end(), agg_items.begin(), agg_items.end());
 	b->nitems = (u32)items.size();
 	b->nbuild = (u32)build_items.size();
-	const size_t ctl_words = 2;   // two queue counters (u32), one per launch
+	b->nprobe = (u32)probe_items.size();
+	const size_t ctl_words = 2;   // three queue counters (u32), one per launch
 	if ((rc = b->upload((void**)&b->d_ctl, nullptr, ctl_words * 8)) ||
 		(rc = b->upload((void**)&b->d_plans, plans.data(), plans.size() * sizeof(Plan))) ||
 		(rc = b->upload((void**)&b->d_builds, builds.data(), builds.size() * sizeof(BuildPlan))) ||

@@ -23,7 +23,7 @@ using namespace vh;
 // Variant index -> knobs: 4 group-by strategies x 3 join-table layouts x
 // 4 probe grid sizes = 48 (the analog of the explorer's blend space,
 // explorer_helper.cpp:6-17).
-static const int kModes[4] = {VH_GROUP_DENSE, VH_GROUP_LDSHASH, VH_GROUP_GLOBAL, VH_GROUP_HASH};
+static const int kModes[4] = {VH_GROUP_DENSE, VH_GROUP_LDSHASH, VH_GROUP_PART, VH_GROUP_HASH};
 // join table: {hash layout, hash layout at twice the slots, direct when dense}
 static const int kTableShift[3] = {0, 1, 0};
 static const int kTableLayout[3] = {VH_LAYOUT_HASH, VH_LAYOUT_HASH, VH_LAYOUT_AUTO};

@@ -110,6 +110,21 @@ static int device_cus()
 	return cus;
 }
 
+// GB_PART: groups per partition slice (a power of two whose LDS slice,
+// 12 bytes per sum + a 4-byte count per group, fits in 32 KiB)
+static int part_shift(int nsums)
+{
+	int shift = 12;
+	while (shift > 6 && ((size_t)1 << shift) * ((size_t)nsums * 12 + 4) > 32 * 1024)
+		shift--;
+	return shift;
+}
+
+static bool part_fits(double domain, int nsums)
+{
+	return nsums <= 4 && domain <= (double)kMaxGroupParts * (double)(1u << part_shift(nsums));
+}
+
 static const double kMaxCard = 1099511627776.0;   // QueryConfig::max_card = 2^40
 
 int type_pipeline(const vh_pipeline_desc* d, Typing& ty)
@@ -496,6 +511,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		// bytes per lane) caps the workgroups a CU can hold.
 		if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
+		else if (domain <= (double)kLdsHashSlots && P.nsums <= 4)
+			mode = GB_LDSHASH;      // every group fits the LDS hash table
+		else if (part_fits(domain, P.nsums))
+			mode = GB_PART;
 		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
 			mode = GB_LDSHASH;
 		else if (domain <= (double)(1 << 24))
@@ -515,7 +534,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		return fail(VH_ERR_UNSUPPORTED, "dense global group-by domain %.0f too large", domain);
 	if (mode == GB_HASH && bits > 63)
 		return fail(VH_ERR_UNSUPPORTED, "packed group key needs %d bits", bits);
-	if (mode < GB_REG || mode > GB_LDSHASH)
+	if (mode == GB_PART && !part_fits(domain, P.nsums))
+		return fail(VH_ERR_UNSUPPORTED, "partitioned group-by needs <= %d partitions of the dense domain "
+			"and <= 4 sums", kMaxGroupParts);
+	if (mode < GB_REG || (mode > GB_LDSHASH && mode != GB_PART))
 		return fail(VH_ERR_INVALID, "group mode %d", mode);
 	P.mode = mode;
 	if (mode == GB_HASH) {
@@ -544,6 +566,19 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	if (!grid) {
 		int64_t want = (P.rows + 1023) / 1024;
 		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4 * device_cus()));
+	}
+	if (P.mode == GB_PART) {
+		// a contiguous slab of rows per workgroup, whole chunks of 1024 rows
+		grid = std::min(grid, kMaxGroupRegions);
+		const int64_t rows = std::max<int64_t>(P.rows, 1);
+		P.gregion_rows = ((rows + grid - 1) / grid + 1023) / 1024 * 1024;
+		grid = (int)((rows + P.gregion_rows - 1) / P.gregion_rows);
+		// slices small enough for ~256 partitions (a workgroup each), no
+		// larger than the LDS allows
+		P.gshift = part_shift(P.nsums);
+		while (P.gshift > 6 && (P.domain >> (P.gshift - 1)) < 256)
+			P.gshift--;
+		P.gparts = (int)((P.domain + (1ull << P.gshift) - 1) >> P.gshift);
 	}
 	L.grid = grid;
 	// dense LDS group-by: workgroups leave partials for merge_kernel, or add
@@ -638,6 +673,16 @@ int query_create(Query* q, bool single)
 		if ((rc = q->alloc((void**)&P.partials, (u64)L.grid * W * sizeof(u64))))
 			return rc;
 	}
+	if (P.mode == GB_PART) {
+		const u64 rows = (u64)std::max<int64_t>(P.rows, 1);
+		const u64 W = (u64)P.nsums + 1;
+		const u64 regions = (rows + (u64)P.gregion_rows - 1) / (u64)P.gregion_rows;
+		if ((rc = q->alloc((void**)&P.gtup0, rows * W * sizeof(u64))) ||
+			(rc = q->alloc((void**)&P.gtup1, rows * W * sizeof(u64))) ||
+			(rc = q->alloc((void**)&P.gcnt, regions * (u64)P.gparts * sizeof(u32))) ||
+			(rc = q->alloc((void**)&P.goff, regions * (u64)P.gparts * sizeof(u32))))
+			return rc;
+	}
 	if (P.mode == GB_HASH) {
 		if ((rc = q->alloc((void**)&P.hkeys, groups * sizeof(u64))))
 			return rc;
@@ -659,7 +704,7 @@ int query_create(Query* q, bool single)
 				segs.push_back({q->dir[j], L.dir_words[j], ~0ull, nullptr});
 		}
 	}
-	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH || P.flush) {
+	if (P.mode == GB_GLOBAL || P.mode == GB_HASH || P.mode == GB_LDSHASH || P.mode == GB_PART || P.flush) {
 		segs.push_back({P.out_lo, groups * nsums, 0, nullptr});
 		segs.push_back({(u64*)P.out_hi, groups * nsums, 0, nullptr});
 		segs.push_back({P.out_cnt, groups, 0, nullptr});

@@ -177,7 +177,59 @@ struct Lds {
 	int* hi;
 	u32* cnt;
 	u32* keys;
+	// GB_PART: this region's tuple space (rows [rbase, rbase + rcap)) and
+	// its index in the [region][partition] tables; cnt = per-partition
+	// counts, keys = per-partition cursors, hi = the region's tuple count
+	i64 rbase;
+	u32 rcap;
+	u32 region;
 };
+
+// One slot per active lane of the wave from a counter in memory, with one
+// atomic per wave: lanes are ranked by mbcnt over the active mask.
+__device__ __forceinline__ u32 wave_append(u32* ctr)
+{
+	const u64 mask = __ballot(1);
+	const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+	const int leader = __ffsll((long long)mask) - 1;
+	u32 base = 0;
+	if ((int)(threadIdx.x & (kWave - 1)) == leader)
+		base = atomicAdd(ctr, (u32)__popcll(mask));
+	base = __shfl(base, leader, kWave);
+	return base + rank;
+}
+
+// In-place exclusive prefix sum of a[0, n) in LDS (n <= kWave * 64) by the
+// first wave: each lane sums a contiguous segment, a wave scan of the
+// segment sums, then each lane rewrites its segment.  One barrier; every
+// thread gets the total (through *total, in LDS).
+__device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* total)
+{
+	if (threadIdx.x < (unsigned)kWave) {
+		const int lane = threadIdx.x;
+		const int per = (n + kWave - 1) / kWave;
+		const int b0 = lane * per;
+		u32 t = 0;
+		for (int i = b0; i < b0 + per && i < n; i++)
+			t += a[i];
+		u32 x = t;
+		for (int d = 1; d < kWave; d <<= 1) {
+			const u32 y = __shfl_up(x, d, kWave);
+			if (lane >= d)
+				x += y;
+		}
+		u32 run = x - t;
+		for (int i = b0; i < b0 + per && i < n; i++) {
+			const u32 v = a[i];
+			a[i] = run;
+			run += v;
+		}
+		if (lane == kWave - 1)
+			*total = x;
+	}
+	__syncthreads();
+	return *total;
+}
 
 // Where a row's group lives: a dense index g (GB_REG / GB_LDS / GB_GLOBAL),
 // an LDS hash slot (GB_LDSHASH; -1 = LDS full, use the global arrays at g)
@@ -283,6 +335,192 @@ __device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 
 	}
 }
 
+// GB_PART, one tuple per live row: a slot in the region's tuple space (one
+// atomic per wave), the group index, then each sum's argument; the
+// partition's count in LDS.  Rows beyond the space (JoinN fan-out) add into
+// the global arrays.
+__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, u64*& t)
+{
+	t = nullptr;
+	if (!live)
+		return;
+	const u32 pos = wave_append((u32*)L.hi);
+	if (pos < L.rcap) {
+		t = P.gtup0 + (u64)(L.rbase + pos) * (u64)(P.nsums + 1);
+		t[0] = g;
+		atomicAdd(&L.cnt[g >> P.gshift], 1u);
+	} else {
+		atomicAdd(&P.out_cnt[g], 1ull);
+	}
+}
+
+__device__ __forceinline__ void part_tuple_value(const Plan& P, u64* t, int s, u64 g, u64 v)
+{
+	if (t) {
+		t[1 + s] = v;
+	} else {
+		const SumArg& S = P.sums[s];
+		add_global(&P.out_lo[(u64)s * P.domain + g], &P.out_hi[(u64)s * P.domain + g], v, P.wide && S.res128,
+			S.arg_signed);
+	}
+}
+
+__device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, const Chunk& C, const u64 g[kU],
+	const bool ok[kU])
+{
+	u64* t[kU];
+	bool live[kU];
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		live[u] = C.alive[u];
+		if (live[u] && (!ok[u] || g[u] >= P.domain)) {
+			atomicOr(P.error, 8u);   // group key outside its column's [min, max]
+			live[u] = false;
+		}
+		part_tuple_slot(P, L, g[u], live[u], t[u]);
+	}
+#pragma unroll 1
+	for (int s = 0; s < VH_MAX_AGGRS; s++) {
+		if (s < P.nsums) {
+			u64 v[kU];
+			eval_sum_u(P, P.sums[s], C, v);
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				if (live[u])
+					part_tuple_value(P, t[u], s, g[u], v[u]);
+		}
+	}
+}
+
+// End of a GB_PART region: order its tuples by partition (gtup0 -> gtup1,
+// the same rows' positions) and record each partition's run.
+__device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
+{
+	__syncthreads();
+	const u32 n = (u32)L.hi[0] < L.rcap ? (u32)L.hi[0] : L.rcap;
+	const int NP = P.gparts;
+	u32* cnt = P.gcnt + (u64)L.region * NP;
+	u32* off = P.goff + (u64)L.region * NP;
+	for (int p = threadIdx.x; p < NP; p += blockDim.x) {
+		cnt[p] = L.cnt[p];
+		L.keys[p] = L.cnt[p];
+	}
+	__syncthreads();
+	block_exclusive_scan(L.keys, NP, (u32*)&L.hi[1]);
+	for (int p = threadIdx.x; p < NP; p += blockDim.x)
+		off[p] = L.keys[p];
+	__syncthreads();
+	const u64 W = (u64)P.nsums + 1;
+	constexpr int V = 4;   // tuples per thread in flight
+	for (u32 i0 = 0; i0 < n; i0 += V * blockDim.x) {
+		u64 g[V];
+#pragma unroll
+		for (int v = 0; v < V; v++) {
+			const u32 i = i0 + v * blockDim.x + threadIdx.x;
+			g[v] = i < n ? P.gtup0[(u64)(L.rbase + i) * W] : ~0ull;
+		}
+#pragma unroll
+		for (int v = 0; v < V; v++) {
+			if (g[v] == ~0ull)
+				continue;
+			const u32 i = i0 + v * blockDim.x + threadIdx.x;
+			const u64* src = P.gtup0 + (u64)(L.rbase + i) * W;
+			const u32 pos = atomicAdd(&L.keys[g[v] >> P.gshift], 1u);
+			u64* dst = P.gtup1 + (u64)(L.rbase + pos) * W;
+			dst[0] = g[v];
+			for (u64 w = 1; w < W; w++)
+				dst[w] = src[w];
+		}
+	}
+	__syncthreads();
+}
+
+// GB_PART, one workgroup per partition p: aggregate the partition's runs of
+// every region in an LDS slice of 2^gshift groups, then add the slice into
+// out_* (this workgroup is the only writer of these groups).
+__device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pre, u32* base, u32* total_slot)
+{
+	const u64 S = 1ull << P.gshift;
+	const u64 g0 = (u64)p * S;
+	const u64 ng = S < P.domain - g0 ? S : P.domain - g0;
+	const int ns = P.nsums;
+	u64* lo = lds;
+	int* hi = (int*)(lo + (u64)ns * S);
+	u32* cnt = (u32*)(hi + (u64)ns * S);
+	for (u64 i = threadIdx.x; i < (u64)ns * S; i += blockDim.x) {
+		lo[i] = 0;
+		hi[i] = 0;
+	}
+	for (u64 i = threadIdx.x; i < S; i += blockDim.x)
+		cnt[i] = 0;
+	const int NP = P.gparts;
+	const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
+	for (int r = threadIdx.x; r < R; r += blockDim.x) {
+		pre[r] = P.gcnt[(u64)r * NP + p];
+		base[r] = P.goff[(u64)r * NP + p];
+	}
+	__syncthreads();
+	const u32 total = block_exclusive_scan(pre, R, total_slot);
+	// tuple i of this partition (0 <= i < total) sits at base[r] + i for the
+	// last region r with pre[r] <= i
+	for (int r = threadIdx.x; r < R; r += blockDim.x)
+		base[r] = (u32)((i64)r * P.gregion_rows) + base[r] - pre[r];
+	__syncthreads();
+	const u64 W = (u64)ns + 1;
+	constexpr int V = 8;   // tuples per thread in flight
+	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
+		const u64* t[V];
+		u64 g[V];
+#pragma unroll
+		for (int v = 0; v < V; v++) {
+			const u32 i = i0 + v * blockDim.x + threadIdx.x;
+			t[v] = nullptr;
+			g[v] = 0;
+			if (i < total) {
+				int a = 0, b = R - 1;
+				while (a < b) {
+					const int mid = (a + b + 1) >> 1;
+					if (pre[mid] <= i)
+						a = mid;
+					else
+						b = mid - 1;
+				}
+				t[v] = P.gtup1 + (u64)(base[a] + i) * W;
+				g[v] = t[v][0];
+			}
+		}
+#pragma unroll
+		for (int v = 0; v < V; v++) {
+			if (!t[v])
+				continue;
+			const u64 li = g[v] - g0;
+			atomicAdd(&cnt[li], 1u);
+			for (int s2 = 0; s2 < ns; s2++) {
+				const SumArg& A = P.sums[s2];
+				add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], t[v][1 + s2], P.wide && A.wide,
+					A.arg_signed);
+			}
+		}
+	}
+	__syncthreads();
+	for (u64 li = threadIdx.x; li < ng; li += blockDim.x) {
+		const u32 c = cnt[li];
+		if (!c)
+			continue;
+		const u64 g = g0 + li;
+		P.out_cnt[g] += c;
+		for (int s2 = 0; s2 < ns; s2++) {
+			const SumArg& A = P.sums[s2];
+			const u64 l = lo[(u64)s2 * S + li];
+			const i64 h = (P.wide && A.wide) ? (i64)hi[(u64)s2 * S + li] : (A.arg_signed ? (i64)l >> 63 : 0);
+			const u64 idx = (u64)s2 * P.domain + g;
+			const u64 o = P.out_lo[idx];
+			P.out_lo[idx] = o + l;
+			P.out_hi[idx] += h + ((o + l) < o ? 1 : 0);
+		}
+	}
+}
+
 // Group index (dense modes) or packed key (hash mode) of the chunk's live
 // rows, their group slots, then one sum at a time: its argument for all U
 // rows, added to each row's group.
@@ -308,6 +546,10 @@ __device__ __forceinline__ void consume_u(const Plan& P, const Lds& L, const Chu
 				g[u] += MODE == GB_HASH ? (off << P.keys[k].shift) : off * P.keys[k].mult;
 			}
 		}
+	}
+	if (MODE == GB_PART) {
+		part_consume_u(P, L, C, g, ok);
+		return;
 	}
 	i64 slot[kU];
 	bool live[kU];
@@ -515,6 +757,10 @@ __device__ __forceinline__ Lds lds_layout(const Plan& P, u64* lds)
 		L.hi = (int*)(lds + n);
 		L.cnt = (u32*)(L.hi + n);
 		L.keys = L.cnt + kLdsHashSlots;
+	} else if (MODE == GB_PART) {
+		L.cnt = (u32*)lds;
+		L.keys = L.cnt + P.gparts;
+		L.hi = (int*)(L.keys + P.gparts);   // [0] tuple count, [1] scan total
 	}
 	return L;
 }
@@ -534,6 +780,12 @@ __device__ __forceinline__ void lds_init(const Plan& P, const Lds& L)
 		L.cnt[i] = 0;
 		if (MODE == GB_LDSHASH)
 			L.keys[i] = 0xFFFFFFFFu;
+	}
+	if (MODE == GB_PART) {
+		for (int i = threadIdx.x; i < P.gparts; i += blockDim.x)
+			L.cnt[i] = 0;
+		if (threadIdx.x == 0)
+			L.hi[0] = 0;
 	}
 }
 
@@ -566,6 +818,10 @@ __device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
 template <int MODE>
 __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 {
+	if (MODE == GB_PART) {
+		part_finish(P, L);
+		return;
+	}
 	if (MODE == GB_LDSHASH) {
 		__syncthreads();
 		ldshash_flush(P, L);

@@ -183,6 +183,16 @@ struct Gen {
 			s += "\t\t\tif (!ok) { atomicOr(" + err + ", 8u); return; }\n";
 		else
 			s += "\t\t\tif (!ok || g >= " + dom + ") { atomicOr(" + err + ", 8u); return; }\n";
+		if (mode == GB_PART) {
+			// one tuple: a slot in the region's tuple space, then each sum's argument
+			s += "\t\t\tu64* t;\n\t\t\tpart_tuple_slot(P, L, g, true, t);\n";
+			for (int si = 0; si < P.nsums; si++) {
+				snprintf(b, sizeof b, "\t\t\tpart_tuple_value(P, t, %d, g, %s);\n", si,
+					sum_expr(si, tail_join, tail_pay).c_str());
+				s += b;
+			}
+			return s;
+		}
 		const bool dense = mode == GB_REG || mode == GB_LDS || mode == GB_GLOBAL;
 		if (!dense) {
 			snprintf(b, sizeof b, "\t\t\tconst i64 slot = locate<%d>(P, L, g);\n", mode);
@@ -272,34 +282,48 @@ struct Gen {
 			const std::string ty = ctype_of(cols[i].type);
 			return uniform(gptr(ty), "(" + gptr(ty) + ")" + cols[i].path);
 		};
+		// rows of this workgroup: every tile-th chunk of the table, or (GB_PART)
+		// one contiguous slab, the region its tuples are kept for
+		if (mode == GB_PART) {
+			line("\tconst i64 first = (i64)blockIdx.x * P.gregion_rows;");
+			line("\tconst i64 end = first + P.gregion_rows < rows ? first + P.gregion_rows : rows;");
+			line("\tconst i64 stride = tile;");
+			line("\tL.rbase = first;");
+			line("\tL.rcap = end > first ? (u32)(end - first) : 0u;");
+			line("\tL.region = blockIdx.x;");
+		} else {
+			line("\tconst i64 first = (i64)blockIdx.x * tile;");
+			line("\tconst i64 end = rows;");
+			line("\tconst i64 stride = (i64)gridDim.x * tile;");
+		}
 		if (pipe) {
 			// software pipeline: chunk i+1's columns are requested after chunk
 			// i's first join lookups, so their latency overlaps that lookup and
 			// the aggregation (loads retire in order: the lookup is waited for
 			// without waiting for them)
-			line("\ti64 base = (i64)blockIdx.x * tile;");
+			line("\ti64 base = first;");
 			for (size_t i = 0; i < cols.size(); i++) {
 				snprintf(b, sizeof b, "\t%s c%zu[U];", ctype_of(cols[i].type), i);
 				line(b);
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\tfor (int u = 0; u < U; u++) { const i64 r = base + (i64)u * blockDim.x + "
-					"threadIdx.x; c%zu[u] = %s[r < rows ? r : 0]; }", i, col_ptr((int)i).c_str());
+					"threadIdx.x; c%zu[u] = %s[r < end ? r : 0]; }", i, col_ptr((int)i).c_str());
 				line(b);
 			}
-			line("\tfor (; base < rows; base += (i64)gridDim.x * tile) {");
+			line("\tfor (; base < end; base += stride) {");
 		} else {
-			line("\tfor (i64 base = (i64)blockIdx.x * tile; base < rows; base += (i64)gridDim.x * tile) {");
+			line("\tfor (i64 base = first; base < end; base += stride) {");
 		}
 		auto emit_next = [&]() {
 			if (!pipe || next_emitted)
 				return;
 			next_emitted = true;
 			line("\t\t__builtin_amdgcn_sched_barrier(0);");
-			line("\t\tconst i64 nbase = base + (i64)gridDim.x * tile;");
+			line("\t\tconst i64 nbase = base + stride;");
 			for (size_t i = 0; i < cols.size(); i++) {
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) { const i64 r = nbase + (i64)u * blockDim.x + "
-					"threadIdx.x; n%zu[u] = %s[r < rows ? r : 0]; }", i, col_ptr((int)i).c_str());
+					"threadIdx.x; n%zu[u] = %s[r < end ? r : 0]; }", i, col_ptr((int)i).c_str());
 				line(b);
 			}
 			line("\t\t__builtin_amdgcn_sched_barrier(0);");
@@ -314,7 +338,7 @@ struct Gen {
 		line("#pragma unroll");
 		line("\t\tfor (int u = 0; u < U; u++) {");
 		line("\t\t\trow[u] = base + (i64)u * blockDim.x + threadIdx.x;");
-		line("\t\t\talive[u] = row[u] < rows;");
+		line("\t\t\talive[u] = row[u] < end;");
 		line("\t\t}");
 		auto load = [&](int i) {
 			if (pipe)
@@ -477,7 +501,7 @@ struct Gen {
 		k += "static constexpr int U = " + std::to_string(U) + ";\n";
 		k += "extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)\n{\n";
 		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
-		k += "\tconst Lds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
+		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
 		k += "\tu32 dups_mask = 0;\n";
 		for (int j = 0; j < P.njoins; j++)

@@ -24,5 +24,7 @@ hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t 
 // GB_REG / GB_LDS: merge the per-block partials of a `grid`-block launch
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 size_t pipeline_lds_bytes(const Plan& P);
+// GB_PART: LDS of part_agg_kernel's slice (2^gshift groups)
+size_t part_slice_bytes(const Plan& P);
 
 } // namespace vh

@@ -79,13 +79,15 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const Plan* __restrict__ 
 	}
 }
 
+static constexpr int kPartBlockAgg = 1024;   // workgroup size of part_agg_kernel
+
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict__ Pp)
 {
 	const Plan& P = *Pp;
 	extern __shared__ u64 lds[];
 
-	const Lds L = lds_layout<MODE>(P, lds);
+	Lds L = lds_layout<MODE>(P, lds);
 	lds_init<MODE>(P, L);
 	u32 dups_mask = 0;
 	for (int j = 0; j < P.njoins; j++)
@@ -94,9 +96,28 @@ __global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict
 	__syncthreads();
 
 	const i64 tile = (i64)kU * blockDim.x;
-	process_range<MODE>(P, L, (i64)blockIdx.x * tile, P.rows, (i64)gridDim.x * tile, dups_mask);
+	if (MODE == GB_PART) {
+		// a contiguous slab of rows per workgroup: its region of tuple space
+		const i64 r0 = (i64)blockIdx.x * P.gregion_rows;
+		const i64 r1 = r0 + P.gregion_rows < P.rows ? r0 + P.gregion_rows : P.rows;
+		L.rbase = r0;
+		L.rcap = r1 > r0 ? (u32)(r1 - r0) : 0u;
+		L.region = blockIdx.x;
+		process_range<MODE>(P, L, r0, r1, tile, dups_mask);
+	} else {
+		process_range<MODE>(P, L, (i64)blockIdx.x * tile, P.rows, (i64)gridDim.x * tile, dups_mask);
+	}
 
 	block_epilogue<MODE>(P, L);
+}
+
+__global__ __launch_bounds__(kPartBlockAgg) void part_agg_kernel(const Plan* __restrict__ Pp)
+{
+	extern __shared__ u64 slice[];
+	__shared__ u32 pre[kMaxGroupRegions];
+	__shared__ u32 base[kMaxGroupRegions];
+	__shared__ u32 total;
+	part_agg(*Pp, blockIdx.x, slice, pre, base, &total);
 }
 
 __global__ __launch_bounds__(kBlock) void build_kernel(const BuildPlan* __restrict__ Bp)
@@ -159,38 +180,6 @@ __device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64
 			return;
 		h = (h + 1) & B.mask;
 	}
-}
-
-// In-place exclusive prefix sum of a[0, n) in LDS (n <= kWave * 64) by the
-// first wave: each lane sums a contiguous segment, a wave scan of the
-// segment sums, then each lane rewrites its segment.  One barrier; every
-// thread gets the total (through *total, in LDS).
-__device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* total)
-{
-	if (threadIdx.x < (unsigned)kWave) {
-		const int lane = threadIdx.x;
-		const int per = (n + kWave - 1) / kWave;
-		const int b0 = lane * per;
-		u32 t = 0;
-		for (int i = b0; i < b0 + per && i < n; i++)
-			t += a[i];
-		u32 x = t;
-		for (int d = 1; d < kWave; d <<= 1) {
-			const u32 y = __shfl_up(x, d, kWave);
-			if (lane >= d)
-				x += y;
-		}
-		u32 run = x - t;
-		for (int i = b0; i < b0 + per && i < n; i++) {
-			const u32 v = a[i];
-			a[i] = run;
-			run += v;
-		}
-		if (lane == kWave - 1)
-			*total = x;
-	}
-	__syncthreads();
-	return *total;
 }
 
 __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPlan* __restrict__ Bp)
@@ -353,11 +342,22 @@ size_t pipeline_lds_bytes(const Plan& P)
 		return (size_t)(P.domain * P.nsums * 12 + P.domain * 4 + 16);
 	if (P.mode == GB_LDSHASH)
 		return (size_t)P.nsums * kLdsHashSlots * 12 + kLdsHashSlots * 8 + 16;
+	if (P.mode == GB_PART)
+		return (size_t)P.gparts * 8 + 16;
 	return 0;
+}
+
+size_t part_slice_bytes(const Plan& P)
+{
+	return ((size_t)1 << P.gshift) * ((size_t)P.nsums * 12 + 4) + 16;
 }
 
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s)
 {
+	if (P.mode == GB_PART) {
+		hipLaunchKernelGGL(part_agg_kernel, dim3((unsigned)P.gparts), dim3(kPartBlockAgg), part_slice_bytes(P), s, dP);
+		return hipGetLastError();
+	}
 	if ((P.mode != GB_REG && P.mode != GB_LDS) || P.flush)
 		return hipSuccess;
 	const u64 elems = P.domain * (u64)P.nsums + P.domain;
@@ -374,6 +374,7 @@ hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t 
 	case GB_LDS: VH_LAUNCH(GB_LDS); break;
 	case GB_GLOBAL: VH_LAUNCH(GB_GLOBAL); break;
 	case GB_LDSHASH: VH_LAUNCH(GB_LDSHASH); break;
+	case GB_PART: VH_LAUNCH(GB_PART); break;
 	default: VH_LAUNCH(GB_HASH); break;
 	}
 #undef VH_LAUNCH

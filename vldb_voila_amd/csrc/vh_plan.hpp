@@ -86,7 +86,22 @@ struct KeyT {
 	int pad;
 };
 
-enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH = 4 };
+// values match enum vh_group_mode (5 there is VH_GROUP_DENSE, which the
+// lowering resolves to GB_REG or GB_LDS)
+enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH = 4, GB_PART = 6 };
+
+// GB_PART: partitioned dense group-by for domains too large for LDS.  The
+// probe rows are cut into regions of gregion_rows rows (a workgroup's slab,
+// or a batched work item).  A region writes one tuple per qualifying row,
+// (g, argument of every sum) as nsums + 1 words, into gtup0 at its rows'
+// own positions, counting them per partition (g >> gshift) in LDS; at its
+// end it reorders them by partition into gtup1 and records, per partition,
+// the run's length (gcnt) and offset (goff) in [region][partition] tables.
+// part_agg_kernel then aggregates each partition's runs in an LDS slice and
+// adds the slice into out_*.  A row with more than one match (JoinN) that
+// finds its region's tuple space full adds straight into out_* with atomics.
+static constexpr int kMaxGroupParts = 4096;
+static constexpr int kMaxGroupRegions = 2048;
 
 // GB_LDSHASH: per-block LDS open-addressing table over the dense group index,
 // kLdsHashSlots entries, at most kLdsHashProbes probes; a tuple that finds no
@@ -122,6 +137,14 @@ struct Plan {
 	// hash mode: packed keys, EMPTY = ~0
 	u64* hkeys;
 	u64 hmask;
+	// partitioned mode (GB_PART)
+	u64* gtup0;
+	u64* gtup1;
+	u32* gcnt;
+	u32* goff;
+	i64 gregion_rows;
+	int gshift;
+	int gparts;
 	u32* error;          // bit 0: hash table full, bit 3: group key outside [min, max]
 };
 
