@@ -41,53 +41,68 @@ __device__ __forceinline__ void probe_item_m(const Plan& P, i64 r0, i64 r1, u64*
 	__syncthreads();
 }
 
-__device__ void probe_item(const Plan& P, i64 r0, i64 r1, u64* lds)
+// A workgroup's next item of this launch, or false when its queue is empty.
+__device__ __forceinline__ bool next_item(const BatchItem* __restrict__ items, u32 first, u32 count,
+	u32* __restrict__ counter, u32* slot, BatchItem& I)
 {
-	u32 dups_mask = 0;
-	for (int j = 0; j < P.njoins; j++)
-		if (*P.joins[j].dups)
-			dups_mask |= 1u << j;
-	switch (P.mode) {
-	case GB_LDS: probe_item_m<GB_LDS>(P, r0, r1, lds, dups_mask); break;
-	case GB_LDSHASH: probe_item_m<GB_LDSHASH>(P, r0, r1, lds, dups_mask); break;
-	case GB_GLOBAL: probe_item_m<GB_GLOBAL>(P, r0, r1, lds, dups_mask); break;
-	case GB_PART: probe_item_m<GB_PART>(P, r0, r1, lds, dups_mask); break;
-	default: probe_item_m<GB_HASH>(P, r0, r1, lds, dups_mask); break;
+	if (threadIdx.x == 0)
+		*slot = atomicAdd(counter, 1u);
+	__syncthreads();
+	const u32 k = *slot;
+	__syncthreads();
+	if (k >= count)
+		return false;
+	I = items[first + k];
+	return true;
+}
+
+// Items [first, first + count) of the queue; *counter is this launch's
+// cursor.  One kernel per kind of item (and per group-by strategy), so each
+// launch holds only the registers its own code needs.
+__global__ __launch_bounds__(256) void batch_build_kernel(const BuildPlan* __restrict__ builds,
+	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
+{
+	__shared__ u32 slot;
+	BatchItem I;
+	while (next_item(items, first, count, counter, &slot, I)) {
+		const BuildPlan& B = builds[I.build];
+		for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
+			build_chunk(B, base, I.r1);
 	}
 }
 
-// Items [first, first + count) of the queue; ctl[0] is this launch's counter.
-__global__ __launch_bounds__(256) void batch_kernel(const Plan* __restrict__ plans,
-	const BuildPlan* __restrict__ builds, const BatchItem* __restrict__ items, u32 first, u32 count,
-	u32* __restrict__ counter)
+template <int MODE>
+__global__ __launch_bounds__(256) void batch_probe_kernel(const Plan* __restrict__ plans,
+	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
 {
 	extern __shared__ u64 lds[];
-	__shared__ u32 item_idx;
-	for (;;) {
-		if (threadIdx.x == 0)
-			item_idx = atomicAdd(counter, 1u);
+	__shared__ u32 slot;
+	BatchItem I;
+	while (next_item(items, first, count, counter, &slot, I)) {
+		const Plan& P = plans[I.plan];
+		u32 dups_mask = 0;
+		for (int j = 0; j < P.njoins; j++)
+			if (*P.joins[j].dups)
+				dups_mask |= 1u << j;
+		probe_item_m<MODE>(P, I.r0, I.r1, lds, dups_mask);
+	}
+}
+
+// GB_PART partition aggregation: item r0 = partition; LDS = the slice, then
+// two per-region arrays and a total.
+__global__ __launch_bounds__(256) void batch_agg_kernel(const Plan* __restrict__ plans,
+	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
+{
+	extern __shared__ u64 lds[];
+	__shared__ u32 slot;
+	BatchItem I;
+	while (next_item(items, first, count, counter, &slot, I)) {
+		const Plan& P = plans[I.plan];
+		const u64 slice_words = (((u64)1 << P.gshift) * ((u64)P.nsums * 12 + 4) + 7) / 8;
+		const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
+		u32* pre = (u32*)(lds + slice_words);
+		part_agg(P, (int)I.r0, lds, pre, pre + R, pre + 2 * R);
 		__syncthreads();
-		const u32 k = item_idx;
-		__syncthreads();
-		if (k >= count)
-			break;
-		const BatchItem I = items[first + k];
-		if (I.phase == 0) {
-			const BuildPlan& B = builds[I.build];
-			for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
-				build_chunk(B, base, I.r1);
-		} else if (I.phase == 1) {
-			probe_item(plans[I.plan], I.r0, I.r1, lds);
-		} else {
-			// GB_PART: partition r0 of the plan; LDS = slice, then two
-			// per-region arrays
-			const Plan& P = plans[I.plan];
-			const u64 slice_words = (((u64)1 << P.gshift) * ((u64)P.nsums * 12 + 4) + 7) / 8;
-			const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
-			u32* pre = (u32*)(lds + slice_words);
-			part_agg(P, (int)I.r0, lds, pre, pre + R, pre + 2 * R);
-			__syncthreads();
-		}
 	}
 }
 
@@ -113,13 +128,29 @@ hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s)
 	return hipGetLastError();
 }
 
-hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 first,
-	u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s)
+hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items,
+	u32 first, u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s)
 {
 	if (!count)
 		return hipSuccess;
-	hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), lds_bytes, s, d_plans, d_builds, d_items, first, count,
-		d_counter);
+#define VH_PROBE(M) hipLaunchKernelGGL((batch_probe_kernel<M>), dim3(grid), dim3(256), lds_bytes, s, d_plans, \
+	d_items, first, count, d_counter)
+	switch (kind) {
+	case kBatchBuild:
+		hipLaunchKernelGGL(batch_build_kernel, dim3(grid), dim3(256), 0, s, d_builds, d_items, first, count,
+			d_counter);
+		break;
+	case kBatchAgg:
+		hipLaunchKernelGGL(batch_agg_kernel, dim3(grid), dim3(256), lds_bytes, s, d_plans, d_items, first, count,
+			d_counter);
+		break;
+	case GB_LDS: VH_PROBE(GB_LDS); break;
+	case GB_LDSHASH: VH_PROBE(GB_LDSHASH); break;
+	case GB_GLOBAL: VH_PROBE(GB_GLOBAL); break;
+	case GB_PART: VH_PROBE(GB_PART); break;
+	default: VH_PROBE(GB_HASH); break;
+	}
+#undef VH_PROBE
 	return hipGetLastError();
 }
 

@@ -26,8 +26,12 @@ struct FillSeg {
 
 // One launch filling every segment (the batch's and a query's clear pass).
 hipError_t launch_multi_fill(const FillSeg* d_segs, int nseg, hipStream_t s);
-// Items [first, first + count) with the launch's own queue counter.
-hipError_t launch_batch(const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items, u32 first,
-	u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s);
+// Launch kinds: build items, partition-aggregation items, or probe items of
+// one group-by strategy (the GroupMode value).
+static constexpr int kBatchBuild = -1;
+static constexpr int kBatchAgg = -2;
+// Items [first, first + count) of one kind with the launch's own queue counter.
+hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items,
+	u32 first, u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s);
 
 } // namespace vh

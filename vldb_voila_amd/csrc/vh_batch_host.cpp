@@ -23,6 +23,13 @@ using namespace vh;
 			return fail(VH_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e));           \
 	} while (0)
 
+struct BatchLaunch {
+	int kind;            // kBatchBuild, kBatchAgg or a GroupMode (probe items)
+	u32 first;
+	u32 count;
+	size_t lds;
+};
+
 struct vh_batch {
 	std::vector<Query*> q;
 	std::vector<void*> allocs;
@@ -32,12 +39,12 @@ struct vh_batch {
 	FillSeg* d_segs = nullptr;
 	u32* d_ctl = nullptr;
 	u32 nitems = 0;
-	u32 nbuild = 0;
-	u32 nprobe = 0;
+	std::vector<BatchLaunch> launches;
 	size_t agg_lds = 0;
 	int nsegs = 0;
 	int grid = 0;
 	size_t lds = 0;
+	size_t mode_lds[8] = {};
 	hipGraphExec_t graph = nullptr;
 	hipGraph_t graph_def = nullptr;
 	hipStream_t graph_stream = nullptr;
@@ -72,20 +79,17 @@ static int tile_rows_of(const vh_config& c)
 	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
 }
 
-// clear, then every build item, then every probe item, then the partition
-// aggregation items of partitioned group-bys
+// clear, then every build item, then the probe items of each group-by
+// strategy (a launch each), then the partition aggregation items of
+// partitioned group-bys
 static hipError_t enqueue(vh_batch* b, hipStream_t s)
 {
 	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
-	if (e == hipSuccess)
-		e = launch_batch(b->d_plans, b->d_builds, b->d_items, 0, b->nbuild, b->d_ctl, b->grid, 0, s);
-	if (e == hipSuccess)
-		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild, b->nprobe, b->d_ctl + 1, b->grid, b->lds,
+	for (size_t k = 0; k < b->launches.size() && e == hipSuccess; k++) {
+		const BatchLaunch& l = b->launches[k];
+		e = launch_batch(l.kind, b->d_plans, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k, b->grid, l.lds,
 			s);
-	const u32 nagg = b->nitems - b->nbuild - b->nprobe;
-	if (e == hipSuccess && nagg)
-		e = launch_batch(b->d_plans, b->d_builds, b->d_items, b->nbuild + b->nprobe, nagg, b->d_ctl + 2, b->grid,
-			b->agg_lds, s);
+	}
 	return e;
 }
 
@@ -139,7 +143,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 			fail(rc, "pipeline %d: %s", i, last_error());
 			break;
 		}
-		b->lds = std::max(b->lds, pipeline_lds_bytes(P));
+		b->mode_lds[P.mode] = std::max(b->mode_lds[P.mode], pipeline_lds_bytes(P));
 		const int first_build = (int)builds.size();
 		for (int j = 0; j < P.njoins; j++) {
 			builds.push_back(q->L.builds[j]);
@@ -150,7 +154,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
 		}
 		for (i64 r0 = 0; r0 < P.rows; r0 += T)
-			probe_items.push_back({i, 1, -1, 0, r0, std::min<i64>(r0 + T, P.rows)});
+			probe_items.push_back({i, 1, P.mode, 0, r0, std::min<i64>(r0 + T, P.rows)});
 		if (P.mode == GB_PART) {
 			for (int p = 0; p < P.gparts; p++)
 				agg_items.push_back({i, 2, -1, 0, p, p + 1});
@@ -171,13 +175,28 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		delete b;
 		return rc;
 	}
+	// queue: build items, probe items grouped by group-by strategy (the
+	// probe item's build field holds its plan's mode), aggregation items
+	std::stable_sort(probe_items.begin(), probe_items.end(),
+		[](const BatchItem& x, const BatchItem& y) { return x.build < y.build; });
 	std::vector<BatchItem> items = build_items;
+	b->launches.push_back({kBatchBuild, 0, (u32)build_items.size(), 0});
+	for (size_t k = 0; k < probe_items.size();) {
+		size_t e = k;
+		while (e < probe_items.size() && probe_items[e].build == probe_items[k].build)
+			e++;
+		const int mode = probe_items[k].build;
+		b->launches.push_back({mode, (u32)(build_items.size() + k), (u32)(e - k), b->mode_lds[mode]});
+		k = e;
+	}
 	items.insert(items.end(), probe_items.begin(), probe_items.end());
+	b->launches.push_back({kBatchAgg, (u32)items.size(), (u32)agg_items.size(), b->agg_lds});
 	items.insert(items.end(), agg_items.begin(), agg_items.end());
+	for (const BatchLaunch& l : b->launches)
+		if (l.kind != kBatchBuild)
+			b->lds = std::max(b->lds, l.lds);
 	b->nitems = (u32)items.size();
-	b->nbuild = (u32)build_items.size();
-	b->nprobe = (u32)probe_items.size();
-	const size_t ctl_words = 2;   // three queue counters (u32), one per launch
+	const size_t ctl_words = 4;   // queue counters (u32), one per launch (at most 7)
 	if ((rc = b->upload((void**)&b->d_ctl, nullptr, ctl_words * 8)) ||
 		(rc = b->upload((void**)&b->d_plans, plans.data(), plans.size() * sizeof(Plan))) ||
 		(rc = b->upload((void**)&b->d_builds, builds.data(), builds.size() * sizeof(BuildPlan))) ||
