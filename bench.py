@@ -246,7 +246,10 @@ def bench_batch(args):
         descs.append(workloads.batch_pipeline(prm, b, p, runtime.addrs(db), runtime.addrs(dp)).desc())
         hosts.append((prm, b, p))
         tuples += prm["probe_rows"] + prm["build_rows"]
+    # the same pipelines as plain launches one after another: the generic
+    # kernel the batch runs, and the default (generated) kernels
     queries = [runtime.Query(d, runtime.make_config(kernel=1)) for d in descs]
+    queries_gen = [runtime.Query(d) for d in descs]
     batch = runtime.Batch(descs)
     for _ in range(max(args.warmup, 1)):
         batch.run_graph(stream)
@@ -258,21 +261,32 @@ def bench_batch(args):
     if gpu_rows != seq_rows:
         print(json.dumps({"error": "batched and sequential results differ"}), flush=True)
         sys.exit(1)
-    e0, e1, e2 = runtime.Event(), runtime.Event(), runtime.Event()
+    for q in queries_gen:
+        q.run(stream)
+    stream.synchronize()
+    if [sorted(q.result(stream).splitlines()) for q in queries_gen] != seq_rows:
+        print(json.dumps({"error": "generated and generic kernels differ"}), flush=True)
+        sys.exit(1)
+    e0, e1, e2, e3 = runtime.Event(), runtime.Event(), runtime.Event(), runtime.Event()
     K = args.steps
     wall0 = time.perf_counter()
     e0.record(stream)
     for _ in range(K):
         batch.run_graph(stream)
     e1.record(stream)
+    wall = time.perf_counter() - wall0
     for _ in range(K):
         for q in queries:
             q.run(stream)
     e2.record(stream)
+    for _ in range(K):
+        for q in queries_gen:
+            q.run(stream)
+    e3.record(stream)
     stream.synchronize()
-    wall = time.perf_counter() - wall0
     batched_ms = e0.elapsed_ms(e1) / K
     seq_ms = e1.elapsed_ms(e2) / K
+    seq_gen_ms = e2.elapsed_ms(e3) / K
     binfo = batch.info()
     # every probe row's columns (fk 4 + a 4 + g 4 + v 8 + w 8 B) and every
     # build row's key and payload (8 B)
@@ -284,7 +298,9 @@ def bench_batch(args):
         "data": "synthetic (seed 42, counter-based splitmix64)",
         "config": {"workload": "256 uneven select-join-group-by pipelines, one hipGraph of persistent launches",
                    "input_tuples": tuples, "batch": binfo, "sequential_ms": seq_ms,
-                   "speedup_vs_sequential": seq_ms / batched_ms},
+                   "speedup_vs_sequential": seq_ms / batched_ms,
+                   "sequential_generated_ms": seq_gen_ms,
+                   "speedup_vs_sequential_generated": seq_gen_ms / batched_ms},
         "roofline": {"bound": "hbm", "kernel": "whole batch (clear + builds + probes)",
                      "achieved": alg / (batched_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
