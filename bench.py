@@ -40,6 +40,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of each workload's probe kernel, from rocprofv3
+# FETCH_SIZE + WRITE_SIZE runs scaled by a copy of known size; each file
+# gives the commands and the counter values
+TRAFFIC_FILES = {"join": "traffic_join_1.5Mx6M.md", "filter": "traffic_filter_groupby_6M.md",
+                 "star": "traffic_star_60M.md"}
 METRIC = "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend"
 
 
@@ -106,6 +111,21 @@ def workload_star(datagen, workloads, theta):
             "tables": [fact, date, cust, supp, part], "make": make, "tuples": 60_000_000 + dims,
             "probe_bytes": lambda q: 60_000_000 * 32,
             "config": {"fact_rows": 60_000_000, "dimension_rows": dims, "zipf_theta": theta}}
+
+
+def measured_traffic(workload, zipf):
+    """Counter-derived HBM bytes per probe launch recorded under profiles/,
+    or None (no record for this workload)."""
+    name = TRAFFIC_FILES.get(workload)
+    if name is None or (workload == "star" and zipf):
+        return None
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
 
 
 def main():
@@ -194,7 +214,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, args.zipf),
             "avg_kernel_ms": probe_avg_ms,
             "algorithmic_bytes": probe_bytes,
         },

@@ -88,10 +88,38 @@ def test_source_reflects_the_plan():
     d = workloads.join_aggregate_pipeline(b, p).desc()
     direct = source(d)
     hashed = source(d, join_layout=abi.VH_LAYOUT_HASH)
-    assert "uint8_t*)P.joins[0].dir" in direct
-    assert "P.joins[0].slots" in hashed and ".dir" not in hashed
-    # predicate column first, all columns loaded before the predicate (eager)
+    # a 2000-key direct table fits in LDS: copied there, looked up there
+    assert "P.joins[0].dir" in direct and "td0[off[u]]" in direct
+    assert "P.joins[0].slots" in hashed and ".dir" not in hashed and "td0" not in hashed
+    # the generic kernel reads no LDS copy, so the source of an unknown
+    # kernel choice still names the global table
+    big_b, big_p = datagen.gen_join_aggregate(200_000, 6000, seed=5)
+    big = source(workloads.join_aggregate_pipeline(big_b, big_p).desc())
+    assert "uint8_t*)P.joins[0].dir" in big and "td0" not in big and "tb0" not in big
+    # eager: the vector loop loads every column of a chunk before the
+    # predicate; the remaining rows take the scalar loop
     eager = source(d)
+    assert "q2[p] = __builtin_nontemporal_load" in eager
+    assert eager.index("compare(") > eager.index("q2[p] = __builtin_nontemporal_load")
+    assert "c2[u] = " in eager
+    # lazy: scalar loop only, columns no predicate reads after the predicate
     lazy = source(d, kernel=abi.VH_KERNEL_JIT_LAZY)
-    assert eager.index("compare(") > eager.index("c2[u] =")
+    assert "__builtin_nontemporal_load" not in lazy
     assert lazy.index("compare(") < lazy.index("c2[u] =")
+
+
+def test_filtered_dimension_keeps_a_key_bitmap_in_lds():
+    """A filtered build too large for an LDS copy of its table keeps a bitmap
+    of the keys it holds in LDS; unfiltered dense builds keep none."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    n = 300_000
+    dim = {"k": np.arange(1, n + 1, dtype=np.int32), "r": rng.integers(0, 5, n).astype(np.int32),
+           "v": rng.integers(0, 25, n).astype(np.int32)}
+    fact = {"fk": rng.integers(1, n + 1, 10_000).astype(np.int32), "m": rng.integers(0, 100, 10_000).astype(np.int64)}
+    pl = Pipeline(table_from_arrays("f", fact))
+    pl.join(table_from_arrays("d", dim), "k", "fk", payload=["v"], build_preds=[("r", "==", 2)], names=["v"])
+    pl.groupby(["v"], [("sum", col("m"))])
+    src = source(pl.desc())
+    assert "tb0[off[u] >> 5]" in src and "P.joins[0].bits" in src
+    assert "td0" not in src

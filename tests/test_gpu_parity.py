@@ -250,6 +250,36 @@ def test_build_side_filter_two_joins():
         assert got == want, mode
 
 
+def test_lds_key_bitmap_with_repeated_keys():
+    """A filtered build too large for an LDS copy of its table: the generated
+    kernel tests a key bitmap in LDS before the table in global memory; keys
+    repeated in the build (JoinN, last join) still reach every row."""
+    print("LDS key bitmap, filtered build with repeated keys", flush=True)
+    rng = np.random.default_rng(23)
+    n = 400_000
+    keys = np.arange(1, n + 1, dtype=np.int32)
+    keys[rng.integers(0, n, 5000)] = rng.integers(1, n + 1, 5000).astype(np.int32)   # repeats
+    dim = {"k": keys, "r": rng.integers(0, 4, n).astype(np.int32), "v": rng.integers(0, 9, n).astype(np.int32)}
+    fact = {"fk": rng.integers(-5, n + 10, 500_000).astype(np.int32),
+            "m": rng.integers(-10**6, 10**6, 500_000).astype(np.int64)}
+
+    def make(fa, da):
+        p = Pipeline(table_from_arrays("f", fact, fa))
+        p.join(table_from_arrays("d", dim, da), "k", "fk", payload=["v"], build_preds=[("r", "==", 1)],
+               variant=abi.VH_JOINN, names=["v"])
+        p.groupby(["v"], [("sum", col("m")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None, None).desc()))
+    p, devs = on_gpu([fact, dim], make)
+    q = runtime.Query(p.desc())
+    kernel, src = q.kernel()
+    assert kernel == abi.VH_KERNEL_JIT and "tb0[off[u] >> 5]" in src
+    for _ in range(2):   # the bitmap is rebuilt with the table on every run
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+    q.free()
+
+
 def test_key_outside_declared_range_is_an_error():
     print("bad metadata", flush=True)
     t = {"k": np.array([1, 2, 3, 9], dtype=np.int32), "v": np.array([1, 2, 3, 4], dtype=np.int64)}

@@ -46,7 +46,7 @@ __device__ __forceinline__ bool next_item(const BatchItem* __restrict__ items, u
 	u32* __restrict__ counter, u32* slot, BatchItem& I)
 {
 	if (threadIdx.x == 0)
-		*slot = atomicAdd(counter, 1u);
+		*slot = gatomic_add(counter, 1u);
 	__syncthreads();
 	const u32 k = *slot;
 	__syncthreads();

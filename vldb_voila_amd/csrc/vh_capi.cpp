@@ -224,6 +224,8 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 	if (rc)
 		return rc;
 	const int kernel = cfg ? cfg->kernel : VH_KERNEL_AUTO;
+	if (kernel != VH_KERNEL_GENERIC)
+		assign_lds_tables(L, desc);
 	const std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel));
 	*len = src.size();
 	if (buf) {

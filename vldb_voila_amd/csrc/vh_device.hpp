@@ -17,6 +17,48 @@ typedef unsigned int u32;
 
 static constexpr int kWave = 64;
 
+// Global memory through address-space-1 pointers.  Loads and atomics through
+// generic pointers compile to FLAT instructions, which count against both
+// the vector-memory and the LDS counters; while one is outstanding the
+// compiler can only wait for all vector memory operations together, which
+// serialises a loop that keeps loads in flight across iterations.  The
+// atomics are relaxed and device-scoped, as HIP's atomicAdd etc. are.
+#define VH_GLOBAL __attribute__((address_space(1)))
+
+__device__ __forceinline__ u32 gatomic_or(u32* p, u32 v)
+{
+	return __hip_atomic_fetch_or((VH_GLOBAL u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u32 gatomic_and(u32* p, u32 v)
+{
+	return __hip_atomic_fetch_and((VH_GLOBAL u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u32 gatomic_add(u32* p, u32 v)
+{
+	return __hip_atomic_fetch_add((VH_GLOBAL u32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ u64 gatomic_add(u64* p, u64 v)
+{
+	return __hip_atomic_fetch_add((VH_GLOBAL u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Compare-and-swap; returns the value found (== cmp when it swapped).
+__device__ __forceinline__ u64 gatomic_cas(u64* p, u64 cmp, u64 v)
+{
+	__hip_atomic_compare_exchange_strong((VH_GLOBAL u64*)p, &cmp, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+		__HIP_MEMORY_SCOPE_AGENT);
+	return cmp;
+}
+
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, i64 i)
+{
+	return ((const VH_GLOBAL T*)p)[i];
+}
+
 // voila_hash<T>::hash on the value converted to u64 (runtime.hpp:413-424).
 __device__ __forceinline__ u64 voila_hash(u64 x)
 {
@@ -49,13 +91,13 @@ __device__ __forceinline__ int type_width(int t)
 __device__ __forceinline__ i64 load1(const void* p, int t, i64 i)
 {
 	switch (t) {
-	case VH_U8: return ((const uint8_t*)p)[i];
-	case VH_I8: return ((const int8_t*)p)[i];
-	case VH_U16: return ((const uint16_t*)p)[i];
-	case VH_I16: return ((const int16_t*)p)[i];
-	case VH_U32: return ((const uint32_t*)p)[i];
-	case VH_I32: return ((const int32_t*)p)[i];
-	default: return ((const i64*)p)[i];
+	case VH_U8: return gload((const uint8_t*)p, i);
+	case VH_I8: return gload((const int8_t*)p, i);
+	case VH_U16: return gload((const uint16_t*)p, i);
+	case VH_I16: return gload((const int16_t*)p, i);
+	case VH_U32: return gload((const uint32_t*)p, i);
+	case VH_I32: return gload((const int32_t*)p, i);
+	default: return gload((const i64*)p, i);
 	}
 }
 
@@ -127,17 +169,17 @@ __device__ __forceinline__ bool compare(int cmp, int t, i64 a, i64 b)
 	}
 }
 
-// Exact 128-bit accumulation into (lo: u64, hi: i64) words in global or LDS
+// Exact 128-bit accumulation into (lo: u64, hi: i64) words in global
 // memory: the low word is added atomically, its carry and the sign extension
 // of the addend go to the high word.  Integer addition commutes, so the
 // result is independent of the order in which blocks merge.
 __device__ __forceinline__ void atomic_add_i128(u64* lo, i64* hi, u64 add_lo, i64 add_hi)
 {
-	u64 old = atomicAdd(lo, add_lo);
+	u64 old = gatomic_add(lo, add_lo);
 	i64 carry = (old + add_lo) < old ? 1 : 0;
 	i64 h = add_hi + carry;
 	if (h)
-		atomicAdd((u64*)hi, (u64)h);
+		gatomic_add((u64*)hi, (u64)h);
 }
 
 __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m)

@@ -409,7 +409,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		const bool dense = krange <= 4.0 * (double)std::max<int64_t>(jd.build.rows, 0) + 65536.0;
 		if (layout != VH_LAYOUT_HASH && krange <= (double)(1u << 30) && (layout == VH_LAYOUT_DIRECT || dense)) {
 			B.dir_w = prange <= 254 ? 1 : (prange <= 65534 ? 2 : 4);
-			L.dir_words[j] = ((uint64_t)krange * (uint64_t)B.dir_w + 7) / 8;
+			// whole 16-byte units, so that an LDS copy may read it 16 bytes at a time
+			L.dir_words[j] = ((uint64_t)krange * (uint64_t)B.dir_w + 15) / 16 * 2;
 			// partitioned build: slices of about 8192 rows (one workgroup
 			// each), at most kMaxSliceBytes of LDS and kMaxBuildParts of them
 			const uint64_t R = (uint64_t)krange;
@@ -511,8 +512,10 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		// bytes per lane) caps the workgroups a CU can hold.
 		if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
-		else if (domain <= (double)kLdsHashSlots && P.nsums <= 4)
-			mode = GB_LDSHASH;      // every group fits the LDS hash table
+		else if (domain <= 16.0 * kLdsHashSlots && P.nsums <= 4)
+			mode = GB_LDSHASH;      // groups present are often far fewer than the
+			                        // domain (keys from filtered builds); those the
+			                        // LDS table cannot hold add into global memory
 		else if (part_fits(domain, P.nsums))
 			mode = GB_PART;
 		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
@@ -560,12 +563,16 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		L.wide |= P.sums[s].res128 != 0;
 	P.wide = L.wide ? 1 : 0;
 
-	// launch shape: one resident wave of workgroups (4 per CU at this
-	// kernel's occupancy), fewer when there are not 1024 rows per workgroup
+	// launch shape: one resident wave of workgroups, fewer when there are not
+	// 1024 rows per workgroup.  The generated kernel runs best with 2 per CU
+	// (measured: c1 probe 35 us at 512 workgroups, 37.6 at 1024, 41 at 2048 on
+	// MI355X; each workgroup pays a fixed start-up and partials write), the
+	// generic kernel with 4.
 	int grid = cfg && cfg->grid > 0 ? cfg->grid : 0;
 	if (!grid) {
 		int64_t want = (P.rows + 1023) / 1024;
-		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, 4 * device_cus()));
+		const int per_cu = (cfg && cfg->kernel == VH_KERNEL_GENERIC) ? 4 : 2;
+		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * device_cus()));
 	}
 	if (P.mode == GB_PART) {
 		// a contiguous slab of rows per workgroup, whole chunks of 1024 rows
@@ -617,6 +624,39 @@ int Query::alloc(void** p, size_t bytes)
 		return fail(VH_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
 	allocs.push_back(*p);
 	return VH_OK;
+}
+
+// Join tables the generated probe kernel copies into LDS, in join order
+// (the first joins see the most rows): the whole dir[] while it fits what is
+// left of kLdsTableBudget, else its key bitmap when the build is filtered or
+// its keys are sparse, so that most lookups of keys without a row end in
+// LDS.  Direct tables only; the generic kernel ignores these fields.
+static constexpr size_t kLdsTableBudget = 64 * 1024;
+
+void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d)
+{
+	Plan& P = L.plan;
+	size_t used = groupby_lds_bytes(P);
+	for (int j = 0; j < P.njoins; j++) {
+		JoinT& J = P.joins[j];
+		J.lds = LDS_NONE;
+		J.lds_off = 0;
+		J.bits = nullptr;
+		if (!J.dir_w)
+			continue;
+		const double keys = (double)(J.kmax - J.kmin) + 1;
+		const bool misses = d->joins[j].build.npreds > 0 || keys > 2.0 * (double)d->joins[j].build.rows;
+		J.lds = LDS_DIR;
+		if (used + join_lds_bytes(J) > kLdsTableBudget) {
+			J.lds = LDS_BITS;
+			if (!misses || used + join_lds_bytes(J) > kLdsTableBudget) {
+				J.lds = LDS_NONE;
+				continue;
+			}
+		}
+		J.lds_off = (u32)used;
+		used += join_lds_bytes(J);
+	}
 }
 
 int query_create(Query* q, bool single)
@@ -716,6 +756,18 @@ int query_create(Query* q, bool single)
 		return rc;
 	HIP_TRY(hipMemcpy(q->dclear, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
 
+	// join tables the generated probe kernel keeps in LDS
+	if (single && q->cfg.kernel != VH_KERNEL_GENERIC) {
+		assign_lds_tables(L, &q->desc);
+		for (int j = 0; j < P.njoins; j++) {
+			if (P.joins[j].lds != LDS_BITS)
+				continue;
+			if ((rc = q->alloc((void**)&q->bits[j], join_lds_bytes(P.joins[j]))))
+				return rc;
+			P.joins[j].bits = q->bits[j];
+		}
+	}
+
 	// device copies of the plans
 	if ((rc = q->alloc((void**)&q->dplan, sizeof(Plan))))
 		return rc;
@@ -758,6 +810,10 @@ int query_run(Query* q, hipStream_t s, int phase)
 			else if (L.builds[j].rows > 0)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
 		}
+		for (int j = 0; j < P.njoins; j++)
+			if (P.joins[j].lds == LDS_BITS)
+				HIP_TRY(launch_dir_bits(q->dir[j], P.joins[j].dir_w, (u64)(P.joins[j].kmax - P.joins[j].kmin) + 1,
+					q->bits[j], s));
 	}
 	if (phase == -1 || phase == 2) {
 		if (q->jit_fn)

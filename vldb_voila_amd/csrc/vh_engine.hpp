@@ -46,6 +46,7 @@ struct Query {
 	u32* flags = nullptr;
 	u64* slots[VH_MAX_JOINS] = {};
 	u64* dir[VH_MAX_JOINS] = {};
+	u32* bits[VH_MAX_JOINS] = {};      // JoinT::lds == LDS_BITS: key bitmap of dir
 	struct FillSeg* dclear = nullptr;   // segments of the per-run clear pass
 	int nclear = 0;
 	Plan* dplan = nullptr;
@@ -70,6 +71,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
 // single: a query of its own (generated probe kernel per cfg.kernel,
 // partitioned builds); false for the pipelines of a batch
 int query_create(Query* q, bool single = true);
+// Join tables a generated probe kernel keeps in LDS (JoinT::lds, lds_off)
+void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d);
 int query_run(Query* q, hipStream_t s, int phase);
 int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows);
 

@@ -35,10 +35,10 @@ __device__ __forceinline__ u32 dir_empty(int w)
 __device__ __forceinline__ u32 dir_load(const void* dir, int w, u32 off)
 {
 	if (w == 1)
-		return ((const uint8_t*)dir)[off];
+		return gload((const uint8_t*)dir, off);
 	if (w == 2)
-		return ((const uint16_t*)dir)[off];
-	return ((const u32*)dir)[off];
+		return gload((const uint16_t*)dir, off);
+	return gload((const u32*)dir, off);
 }
 
 // The 32-bit word holding key offset `off`, and the bit mask of its entry.
@@ -88,7 +88,7 @@ __device__ __forceinline__ void load_col_u(const void* p, int t, const Chunk& C,
 {
 #define VH_LOADU(T)                                                   \
 	_Pragma("unroll") for (int u = 0; u < kU; u++)                    \
-		out[u] = C.alive[u] ? (i64)((const T*)p)[C.row[u]] : 0;
+		out[u] = C.alive[u] ? (i64)gload((const T*)p, C.row[u]) : 0;
 	switch (t) {
 	case VH_U8: VH_LOADU(uint8_t); break;
 	case VH_I8: VH_LOADU(int8_t); break;
@@ -256,17 +256,17 @@ __device__ __forceinline__ i64 locate(const Plan& P, const Lds& L, u64 g)
 	if (MODE == GB_HASH) {
 		u64 h = voila_hash(g) & P.hmask;
 		for (u64 n = 0; n <= P.hmask && n < kMaxProbe; n++) {
-			const u64 k = P.hkeys[h];
+			const u64 k = gload(P.hkeys, (i64)h);
 			if (k == g)
 				return (i64)h;
 			if (k == kEmpty) {
-				const u64 old = atomicCAS(&P.hkeys[h], kEmpty, g);
+				const u64 old = gatomic_cas(&P.hkeys[h], kEmpty, g);
 				if (old == kEmpty || old == g)
 					return (i64)h;
 			}
 			h = (h + 1) & P.hmask;
 		}
-		atomicOr(P.error, 1u);   // group hash table full
+		gatomic_or(P.error, 1u);   // group hash table full
 		return -1;
 	}
 	return (i64)g;
@@ -289,7 +289,7 @@ __device__ __forceinline__ void add_global(u64* lo, i64* hi, u64 v, bool exact, 
 	if (exact)
 		atomic_add_i128(lo, hi, v, (sgn && (i64)v < 0) ? -1 : 0);
 	else
-		atomicAdd(lo, v);
+		gatomic_add(lo, v);
 }
 
 template <int MODE>
@@ -302,9 +302,9 @@ __device__ __forceinline__ void add_count(const Plan& P, const Lds& L, u64 g, i6
 	else if (MODE == GB_LDSHASH && slot >= 0)
 		atomicAdd(&L.cnt[slot], 1u);
 	else if (MODE == GB_HASH)
-		atomicAdd(&P.out_cnt[slot], 1ull);
+		gatomic_add(&P.out_cnt[slot], 1ull);
 	else
-		atomicAdd(&P.out_cnt[g], 1ull);
+		gatomic_add(&P.out_cnt[g], 1ull);
 }
 
 template <int MODE>
@@ -339,22 +339,22 @@ __device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 
 // atomic per wave), the group index, then each sum's argument; the
 // partition's count in LDS.  Rows beyond the space (JoinN fan-out) add into
 // the global arrays.
-__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, u64*& t)
+__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, VH_GLOBAL u64*& t)
 {
 	t = nullptr;
 	if (!live)
 		return;
 	const u32 pos = wave_append((u32*)L.hi);
 	if (pos < L.rcap) {
-		t = P.gtup0 + (u64)(L.rbase + pos) * (u64)(P.nsums + 1);
+		t = (VH_GLOBAL u64*)P.gtup0 + (u64)(L.rbase + pos) * (u64)(P.nsums + 1);
 		t[0] = g;
 		atomicAdd(&L.cnt[g >> P.gshift], 1u);
 	} else {
-		atomicAdd(&P.out_cnt[g], 1ull);
+		gatomic_add(&P.out_cnt[g], 1ull);
 	}
 }
 
-__device__ __forceinline__ void part_tuple_value(const Plan& P, u64* t, int s, u64 g, u64 v)
+__device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL u64* t, int s, u64 g, u64 v)
 {
 	if (t) {
 		t[1 + s] = v;
@@ -368,13 +368,13 @@ __device__ __forceinline__ void part_tuple_value(const Plan& P, u64* t, int s, u
 __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, const Chunk& C, const u64 g[kU],
 	const bool ok[kU])
 {
-	u64* t[kU];
+	VH_GLOBAL u64* t[kU];
 	bool live[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
 		live[u] = C.alive[u];
 		if (live[u] && (!ok[u] || g[u] >= P.domain)) {
-			atomicOr(P.error, 8u);   // group key outside its column's [min, max]
+			gatomic_or(P.error, 8u);   // group key outside its column's [min, max]
 			live[u] = false;
 		}
 		part_tuple_slot(P, L, g[u], live[u], t[u]);
@@ -560,7 +560,7 @@ __device__ __forceinline__ void consume_u(const Plan& P, const Lds& L, const Chu
 		if (!live[u])
 			continue;
 		if (!ok[u] || (MODE != GB_HASH && g[u] >= P.domain)) {
-			atomicOr(P.error, 8u);   // group key outside its column's [min, max]
+			gatomic_or(P.error, 8u);   // group key outside its column's [min, max]
 			live[u] = false;
 			continue;
 		}
@@ -641,7 +641,7 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 			} else {
 #pragma unroll
 			for (int u = 0; u < kU; u++)
-				s[u] = C.alive[u] ? J.slots[h[u]] : kEmpty;
+				s[u] = C.alive[u] ? gload(J.slots, (i64)h[u]) : kEmpty;
 #pragma unroll
 			for (int u = 0; u < kU; u++) {
 				if (!C.alive[u])
@@ -650,10 +650,10 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				while (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {
 					n++;
 					h[u] = (h[u] + 1) & J.mask;
-					s[u] = J.slots[h[u]];
+					s[u] = gload(J.slots, (i64)h[u]);
 				}
 				if (n >= kMaxProbe)
-					atomicOr(P.error, 32u);
+					gatomic_or(P.error, 32u);
 				if (s[u] == kEmpty || (u32)s[u] != off[u])
 					C.alive[u] = false;
 				else
@@ -665,7 +665,7 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 #pragma unroll
 				for (int u = 0; u < kU; u++)
 					if (C.alive[u])
-						atomicOr(P.error, 16u);
+						gatomic_or(P.error, 16u);
 			}
 			if (J.last) {
 #pragma unroll
@@ -693,10 +693,10 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				u64 h = (h_last[u] + 1) & J.mask;
 				for (u64 n = 0;; n++) {
 					if (n >= kMaxProbe) {
-						atomicOr(P.error, 32u);
+						gatomic_or(P.error, 32u);
 						break;
 					}
-					const u64 s = J.slots[h];
+					const u64 s = gload(J.slots, (i64)h);
 					if (s == kEmpty)
 						break;
 					if ((u32)s == off_last[u]) {
@@ -796,7 +796,7 @@ __device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
 		const u32 key = L.keys[i];
 		if (key == 0xFFFFFFFFu)
 			continue;
-		atomicAdd(&P.out_cnt[key], (u64)L.cnt[i]);
+		gatomic_add(&P.out_cnt[key], (u64)L.cnt[i]);
 		for (int s = 0; s < P.nsums; s++) {
 			const SumArg& S = P.sums[s];
 			const u64 lo = L.lo[s * kLdsHashSlots + i];
@@ -806,7 +806,7 @@ __device__ __forceinline__ void ldshash_flush(const Plan& P, const Lds& L)
 			if (S.res128)
 				atomic_add_i128(&P.out_lo[idx], &P.out_hi[idx], lo, hi);
 			else
-				atomicAdd(&P.out_lo[idx], lo);
+				gatomic_add(&P.out_lo[idx], lo);
 		}
 	}
 }
@@ -876,7 +876,7 @@ __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 					if (!P.flush)
 						mine[(2 * D * P.nsums + g) * nb] = c;
 					else if (c)
-						atomicAdd(&P.out_cnt[g], c);
+						gatomic_add(&P.out_cnt[g], c);
 				}
 			}
 		}
@@ -898,7 +898,7 @@ __device__ __forceinline__ void block_epilogue(const Plan& P, const Lds& L)
 			if (!P.flush)
 				mine[(2 * n + i) * nb] = L.cnt[i];
 			else if (L.cnt[i])
-				atomicAdd(&P.out_cnt[i], (u64)L.cnt[i]);
+				gatomic_add(&P.out_cnt[i], (u64)L.cnt[i]);
 		}
 	}
 }
@@ -912,7 +912,7 @@ __device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
 		const u32 c = L.cnt[g];
 		if (!c)
 			continue;
-		atomicAdd(&P.out_cnt[g], (u64)c);
+		gatomic_add(&P.out_cnt[g], (u64)c);
 		for (int s = 0; s < P.nsums; s++) {
 			const SumArg& S = P.sums[s];
 			const u64 lo = L.lo[s * D + g];
@@ -920,7 +920,7 @@ __device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
 			if (S.res128)
 				atomic_add_i128(&P.out_lo[s * D + g], &P.out_hi[s * D + g], lo, hi);
 			else
-				atomicAdd(&P.out_lo[s * D + g], lo);
+				gatomic_add(&P.out_lo[s * D + g], lo);
 		}
 	}
 }
@@ -959,7 +959,7 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 		const i64 key = ok[u] ? load1(B.key, B.key_type, row[u]) : B.kmin;
 		const u32 pay = (ok[u] && B.npay) ? (u32)(load1(B.pay, B.pay_type, row[u]) - B.pmin) : 0u;
 		if (ok[u] && (key < B.kmin || key > B.kmax)) {
-			atomicOr(B.error, 4u);
+			gatomic_or(B.error, 4u);
 			ok[u] = false;
 		}
 		off[u] = (u32)(key - B.kmin);
@@ -979,12 +979,12 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 		}
 #pragma unroll
 		for (int u = 0; u < kU; u++)
-			seen[u] = ok[u] ? atomicAnd(word[u], bits[u] | ~mask[u]) : 0xFFFFFFFFu;
+			seen[u] = ok[u] ? gatomic_and(word[u], bits[u] | ~mask[u]) : 0xFFFFFFFFu;
 		bool spill = false;
 #pragma unroll
 		for (int u = 0; u < kU; u++) {
 			if (ok[u] && (seen[u] & mask[u]) != mask[u])
-				atomicOr(word[u], seen[u] & mask[u]);   // a repeated key: restore
+				gatomic_or(word[u], seen[u] & mask[u]);   // a repeated key: restore
 			else
 				ok[u] = false;                          // first row of its key
 			spill = spill || ok[u];
@@ -995,20 +995,20 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 	}
 #pragma unroll
 	for (int u = 0; u < kU; u++)
-		old[u] = ok[u] ? atomicCAS(&B.slots[h[u]], kEmpty, val[u]) : kEmpty;
+		old[u] = ok[u] ? gatomic_cas(&B.slots[h[u]], kEmpty, val[u]) : kEmpty;
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
 		if (!ok[u])
 			continue;
 		for (u64 n = 0; old[u] != kEmpty; n++) {
 			if (n >= kMaxProbe || n >= B.mask) {
-				atomicOr(B.error, 32u);
+				gatomic_or(B.error, 32u);
 				break;
 			}
 			if ((u32)old[u] == off[u])
 				*B.dups = 1u;
 			h[u] = (h[u] + 1) & B.mask;
-			old[u] = atomicCAS(&B.slots[h[u]], kEmpty, val[u]);
+			old[u] = gatomic_cas(&B.slots[h[u]], kEmpty, val[u]);
 		}
 	}
 }

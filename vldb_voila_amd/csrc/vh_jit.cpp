@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -50,6 +51,16 @@ const char* ctype_of(int t)
 	}
 }
 
+int host_width(int t)
+{
+	switch (t) {
+	case VH_U8: case VH_I8: return 1;
+	case VH_U16: case VH_I16: return 2;
+	case VH_U32: case VH_I32: return 4;
+	default: return 8;
+	}
+}
+
 struct Col {
 	const void* ptr;
 	int type;
@@ -59,9 +70,11 @@ struct Col {
 
 struct Gen {
 	const Plan& P;
-	bool eager;
-	bool pipe;          // load the next chunk's columns while this one's joins resolve
+	bool eager;         // scalar path: load every column before the predicates
+	bool pipe;          // vector path: the next chunk's columns are requested
+	                    // behind this chunk's last table lookup
 	int U;
+	bool vec = false;   // vector main loop (decided in source())
 	std::vector<Col> cols;
 	// plan fields the kernel reads, copied into locals before the row loop
 	// (the loop's atomics would otherwise make the compiler reload them)
@@ -69,8 +82,7 @@ struct Gen {
 	std::string decls;
 	std::string out;
 
-	Gen(const Plan& p, bool e, bool pp, int u) : P(p), eager(e || pp), pipe(pp), U(u) {}
-	bool next_emitted = false;
+	Gen(const Plan& p, bool e, bool pp, int u) : P(p), eager(e), pipe(pp), U(u) {}
 
 	void line(const std::string& s) { out += s; out += '\n'; }
 
@@ -159,33 +171,39 @@ struct Gen {
 
 	// Aggregate one row (u, with the last join's payload tp): group index,
 	// count, sums; add_count / add_sum of vh_fused.hpp with the plan's
-	// flags as literals for the dense modes.
+	// flags as literals for the dense modes.  Dense group indexes are below
+	// 2^26 (the lowering's limits), so they are computed in 32 bits; the
+	// last key's multiplier is 1 by construction of the mixed radix.
 	std::string consume_body(int mode, int tail_join, const char* tail_pay)
 	{
 		std::string s;
 		char b[1024];
 		const std::string err = uniform("u32*", "P.error");
 		const std::string dom = uniform("u64", "P.domain");
-		s += "\t\t\tu64 g = 0;\n\t\t\tbool ok = true;\n";
+		const bool hash = mode == GB_HASH;
+		s += hash ? "\t\t\tu64 g = 0;\n" : "\t\t\tu32 g = 0;\n";
+		s += "\t\t\tbool ok = true;\n";
 		for (int k = 0; k < P.nkeys; k++) {
 			snprintf(b, sizeof b, "P.keys[%d]", k);
 			const std::string kp = b;
 			s += "\t\t\t{\n\t\t\t\tconst u64 o = (u64)(" + role(P.keys[k].r, kp + ".r", tail_join, tail_pay) + " - " +
 				uniform("i64", kp + ".min") + ");\n";
 			s += "\t\t\t\tok = ok && o <= " + uniform("u64", kp + ".range") + ";\n";
-			if (mode == GB_HASH)
+			if (hash)
 				s += "\t\t\t\tg += o << " + uniform("int", kp + ".shift") + ";\n";
+			else if (k == P.nkeys - 1)
+				s += "\t\t\t\tg += (u32)o;\n";
 			else
-				s += "\t\t\t\tg += o * " + uniform("u64", kp + ".mult") + ";\n";
+				s += "\t\t\t\tg += (u32)o * (u32)" + uniform("u64", kp + ".mult") + ";\n";
 			s += "\t\t\t}\n";
 		}
-		if (mode == GB_HASH)
-			s += "\t\t\tif (!ok) { atomicOr(" + err + ", 8u); return; }\n";
+		if (hash)
+			s += "\t\t\tif (!ok) { gatomic_or(" + err + ", 8u); return; }\n";
 		else
-			s += "\t\t\tif (!ok || g >= " + dom + ") { atomicOr(" + err + ", 8u); return; }\n";
+			s += "\t\t\tif (!ok || (u64)g >= " + dom + ") { gatomic_or(" + err + ", 8u); return; }\n";
 		if (mode == GB_PART) {
 			// one tuple: a slot in the region's tuple space, then each sum's argument
-			s += "\t\t\tu64* t;\n\t\t\tpart_tuple_slot(P, L, g, true, t);\n";
+			s += "\t\t\tVH_GLOBAL u64* t;\n\t\t\tpart_tuple_slot(P, L, g, true, t);\n";
 			for (int si = 0; si < P.nsums; si++) {
 				snprintf(b, sizeof b, "\t\t\tpart_tuple_value(P, t, %d, g, %s);\n", si,
 					sum_expr(si, tail_join, tail_pay).c_str());
@@ -206,7 +224,7 @@ struct Gen {
 		} else if (mode == GB_LDS) {
 			s += "\t\t\tatomicAdd(&L.cnt[g], 1u);\n";
 		} else {
-			s += "\t\t\tatomicAdd(&" + uniform("u64*", "P.out_cnt") + "[g], 1ull);\n";
+			s += "\t\t\tgatomic_add(&" + uniform("u64*", "P.out_cnt") + "[g], 1ull);\n";
 		}
 		for (int si = 0; si < P.nsums; si++) {
 			const SumArg& S = P.sums[si];
@@ -219,7 +237,7 @@ struct Gen {
 				s += b;
 			} else if (mode == GB_REG) {
 				snprintf(b, sizeof b,
-					"\t\t\t{\n\t\t\t\tconst u64 v = %s;\n\t\t\t\tconst u64 idx = (g * %d + %d) * blockDim.x + threadIdx.x;\n"
+					"\t\t\t{\n\t\t\t\tconst u64 v = %s;\n\t\t\t\tconst u32 idx = (g * %du + %du) * blockDim.x + threadIdx.x;\n"
 					"\t\t\t\tconst u64 o = L.lo[idx];\n\t\t\t\tconst u64 n = o + v;\n\t\t\t\tL.lo[idx] = n;\n",
 					v.c_str(), P.nsums, si);
 				s += b;
@@ -227,7 +245,7 @@ struct Gen {
 					s += std::string("\t\t\t\tL.hi[idx] += (int)(n < o) - (int)(") + sgn + " && (i64)v < 0);\n";
 				s += "\t\t\t}\n";
 			} else if (mode == GB_LDS) {
-				snprintf(b, sizeof b, "\t\t\t{\n\t\t\t\tconst u64 idx = (u64)%d * %s + g;\n"
+				snprintf(b, sizeof b, "\t\t\t{\n\t\t\t\tconst u32 idx = %du * (u32)%s + g;\n"
 					"\t\t\t\tadd_lds_exact(&L.lo[idx], &L.hi[idx], %s, %s, %s);\n\t\t\t}\n",
 					si, dom.c_str(), v.c_str(), track ? "true" : "false", sgn);
 				s += b;
@@ -240,6 +258,222 @@ struct Gen {
 			}
 		}
 		return s;
+	}
+
+	std::string col_ptr(int i)
+	{
+		const std::string ty = ctype_of(cols[i].type);
+		return uniform(gptr(ty), "(" + gptr(ty) + ")" + cols[i].path);
+	}
+
+	// Loads of the vector path: pair p of the lane holds rows
+	// chunk * 64U + 128p + 2 lane + {0, 1}; each instruction of a wave reads
+	// a contiguous range (128 rows), streamed past the caches (nontemporal).
+	void load_pairs(const std::string& chunk)
+	{
+		char b[512];
+		line("\t\t{");
+		line("\t\t\tconst i64 pb = " + chunk + " * (32 * U) + lane;");
+		for (size_t i = 0; i < cols.size(); i++) {
+			const std::string ty = ctype_of(cols[i].type);
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\t\tfor (int p = 0; p < U / 2; p++) q%zu[p] = __builtin_nontemporal_load(&((%s)%s)[pb + p * 64]);",
+				i, gptr("vh2_" + ty).c_str(), col_ptr((int)i).c_str());
+			line(b);
+		}
+		line("\t\t}");
+	}
+
+	// Predicates, joins and the aggregation of the rows of one chunk whose
+	// columns c<i>[U] are loaded (vector path) or are loaded here (scalar
+	// path: c<i>[u] = column[row[u]] for live rows, row 0 otherwise).
+	// `next` requests the next chunk's columns (vector path).
+	void body(int mode, bool scalar)
+	{
+		char b[1024];
+		const int last = P.njoins - 1;
+		const bool tail = P.njoins > 0 && P.joins[last].variant == VH_JOINN;
+		const std::string err = uniform("u32*", "P.error");
+		bool next_done = !( !scalar && pipe);
+		auto emit_next = [&]() {
+			if (next_done)
+				return;
+			next_done = true;
+			// A wave's last iteration loads chunk 0 instead, which every
+			// wave reads then, so those loads share lines in L2 instead of
+			// re-reading each wave's own chunk from memory; no branch, which
+			// would make the compiler wait for all loads where paths join.
+			line("\t\t__builtin_amdgcn_sched_barrier(0);");
+			load_pairs("(w + nwv < nfull ? w + nwv : 0)");
+			line("\t\t__builtin_amdgcn_sched_barrier(0);");
+		};
+		auto load = [&](int i) {
+			if (!scalar)
+				return;   // loaded before the body
+			const std::string ty = ctype_of(cols[i].type);
+			snprintf(b, sizeof b, "\t\t%s c%d[U];", ty.c_str(), i);
+			line(b);
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%d[u] = %s[VH_LIVE(u) ? row[u] : 0];", i,
+				col_ptr(i).c_str());
+			line(b);
+		};
+		for (size_t i = 0; i < cols.size(); i++)
+			if (cols[i].pred)
+				load((int)i);
+		if (eager || !scalar)
+			for (size_t i = 0; i < cols.size(); i++)
+				if (!cols[i].pred)
+					load((int)i);
+		for (int p = 0; p < P.npreds; p++) {
+			snprintf(b, sizeof b, "P.preds[%d].r", p);
+			const std::string v = role(P.preds[p].r, b);
+			snprintf(b, sizeof b, "P.preds[%d].value", p);
+			const std::string val = uniform("i64", b);
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) VH_KILL(u, !compare(%d, %d, %s, %s));",
+				P.preds[p].cmp, P.preds[p].r.type, v.c_str(), val.c_str());
+			line(b);
+		}
+		if (!eager && scalar)
+			for (size_t i = 0; i < cols.size(); i++)
+				if (!cols[i].pred)
+					load((int)i);
+		if (P.njoins == 0)
+			emit_next();
+
+		// joins
+		for (int j = 0; j < P.njoins; j++) {
+			const JoinT& J = P.joins[j];
+			snprintf(b, sizeof b, "P.joins[%d]", j);
+			const std::string jp = b;
+			const std::string kmin = uniform("i64", jp + ".kmin"), kmax = uniform("i64", jp + ".kmax");
+			const std::string mask = uniform("u64", jp + ".mask");
+			const std::string slots = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".slots");
+			snprintf(b, sizeof b, "\t\tu32 pay%d[U];", j);
+			line(b);
+			line("\t\t{");
+			line("\t\t\tu32 off[U];");
+			line("#pragma unroll");
+			line("\t\t\tfor (int u = 0; u < U; u++) {");
+			line("\t\t\t\tconst i64 key = " + role(J.key, jp + ".key") + ";");
+			line("\t\t\t\tVH_KILL(u, key < " + kmin + " || key > " + kmax + ");");
+			line("\t\t\t\toff[u] = VH_LIVE(u) ? (u32)(key - " + kmin + ") : 0u;");
+			line("\t\t\t}");
+			if (J.dir_w) {
+				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
+				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
+				const std::string dir = uniform(gptr(et), "(" + gptr(et) + ")" + jp + ".dir");
+				if (J.lds == LDS_BITS) {
+					// keys without a row end here, in LDS; dead rows then read entry 0
+					line("#pragma unroll");
+					snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) VH_KILL(u, !((tb%d[off[u] >> 5] >> (off[u] & 31u)) & 1u));", j);
+					line(b);
+					line("#pragma unroll");
+					line("\t\t\tfor (int u = 0; u < U; u++) off[u] = VH_LIVE(u) ? off[u] : 0u;");
+				}
+				line("#pragma unroll");
+				if (J.lds == LDS_DIR)
+					snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = (u32)td%d[off[u]];", j, j);
+				else
+					snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = (u32)%s[off[u]];", j, dir.c_str());
+				line(b);
+				if (j == last)
+					emit_next();
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) VH_KILL(u, pay%d[u] == %s);", j, ev);
+				line(b);
+			} else {
+				line("\t\t\tu64 h[U], s[U];");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + mask + ";");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = " + slots + "[h[u]];");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) {");
+				snprintf(b, sizeof b, "\t\t\t\tpay%d[u] = 0;", j);
+				line(b);
+				line("\t\t\t\tif (!VH_LIVE(u)) continue;");
+				line("\t\t\t\tu64 n = 0;");
+				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {");
+				line("\t\t\t\t\tn++;");
+				line("\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
+				line("\t\t\t\t\ts[u] = " + slots + "[h[u]];");
+				line("\t\t\t\t}");
+				line("\t\t\t\tif (n >= kMaxProbe) gatomic_or(" + err + ", 32u);");
+				line("\t\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) VH_KILL(u, 1);");
+				snprintf(b, sizeof b, "\t\t\t\telse pay%d[u] = (u32)(s[u] >> 32);", j);
+				line(b);
+				line("\t\t\t}");
+				if (j == last)
+					emit_next();
+			}
+			if (j != last && J.variant == VH_JOINN) {
+				// a first-match join may not fan out: reported, as the generic kernel does
+				snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u)", j);
+				line(b);
+				line("#pragma unroll");
+				line("\t\t\t\tfor (int u = 0; u < U; u++) if (VH_LIVE(u)) gatomic_or(" + err + ", 16u);");
+			}
+			if (j == last) {
+				// aggregate inside the last join's scope (its tail needs off / h)
+				line("\t\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
+				out += consume_body(mode, last, "tp");
+				line("\t\t\t};");
+				line("#pragma unroll");
+				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (VH_LIVE(u)) consume(u, pay%d[u]);", j);
+				line(b);
+				if (tail) {
+					// further matches: after the first one in the probe sequence
+					// (hash layout) or in the overflow slots from voila_hash(off)
+					// on (direct layout); none without a repeated build key
+					snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u) {", j);
+					line(b);
+					line("#pragma unroll");
+					line("\t\t\t\tfor (int u = 0; u < U; u++) {");
+					line("\t\t\t\t\tif (!VH_LIVE(u)) continue;");
+					if (J.dir_w)
+						line("\t\t\t\t\tu64 hh = voila_hash((u64)off[u]) & " + mask + ";");
+					else
+						line("\t\t\t\t\tu64 hh = (h[u] + 1) & " + mask + ";");
+					line("\t\t\t\t\tfor (u64 n = 0;; n++) {");
+					line("\t\t\t\t\t\tif (n >= kMaxProbe) { gatomic_or(" + err + ", 32u); break; }");
+					line("\t\t\t\t\t\tconst u64 sv = " + slots + "[hh];");
+					line("\t\t\t\t\t\tif (sv == kEmpty) break;");
+					line("\t\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
+					line("\t\t\t\t\t\thh = (hh + 1) & " + mask + ";");
+					line("\t\t\t\t\t}");
+					line("\t\t\t\t}");
+					line("\t\t\t}");
+				}
+			}
+			line("\t\t}");
+		}
+		if (P.njoins == 0) {
+			line("\t\t{");
+			line("\t\t\tauto consume = [&](int u) __attribute__((always_inline)) {");
+			out += consume_body(mode, -1, nullptr);
+			line("\t\t\t};");
+			line("#pragma unroll");
+			line("\t\t\tfor (int u = 0; u < U; u++) if (VH_LIVE(u)) consume(u);");
+			line("\t\t}");
+		}
+	}
+
+	// Rows [first, end) of the scalar path: chunks of U * blockDim rows, row
+	// base + u * blockDim + thread, every `stride` rows.
+	void scalar_loop(int mode, const std::string& first, const std::string& end, const std::string& stride)
+	{
+		line("\tfor (i64 base = " + first + "; base < " + end + "; base += " + stride + ") {");
+		line("\t\ti64 row[U];");
+		line("\t\tu32 live = 0;");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) {");
+		line("\t\t\trow[u] = base + (i64)u * blockDim.x + threadIdx.x;");
+		line("\t\t\tlive |= (u32)(row[u] < " + end + ") << u;");
+		line("\t\t}");
+		body(mode, true);
+		line("\t}");
 	}
 
 	std::string source(int mode)
@@ -268,228 +502,61 @@ struct Gen {
 				if (T.sb)
 					note_role(T.b, std::string(b) + ".b", false);
 			}
+		// The vector main loop needs every probe column aligned to a pair of
+		// its values, an even U and a grid-strided row mapping (not GB_PART's
+		// slab per workgroup); otherwise every row takes the scalar path.
+		vec = eager && U % 2 == 0 && mode != GB_PART;
+		int row_bytes = 0;
+		for (const Col& c : cols) {
+			vec = vec && ((uintptr_t)c.ptr % (uintptr_t)(2 * host_width(c.type))) == 0;
+			row_bytes += host_width(c.type);
+		}
+		if (U == 0) {
+			// measured on MI355X: 8 rows per lane for 16 (c1 probe) and 32
+			// bytes (star fact) of columns per row, 2 for 36 (filter)
+			U = vec ? (row_bytes <= 32 ? 8 : 2) : 4;
+		}
 
-		const int last = P.njoins - 1;
-		const bool tail = P.njoins > 0 && P.joins[last].variant == VH_JOINN;
-		const std::string err = uniform("u32*", "P.error");
-
-		// ---- the row loop (declarations of hoisted plan fields go first) ----
-		// Loads are unconditional so they issue back to back: rows that do
-		// not qualify read row 0 (a line every such lane shares) and their
-		// value is discarded.  A load under a per-lane branch would be
-		// followed by a wait before the branch rejoins.
-		auto col_ptr = [&](int i) {
-			const std::string ty = ctype_of(cols[i].type);
-			return uniform(gptr(ty), "(" + gptr(ty) + ")" + cols[i].path);
-		};
-		// rows of this workgroup: every tile-th chunk of the table, or (GB_PART)
-		// one contiguous slab, the region its tuples are kept for
 		if (mode == GB_PART) {
+			// rows of this workgroup: one contiguous slab, the region its tuples are kept for
 			line("\tconst i64 first = (i64)blockIdx.x * P.gregion_rows;");
 			line("\tconst i64 end = first + P.gregion_rows < rows ? first + P.gregion_rows : rows;");
-			line("\tconst i64 stride = tile;");
 			line("\tL.rbase = first;");
 			line("\tL.rcap = end > first ? (u32)(end - first) : 0u;");
 			line("\tL.region = blockIdx.x;");
+			scalar_loop(mode, "first", "end", "tile");
+		} else if (!vec) {
+			scalar_loop(mode, "(i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
 		} else {
-			line("\tconst i64 first = (i64)blockIdx.x * tile;");
-			line("\tconst i64 end = rows;");
-			line("\tconst i64 stride = (i64)gridDim.x * tile;");
-		}
-		if (pipe) {
-			// software pipeline: chunk i+1's columns are requested after chunk
-			// i's first join lookups, so their latency overlaps that lookup and
-			// the aggregation (loads retire in order: the lookup is waited for
-			// without waiting for them)
-			line("\ti64 base = first;");
+			// full chunks of 64U rows, one per wave at a time, then the
+			// remaining rows (fewer than 64U) on the scalar path
+			line("\tconst int lane = threadIdx.x & 63;");
+			line("\tconst i64 wv = (i64)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + "
+				"(threadIdx.x >> 6)));");
+			line("\tconst i64 nwv = (i64)gridDim.x * (blockDim.x >> 6);");
+			line("\tconst i64 nfull = rows / (64 * U);");
+			line("\tif (wv < nfull) {");
 			for (size_t i = 0; i < cols.size(); i++) {
-				snprintf(b, sizeof b, "\t%s c%zu[U];", ctype_of(cols[i].type), i);
-				line(b);
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\tfor (int u = 0; u < U; u++) { const i64 r = base + (i64)u * blockDim.x + "
-					"threadIdx.x; c%zu[u] = %s[r < end ? r : 0]; }", i, col_ptr((int)i).c_str());
+				snprintf(b, sizeof b, "\t\tvh2_%s q%zu[U / 2];", ctype_of(cols[i].type), i);
 				line(b);
 			}
-			line("\tfor (; base < end; base += stride) {");
-		} else {
-			line("\tfor (i64 base = first; base < end; base += stride) {");
-		}
-		auto emit_next = [&]() {
-			if (!pipe || next_emitted)
-				return;
-			next_emitted = true;
-			line("\t\t__builtin_amdgcn_sched_barrier(0);");
-			line("\t\tconst i64 nbase = base + stride;");
+			load_pairs("wv");
+			line("\tfor (i64 w = wv; w < nfull; w += nwv) {");
 			for (size_t i = 0; i < cols.size(); i++) {
+				snprintf(b, sizeof b, "\t\t%s c%zu[U];", ctype_of(cols[i].type), i);
+				line(b);
 				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) { const i64 r = nbase + (i64)u * blockDim.x + "
-					"threadIdx.x; n%zu[u] = %s[r < end ? r : 0]; }", i, col_ptr((int)i).c_str());
+				snprintf(b, sizeof b, "\t\tfor (int p = 0; p < U / 2; p++) { c%zu[2 * p] = q%zu[p].x; c%zu[2 * p + 1] = "
+					"q%zu[p].y; }", i, i, i, i);
 				line(b);
 			}
-			line("\t\t__builtin_amdgcn_sched_barrier(0);");
-		};
-		if (pipe)
-			for (size_t i = 0; i < cols.size(); i++) {
-				snprintf(b, sizeof b, "\t\t%s n%zu[U];", ctype_of(cols[i].type), i);
-				line(b);
-			}
-		line("\t\ti64 row[U];");
-		line("\t\tbool alive[U];");
-		line("#pragma unroll");
-		line("\t\tfor (int u = 0; u < U; u++) {");
-		line("\t\t\trow[u] = base + (i64)u * blockDim.x + threadIdx.x;");
-		line("\t\t\talive[u] = row[u] < end;");
-		line("\t\t}");
-		auto load = [&](int i) {
-			if (pipe)
-				return;   // loaded by the previous iteration (or before the loop)
-			const std::string ty = ctype_of(cols[i].type);
-			snprintf(b, sizeof b, "\t\t%s c%d[U];", ty.c_str(), i);
+			snprintf(b, sizeof b, "\t\tu32 live = 0x%xu;", (unsigned)((1ull << U) - 1));
 			line(b);
-			line("#pragma unroll");
-			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%d[u] = %s[alive[u] ? row[u] : 0];", i,
-				col_ptr(i).c_str());
-			line(b);
-		};
-		for (size_t i = 0; i < cols.size(); i++)
-			if (cols[i].pred)
-				load((int)i);
-		if (eager)
-			for (size_t i = 0; i < cols.size(); i++)
-				if (!cols[i].pred)
-					load((int)i);
-		for (int p = 0; p < P.npreds; p++) {
-			snprintf(b, sizeof b, "P.preds[%d].r", p);
-			const std::string v = role(P.preds[p].r, b);
-			snprintf(b, sizeof b, "P.preds[%d].value", p);
-			const std::string val = uniform("i64", b);
-			line("#pragma unroll");
-			snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) alive[u] = alive[u] && compare(%d, %d, %s, %s);",
-				P.preds[p].cmp, P.preds[p].r.type, v.c_str(), val.c_str());
-			line(b);
+			body(mode, false);
+			line("\t}");
+			line("\t}");
+			scalar_loop(mode, "nfull * (64 * U) + (i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
 		}
-		if (!eager)
-			for (size_t i = 0; i < cols.size(); i++)
-				if (!cols[i].pred)
-					load((int)i);
-
-		// joins
-		for (int j = 0; j < P.njoins; j++) {
-			const JoinT& J = P.joins[j];
-			snprintf(b, sizeof b, "P.joins[%d]", j);
-			const std::string jp = b;
-			const std::string kmin = uniform("i64", jp + ".kmin"), kmax = uniform("i64", jp + ".kmax");
-			const std::string mask = uniform("u64", jp + ".mask");
-			const std::string slots = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".slots");
-			snprintf(b, sizeof b, "\t\tu32 pay%d[U];", j);
-			line(b);
-			line("\t\t{");
-			line("\t\t\tu32 off[U];");
-			line("#pragma unroll");
-			line("\t\t\tfor (int u = 0; u < U; u++) {");
-			line("\t\t\t\tconst i64 key = " + role(J.key, jp + ".key") + ";");
-			line("\t\t\t\tif (key < " + kmin + " || key > " + kmax + ") alive[u] = false;");
-			line("\t\t\t\toff[u] = alive[u] ? (u32)(key - " + kmin + ") : 0u;");
-			line("\t\t\t}");
-			if (J.dir_w) {
-				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
-				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
-				const std::string dir = uniform(gptr(et), "(" + gptr(et) + ")" + jp + ".dir");
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) pay%d[u] = (u32)%s[off[u]];", j, dir.c_str());
-				line(b);
-				if (j == 0)
-					emit_next();
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (pay%d[u] == %s) alive[u] = false;", j, ev);
-				line(b);
-				if (j == last && tail) {
-					// repeated keys sit in the overflow slots from voila_hash(off) on
-					line("\t\t\tu64 th[U];");
-					line("#pragma unroll");
-					line("\t\t\tfor (int u = 0; u < U; u++) th[u] = voila_hash((u64)off[u]) & " + mask + ";");
-				}
-			} else {
-				line("\t\t\tu64 h[U], s[U];");
-				line("#pragma unroll");
-				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + mask + ";");
-				line("#pragma unroll");
-				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = " + slots + "[h[u]];");
-				line("#pragma unroll");
-				line("\t\t\tfor (int u = 0; u < U; u++) {");
-				snprintf(b, sizeof b, "\t\t\t\tpay%d[u] = 0;", j);
-				line(b);
-				line("\t\t\t\tif (!alive[u]) continue;");
-				line("\t\t\t\tu64 n = 0;");
-				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {");
-				line("\t\t\t\t\tn++;");
-				line("\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
-				line("\t\t\t\t\ts[u] = " + slots + "[h[u]];");
-				line("\t\t\t\t}");
-				line("\t\t\t\tif (n >= kMaxProbe) atomicOr(" + err + ", 32u);");
-				line("\t\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) alive[u] = false;");
-				snprintf(b, sizeof b, "\t\t\t\telse pay%d[u] = (u32)(s[u] >> 32);", j);
-				line(b);
-				line("\t\t\t}");
-				if (j == last && tail) {
-					line("\t\t\tu64 th[U];");
-					line("#pragma unroll");
-					line("\t\t\tfor (int u = 0; u < U; u++) th[u] = (h[u] + 1) & " + mask + ";");
-				}
-			}
-			if (j != last && J.variant == VH_JOINN) {
-				// a first-match join may not fan out: reported, as the generic kernel does
-				snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u)", j);
-				line(b);
-				line("#pragma unroll");
-				line("\t\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) atomicOr(" + err + ", 16u);");
-			}
-			if (j == last) {
-				emit_next();
-				// aggregate inside the last join's scope (its tail needs th / off)
-				line("\t\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
-				out += consume_body(mode, last, "tp");
-				line("\t\t\t};");
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u, pay%d[u]);", j);
-				line(b);
-				if (tail) {
-					snprintf(b, sizeof b, "\t\t\tif ((dups_mask >> %d) & 1u) {", j);
-					line(b);
-					line("#pragma unroll");
-					line("\t\t\t\tfor (int u = 0; u < U; u++) {");
-					line("\t\t\t\t\tif (!alive[u]) continue;");
-					line("\t\t\t\t\tu64 hh = th[u];");
-					line("\t\t\t\t\tfor (u64 n = 0;; n++) {");
-					line("\t\t\t\t\t\tif (n >= kMaxProbe) { atomicOr(" + err + ", 32u); break; }");
-					line("\t\t\t\t\t\tconst u64 sv = " + slots + "[hh];");
-					line("\t\t\t\t\t\tif (sv == kEmpty) break;");
-					line("\t\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
-					line("\t\t\t\t\t\thh = (hh + 1) & " + mask + ";");
-					line("\t\t\t\t\t}");
-					line("\t\t\t\t}");
-					line("\t\t\t}");
-				}
-			}
-			line("\t\t}");
-		}
-		if (P.njoins == 0) {
-			emit_next();
-			line("\t\t{");
-			line("\t\t\tauto consume = [&](int u) __attribute__((always_inline)) {");
-			out += consume_body(mode, -1, nullptr);
-			line("\t\t\t};");
-			line("#pragma unroll");
-			line("\t\t\tfor (int u = 0; u < U; u++) if (alive[u]) consume(u);");
-			line("\t\t}");
-		}
-		if (pipe)
-			for (size_t i = 0; i < cols.size(); i++) {
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\tfor (int u = 0; u < U; u++) c%zu[u] = n%zu[u];", i, i);
-				line(b);
-			}
-		line("\t}");
 		const std::string loop = out;
 
 		// ---- assemble ----
@@ -498,15 +565,43 @@ struct Gen {
 		for (const char* t : {"int8_t", "uint8_t", "int16_t", "uint16_t", "int32_t", "uint32_t", "int64_t", "uint64_t"})
 			k += std::string("typedef __hip_internal::") + t + " " + t + ";\n";
 		k += "#include \"vh_fused.hpp\"\nusing namespace vh;\n";
+		for (const char* t : {"int8_t", "uint8_t", "int16_t", "uint16_t", "int32_t", "uint32_t", "i64"})
+			k += std::string("typedef ") + t + " vh2_" + t + " __attribute__((ext_vector_type(2)));\n";
+		k += "#define VH_LIVE(u) ((live >> (u)) & 1u)\n";
+		k += "#define VH_KILL(u, c) (live &= ~((u32)(c) << (u)))\n";
 		k += "static constexpr int U = " + std::to_string(U) + ";\n";
 		k += "extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)\n{\n";
 		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
 		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
+		// join tables kept in LDS: copied by every workgroup (16-byte units;
+		// dir[] and the bitmap are allocated in whole units)
+		for (int j = 0; j < P.njoins; j++) {
+			const JoinT& J = P.joins[j];
+			if (!J.lds)
+				continue;
+			const std::string jp = "P.joins[" + std::to_string(j) + "]";
+			const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
+			const std::string keys = "((u64)(" + jp + ".kmax - " + jp + ".kmin) + 1)";
+			const std::string bytes = J.lds == LDS_DIR ? keys + " * " + std::to_string(J.dir_w) + "u"
+				: "(" + keys + " + 31) / 32 * 4";
+			const std::string src = J.lds == LDS_DIR ? jp + ".dir" : jp + ".bits";
+			k += "\t{\n\t\tconst u64 n = (" + bytes + " + 15) / 16;\n";
+			k += "\t\tu64* dst = (u64*)((char*)lds + " + jp + ".lds_off);\n";
+			k += "\t\tconst u64* src = (const u64*)" + src + ";\n";
+			k += "\t\tfor (u64 i = threadIdx.x; i < n; i += blockDim.x) { dst[2 * i] = gload(src, (i64)(2 * i)); "
+				"dst[2 * i + 1] = gload(src, (i64)(2 * i + 1)); }\n\t}\n";
+			if (J.lds == LDS_DIR)
+				k += "\tconst " + std::string(et) + "* const td" + std::to_string(j) + " = (const " + et +
+					"*)((const char*)lds + " + jp + ".lds_off);\n";
+			else
+				k += "\tconst u32* const tb" + std::to_string(j) + " = (const u32*)((const char*)lds + " + jp +
+					".lds_off);\n";
+		}
 		k += "\tu32 dups_mask = 0;\n";
 		for (int j = 0; j < P.njoins; j++)
 			if (P.joins[j].variant == VH_JOINN)
-				k += "\tif (*P.joins[" + std::to_string(j) + "].dups) dups_mask |= " + std::to_string(1u << j) + "u;\n";
+				k += "\tif (gload(P.joins[" + std::to_string(j) + "].dups, 0)) dups_mask |= " + std::to_string(1u << j) + "u;\n";
 		k += "\t(void)dups_mask;\n";
 		k += decls;
 		k += "\t__syncthreads();\n\tconst i64 rows = P.rows;\n\tconst i64 tile = (i64)U * blockDim.x;\n";

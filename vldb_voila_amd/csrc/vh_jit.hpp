@@ -10,24 +10,28 @@
 
 namespace vh {
 
-static constexpr int kJitUnroll = 4;   // default rows per lane per chunk of generated kernels
-
-// Rows per lane of the generated kernel a configuration asks for.
+// Rows per lane of the generated kernel a configuration asks for; 0 = the
+// generator's default (jit_source).
 inline int jit_unroll(const vh_config* cfg)
 {
-	return (cfg && cfg->unroll >= 1 && cfg->unroll <= 16) ? cfg->unroll : kJitUnroll;
+	return (cfg && cfg->unroll >= 1 && cfg->unroll <= 16) ? cfg->unroll : 0;
 }
 
 // Source of the fused probe kernel for P (its mode, joins, keys and sums).
-// eager: load every probe column before the predicates (else columns not
-// read by a predicate are loaded only for qualifying rows); unroll = rows
-// per lane per chunk; pipeline: request the next chunk's columns while this
-// chunk's join lookups are in flight (implies eager).
+// unroll = rows per lane per chunk (0: 8 for the vector loop when a probe
+// row is at most 32 bytes of columns, else 2; 4 for the scalar loop).  eager: every probe column is loaded
+// before the predicates; then, when every column is aligned to a pair of its
+// values and unroll is even, full chunks of 64 * unroll rows go through the
+// vector loop (2-row vector loads per lane, contiguous per instruction) and
+// only the last partial chunk through the scalar loop.  Not eager: scalar
+// loop, columns no predicate reads are loaded for qualifying rows only.
+// pipeline: the vector loop requests the next chunk's columns right behind
+// the last join's table lookups.
 std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline);
 
 // Generated-kernel options of a configuration's vh_kernel value.
 inline bool jit_eager(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
-inline bool jit_pipeline(int kernel) { return kernel == VH_KERNEL_JIT_PIPE; }
+inline bool jit_pipeline(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
 // Compile (or fetch from the cache) `src` for the current device.
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log);
 // Launch a compiled kernel over P (dP: its device copy).  GB_REG / GB_LDS

@@ -23,7 +23,15 @@ static constexpr int kMaxSliceBytes = 32768;     // LDS slice of a partition
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 // GB_REG / GB_LDS: merge the per-block partials of a `grid`-block launch
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s);
+// Dynamic LDS of the probe kernel: the group-by state, then the join tables
+// kept in LDS (JoinT::lds)
 size_t pipeline_lds_bytes(const Plan& P);
+// LDS of the group-by state alone (where the join tables start, rounded up)
+size_t groupby_lds_bytes(const Plan& P);
+// Bytes of join j's LDS copy (JoinT::lds), rounded up to 16
+size_t join_lds_bytes(const JoinT& J);
+// bits[k / 32] bit k % 32 = (dir entry k != all-ones), k < nkeys
+hipError_t launch_dir_bits(const void* dir, int w, u64 nkeys, u32* bits, hipStream_t s);
 // GB_PART: LDS of part_agg_kernel's slice (2^gshift groups)
 size_t part_slice_bytes(const Plan& P);
 

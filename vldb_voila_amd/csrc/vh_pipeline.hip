@@ -16,6 +16,8 @@
 // relalg_translator.cpp:933-1111) and the group-by loop (hash, bucket_lookup,
 // check, bucket_insert, scatter, aggr_sum / aggr_count,
 // relalg_translator.cpp:582-656).
+#include <algorithm>
+
 #include "vh_device.hpp"
 #include "vh_fused.hpp"
 #include "vh_kernels.hpp"
@@ -160,7 +162,7 @@ __device__ __forceinline__ bool build_sel(const BuildPlan& B, i64 row, u32& off)
 		return false;
 	const i64 key = load1(B.key, B.key_type, row);
 	if (key < B.kmin || key > B.kmax) {
-		atomicOr(B.error, 4u);
+		gatomic_or(B.error, 4u);
 		return false;
 	}
 	off = (u32)(key - B.kmin);
@@ -173,10 +175,10 @@ __device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64
 	u64 h = voila_hash((u64)off) & B.mask;
 	for (u64 n = 0;; n++) {
 		if (n >= kMaxProbe || n > B.mask) {
-			atomicOr(B.error, 32u);
+			gatomic_or(B.error, 32u);
 			return;
 		}
-		if (atomicCAS(&B.slots[h], kEmpty, val) == kEmpty)
+		if (gatomic_cas(&B.slots[h], kEmpty, val) == kEmpty)
 			return;
 		h = (h + 1) & B.mask;
 	}
@@ -334,17 +336,62 @@ hipError_t launch_build(const BuildPlan* B, int grid, hipStream_t s)
 	return hipGetLastError();
 }
 
+size_t groupby_lds_bytes(const Plan& P)
+{
+	size_t b = 0;
+	if (P.mode == GB_REG)
+		b = (size_t)(P.domain * P.nsums * kBlock * 12 + P.domain * kBlock * 4 + 16);
+	else if (P.mode == GB_LDS)
+		b = (size_t)(P.domain * P.nsums * 12 + P.domain * 4 + 16);
+	else if (P.mode == GB_LDSHASH)
+		b = (size_t)P.nsums * kLdsHashSlots * 12 + kLdsHashSlots * 8 + 16;
+	else if (P.mode == GB_PART)
+		b = (size_t)P.gparts * 8 + 16;
+	return (b + 15) & ~(size_t)15;
+}
+
+size_t join_lds_bytes(const JoinT& J)
+{
+	const u64 keys = (u64)(J.kmax - J.kmin) + 1;
+	u64 b = 0;
+	if (J.lds == LDS_DIR)
+		b = keys * (u64)J.dir_w;
+	else if (J.lds == LDS_BITS)
+		b = (keys + 31) / 32 * 4;
+	return (size_t)((b + 15) & ~15ull);
+}
+
 size_t pipeline_lds_bytes(const Plan& P)
 {
-	if (P.mode == GB_REG)
-		return (size_t)(P.domain * P.nsums * kBlock * 12 + P.domain * kBlock * 4 + 16);
-	if (P.mode == GB_LDS)
-		return (size_t)(P.domain * P.nsums * 12 + P.domain * 4 + 16);
-	if (P.mode == GB_LDSHASH)
-		return (size_t)P.nsums * kLdsHashSlots * 12 + kLdsHashSlots * 8 + 16;
-	if (P.mode == GB_PART)
-		return (size_t)P.gparts * 8 + 16;
-	return 0;
+	size_t end = groupby_lds_bytes(P);
+	for (int j = 0; j < P.njoins; j++)
+		if (P.joins[j].lds)
+			end = std::max(end, (size_t)P.joins[j].lds_off + join_lds_bytes(P.joins[j]));
+	return end;
+}
+
+__global__ __launch_bounds__(256) void dir_bits_kernel(const void* __restrict__ dir, int w, u64 nkeys,
+	u32* __restrict__ bits)
+{
+	const u64 nwords = (nkeys + 31) / 32;
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (u64)gridDim.x * blockDim.x) {
+		const u32 empty = dir_empty(w);
+		u32 word = 0;
+		for (u32 b = 0; b < 32; b++) {
+			const u64 k = i * 32 + b;
+			if (k < nkeys && dir_load(dir, w, (u32)k) != empty)
+				word |= 1u << b;
+		}
+		bits[i] = word;
+	}
+}
+
+hipError_t launch_dir_bits(const void* dir, int w, u64 nkeys, u32* bits, hipStream_t s)
+{
+	const u64 nwords = (nkeys + 31) / 32;
+	const unsigned grid = (unsigned)std::max<u64>(1, std::min<u64>((nwords + 255) / 256, 1024));
+	hipLaunchKernelGGL(dir_bits_kernel, dim3(grid), dim3(256), 0, s, dir, w, nkeys, bits);
+	return hipGetLastError();
 }
 
 size_t part_slice_bytes(const Plan& P)

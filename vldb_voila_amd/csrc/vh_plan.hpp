@@ -75,7 +75,17 @@ struct JoinT {
 	int last;            // last join of the pipeline: JoinN may match many
 	int pad;
 	const u32* dups;     // set by the build when a key occurs twice
+	// generated probe kernels: a copy of the table in LDS at byte lds_off of
+	// the dynamic LDS, made by every workgroup at its start.  LDS_DIR: the
+	// whole dir[]; LDS_BITS: one bit per key of [kmin, kmax] (set iff dir
+	// holds a row), from `bits`, rewritten after every build by
+	// dir_bits_kernel, so that the lookups of keys without a row stay in LDS
+	int lds;
+	u32 lds_off;
+	const u32* bits;
 };
+
+enum JoinLds { LDS_NONE = 0, LDS_DIR = 1, LDS_BITS = 2 };
 
 struct KeyT {
 	Role r;
