@@ -765,6 +765,8 @@ int query_create(Query* q, bool single)
 			if ((rc = q->alloc((void**)&q->bits[j], join_lds_bytes(P.joins[j]))))
 				return rc;
 			P.joins[j].bits = q->bits[j];
+			if (L.builds[j].nparts)
+				L.builds[j].bits = q->bits[j];   // written with the slices
 		}
 	}
 
@@ -811,7 +813,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
 		}
 		for (int j = 0; j < P.njoins; j++)
-			if (P.joins[j].lds == LDS_BITS)
+			if (P.joins[j].lds == LDS_BITS && !L.builds[j].nparts)
 				HIP_TRY(launch_dir_bits(q->dir[j], P.joins[j].dir_w, (u64)(P.joins[j].kmax - P.joins[j].kmin) + 1,
 					q->bits[j], s));
 	}

@@ -41,6 +41,16 @@ __device__ __forceinline__ u32 dir_load(const void* dir, int w, u32 off)
 	return gload((const u32*)dir, off);
 }
 
+// Entry `off` of a dir[] slice held in LDS.
+__device__ __forceinline__ u32 dir_load_lds(const u32* slice, int w, u32 off)
+{
+	if (w == 1)
+		return ((const uint8_t*)slice)[off];
+	if (w == 2)
+		return ((const uint16_t*)slice)[off];
+	return slice[off];
+}
+
 // The 32-bit word holding key offset `off`, and the bit mask of its entry.
 __device__ __forceinline__ u32* dir_word(void* dir, int w, u32 off, u32& mask, u32& shift)
 {

@@ -155,20 +155,6 @@ static_assert(kMaxParts <= kWave * 64 && kMaxChunks <= kWave * 64, "block_exclus
 static_assert(kPartChunk == kBuildChunkRows && kMaxChunks == kMaxBuildChunks && kMaxParts == kMaxBuildParts,
 	"partitioned build limits shared with the host");
 
-// Predicates and key range of one build row; off = key - kmin.
-__device__ __forceinline__ bool build_sel(const BuildPlan& B, i64 row, u32& off)
-{
-	if (!preds_pass_row(B.preds, B.npreds, row))
-		return false;
-	const i64 key = load1(B.key, B.key_type, row);
-	if (key < B.kmin || key > B.kmax) {
-		gatomic_or(B.error, 4u);
-		return false;
-	}
-	off = (u32)(key - B.kmin);
-	return true;
-}
-
 // Linear-probing insert of a repeated key into the overflow slots.
 __device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64 val)
 {
@@ -201,30 +187,58 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	const i64 r0 = (i64)blockIdx.x * kPartChunk;
 	u32 off[kU], pay[kU], rank[kU];
 	bool ok[kU];
+	// every column of the kU rows is requested before any is tested (rows
+	// past the end read row 0), so the chunk costs one memory round trip
+	i64 key[kU], pv[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
 		const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
-		ok[u] = r < B.rows && build_sel(B, r, off[u]);
-		pay[u] = (ok[u] && B.npay) ? (u32)(load1(B.pay, B.pay_type, r) - B.pmin) : 0u;
+		const i64 rr = r < B.rows ? r : 0;
+		key[u] = load1(B.key, B.key_type, rr);
+		pv[u] = B.npay ? load1(B.pay, B.pay_type, rr) : B.pmin;
+		ok[u] = r < B.rows;
+	}
+#pragma unroll
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < B.npreds) {
+			const Pred& Q = B.preds[p];
+			i64 x[kU];
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
+				x[u] = load1(Q.r.ptr, Q.r.type, r < B.rows ? r : 0);
+			}
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				ok[u] = ok[u] && compare(Q.cmp, Q.r.type, x[u], Q.value);
+		}
+	}
+#pragma unroll
+	for (int u = 0; u < kU; u++) {
+		if (ok[u] && (key[u] < B.kmin || key[u] > B.kmax)) {
+			gatomic_or(B.error, 4u);
+			ok[u] = false;
+		}
+		off[u] = (u32)(key[u] - B.kmin);
+		pay[u] = (u32)(pv[u] - B.pmin);
 	}
 #pragma unroll
 	for (int u = 0; u < kU; u++)
 		rank[u] = ok[u] ? atomicAdd(&hist[off[u] >> B.pshift], 1u) : 0u;
 	__syncthreads();
-	u32* cnt = B.pcnt + (u64)blockIdx.x * P;
+	const u64 G = (u64)B.nchunks;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
-		cnt[p] = hist[p];
+		((VH_GLOBAL u32*)B.pcnt)[(u64)p * G + blockIdx.x] = hist[p];
 	__syncthreads();
 	const u32 n = block_exclusive_scan(hist, P, &total);
-	u32* start = B.poff + (u64)blockIdx.x * P;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
-		start[p] = hist[p];
+		((VH_GLOBAL u32*)B.poff)[(u64)p * G + blockIdx.x] = hist[p];
 #pragma unroll
 	for (int u = 0; u < kU; u++)
 		if (ok[u])
 			staged[hist[off[u] >> B.pshift] + rank[u]] = (u64)off[u] | ((u64)pay[u] << 32);
 	__syncthreads();
-	u64* dst = B.pbuf + r0;
+	VH_GLOBAL u64* dst = (VH_GLOBAL u64*)B.pbuf + r0;
 	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
 		dst[i] = staged[i];
 }
@@ -249,8 +263,8 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 	for (u32 i = threadIdx.x; i < words; i += blockDim.x)
 		slice[i] = 0xFFFFFFFFu;
 	for (int b = threadIdx.x; b < G; b += blockDim.x) {
-		pre[b] = B.pcnt[(u64)b * P + p];
-		base[b] = B.poff[(u64)b * P + p];
+		pre[b] = gload(B.pcnt, (i64)p * G + b);
+		base[b] = gload(B.poff, (i64)p * G + b);
 	}
 	__syncthreads();
 	const u32 total = block_exclusive_scan(pre, G, &total_rows);
@@ -274,7 +288,7 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 					else
 						hi = mid - 1;
 				}
-				v[u] = B.pbuf[base[lo] + i];
+				v[u] = gload(B.pbuf, (i64)base[lo] + i);
 			}
 		}
 #pragma unroll
@@ -294,9 +308,23 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 		}
 	}
 	__syncthreads();
-	u32* dst = (u32*)((char*)B.dir + k0 * (u64)w);
+	VH_GLOBAL u32* dst = (VH_GLOBAL u32*)((char*)B.dir + k0 * (u64)w);
 	for (u32 i = threadIdx.x; i < words; i += blockDim.x)
 		dst[i] = slice[i];
+	if (B.bits) {
+		// the slice's words of the key bitmap (k0 is a multiple of 2^10)
+		const u32 empty = dir_empty(w);
+		VH_GLOBAL u32* bdst = (VH_GLOBAL u32*)B.bits + (k0 >> 5);
+		for (u32 i = threadIdx.x; i < (u32)((nkeys + 31) / 32); i += blockDim.x) {
+			u32 word = 0;
+			for (u32 b = 0; b < 32; b++) {
+				const u64 k = (u64)i * 32 + b;
+				if (k < nkeys && dir_load_lds(slice, w, (u32)k) != empty)
+					word |= 1u << b;
+			}
+			bdst[i] = word;
+		}
+	}
 }
 
 __global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v, const u32* only_if)

@@ -181,11 +181,14 @@ struct BuildPlan {
 	// the keys with (key - kmin) >> pshift == p.  Chunk b (rows
 	// [b * 4096, (b + 1) * 4096)) leaves its selected rows sorted by
 	// partition in pbuf[b * 4096 ...] as (key - kmin) | payload << 32, the
-	// run of partition p being pcnt[b * nparts + p] rows from offset
-	// poff[b * nparts + p].
+	// run of partition p being pcnt[p * nchunks + b] rows from offset
+	// poff[p * nchunks + b] (partition-major, so a partition's runs are
+	// contiguous for the workgroup that builds it).
 	u64* pbuf;
 	u32* pcnt;
 	u32* poff;
+	// key bitmap of dir (JoinT::bits), written with each slice when set
+	u32* bits;
 	int nchunks;
 	int nparts;
 	int pshift;
