@@ -102,10 +102,16 @@ def test_source_reflects_the_plan():
     assert "q2[p] = __builtin_nontemporal_load" in eager
     assert eager.index("compare(") > eager.index("q2[p] = __builtin_nontemporal_load")
     assert "c2[u] = " in eager
-    # lazy: scalar loop only, columns no predicate reads after the predicate
+    # lazy: the vector loop loads each chunk's predicate column at the loop
+    # top (no pipelining) and the other columns after the predicate, for
+    # live rows only (buffer loads); the scalar tail loads them after the
+    # predicate too
     lazy = source(d, kernel=abi.VH_KERNEL_JIT_LAZY)
-    assert "__builtin_nontemporal_load" not in lazy
-    assert lazy.index("compare(") < lazy.index("c2[u] =")
+    assert "const i64 pb = w * (32 * U) + lane;" in lazy
+    assert "__builtin_amdgcn_raw_buffer_load_b128" in lazy   # int64 price, two rows a lane
+    assert lazy.index("compare(") < lazy.index("__builtin_amdgcn_raw_buffer_load")
+    tail = lazy[lazy.index("for (i64 base = nfull"):]
+    assert tail.index("compare(") < tail.index("c2[u] =")
 
 
 def test_filtered_dimension_keeps_a_key_bitmap_in_lds():

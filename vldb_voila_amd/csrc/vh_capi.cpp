@@ -225,8 +225,8 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 		return rc;
 	const int kernel = cfg ? cfg->kernel : VH_KERNEL_AUTO;
 	if (kernel != VH_KERNEL_GENERIC)
-		assign_lds_tables(L, desc);
-	const std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel));
+		choose_probe_shape(L, desc, cfg);
+	const std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel), L.block);
 	*len = src.size();
 	if (buf) {
 		if (cap <= src.size())

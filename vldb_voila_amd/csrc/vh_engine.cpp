@@ -440,6 +440,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		J.npay = jd.npayload;
 		J.variant = jd.variant;
 		J.last = j == d->njoins - 1;
+		J.filtered = B.npreds > 0;
 	}
 
 	// group keys
@@ -626,16 +627,29 @@ int Query::alloc(void** p, size_t bytes)
 	return VH_OK;
 }
 
+static int device_lds_per_block()
+{
+	static int bytes = 0;
+	if (!bytes) {
+		int dev = 0, n = 0;
+		if (hipGetDevice(&dev) == hipSuccess &&
+			hipDeviceGetAttribute(&n, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && n > 0)
+			bytes = n;
+		else
+			bytes = 64 * 1024;
+	}
+	return bytes;
+}
+
 // Join tables the generated probe kernel copies into LDS, in join order
 // (the first joins see the most rows): the whole dir[] while it fits what is
-// left of kLdsTableBudget, else its key bitmap when the build is filtered or
-// its keys are sparse, so that most lookups of keys without a row end in
-// LDS.  Direct tables only; the generic kernel ignores these fields.
-static constexpr size_t kLdsTableBudget = 64 * 1024;
-
-void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d)
+// left of the budget, else its key bitmap when the build is filtered or its
+// keys are sparse, so that most lookups of keys without a row end in LDS.
+// Direct tables only; the generic kernel ignores these fields.
+static void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d, size_t budget)
 {
 	Plan& P = L.plan;
+	const size_t kLdsTableBudget = budget;
 	size_t used = groupby_lds_bytes(P);
 	for (int j = 0; j < P.njoins; j++) {
 		JoinT& J = P.joins[j];
@@ -656,6 +670,29 @@ void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d)
 		}
 		J.lds_off = (u32)used;
 		used += join_lds_bytes(J);
+	}
+}
+
+// With join tables in LDS the generated kernel runs 1024-thread workgroups,
+// one per CU: sixteen waves share one copy of the tables (with 256 threads,
+// 60 KB of tables left two workgroups, two waves per SIMD, for a chain of
+// dependent lookups), and the tables may use the whole LDS of a workgroup
+// (160 KB on gfx950).  Lane-private and partitioned group-bys keep 256
+// threads: their LDS and row slabs are laid out for that.
+void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* cfg)
+{
+	Plan& P = L.plan;
+	const bool fixed = P.mode == GB_REG || P.mode == GB_PART;
+	const size_t budget = fixed ? 64 * 1024 : std::min<size_t>((size_t)device_lds_per_block(), 160 * 1024);
+	assign_lds_tables(L, d, budget);
+	L.block = 256;
+	bool tables = false;
+	for (int j = 0; j < P.njoins; j++)
+		tables = tables || P.joins[j].lds != LDS_NONE;
+	if (tables && !fixed) {
+		L.block = 1024;
+		if (!(cfg && cfg->grid > 0))
+			L.grid = std::max(1, std::min(L.grid, device_cus()));
 	}
 }
 
@@ -756,9 +793,9 @@ int query_create(Query* q, bool single)
 		return rc;
 	HIP_TRY(hipMemcpy(q->dclear, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
 
-	// join tables the generated probe kernel keeps in LDS
+	// join tables the generated probe kernel keeps in LDS, its workgroup size
 	if (single && q->cfg.kernel != VH_KERNEL_GENERIC) {
-		assign_lds_tables(L, &q->desc);
+		choose_probe_shape(L, &q->desc, &q->cfg);
 		for (int j = 0; j < P.njoins; j++) {
 			if (P.joins[j].lds != LDS_BITS)
 				continue;
@@ -784,7 +821,7 @@ int query_create(Query* q, bool single)
 	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_PIPE)
 		return fail(VH_ERR_INVALID, "kernel %d", kernel);
 	if (single && kernel != VH_KERNEL_GENERIC) {
-		q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel));
+		q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel), L.block);
 		std::string log;
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
 		if (rc && kernel != VH_KERNEL_AUTO)
@@ -819,7 +856,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 	}
 	if (phase == -1 || phase == 2) {
 		if (q->jit_fn)
-			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, s));
+			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s));
 		else
 			HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
 	}

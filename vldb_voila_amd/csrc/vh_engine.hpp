@@ -37,6 +37,7 @@ struct Lowered {
 	uint64_t groups = 0;                 // output slots (domain or hash capacity)
 	bool wide = false;
 	int grid = 0;
+	int block = 256;                     // generated probe kernel: threads per workgroup
 };
 
 struct Query {
@@ -71,8 +72,9 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
 // single: a query of its own (generated probe kernel per cfg.kernel,
 // partitioned builds); false for the pipelines of a batch
 int query_create(Query* q, bool single = true);
-// Join tables a generated probe kernel keeps in LDS (JoinT::lds, lds_off)
-void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d);
+// Shape of a generated probe kernel: the join tables it keeps in LDS
+// (JoinT::lds, lds_off) and its workgroup size (L.block, L.grid)
+void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* cfg);
 int query_run(Query* q, hipStream_t s, int phase);
 int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows);
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -66,6 +67,8 @@ struct Col {
 	int type;
 	std::string path;   // expression for the column's address in the kernel
 	bool pred;          // read by a predicate (loaded first)
+	int stage;          // first use: 0 predicates, j + 1 join j's key, njoins + 1 aggregation
+	bool late = false;  // vector loop: loaded for live rows only, before stage `stage`
 };
 
 struct Gen {
@@ -75,6 +78,8 @@ struct Gen {
 	                    // behind this chunk's last table lookup
 	int U;
 	bool vec = false;   // vector main loop (decided in source())
+	int late_stage = 0; // vector loop: stage before which late columns are loaded (0: none)
+	int block = 256;    // threads per workgroup
 	std::vector<Col> cols;
 	// plan fields the kernel reads, copied into locals before the row loop
 	// (the loop's atomics would otherwise make the compiler reload them)
@@ -105,21 +110,22 @@ struct Gen {
 		return name;
 	}
 
-	int col_index(const Role& r, const std::string& path, bool pred)
+	int col_index(const Role& r, const std::string& path, bool pred, int stage = 1 << 20)
 	{
 		for (size_t i = 0; i < cols.size(); i++)
 			if (cols[i].ptr == r.ptr && cols[i].type == r.type) {
 				cols[i].pred = cols[i].pred || pred;
+				cols[i].stage = std::min(cols[i].stage, stage);
 				return (int)i;
 			}
-		cols.push_back({r.ptr, r.type, path + ".ptr", pred});
+		cols.push_back({r.ptr, r.type, path + ".ptr", pred, stage});
 		return (int)cols.size() - 1;
 	}
 
-	void note_role(const Role& r, const std::string& path, bool pred)
+	void note_role(const Role& r, const std::string& path, bool pred, int stage)
 	{
 		if (r.kind == ROLE_COL)
-			col_index(r, path, pred);
+			col_index(r, path, pred, stage);
 	}
 
 	// Value of a role for row u as an i64 expression.  tail_pay names the
@@ -275,6 +281,8 @@ struct Gen {
 		line("\t\t{");
 		line("\t\t\tconst i64 pb = " + chunk + " * (32 * U) + lane;");
 		for (size_t i = 0; i < cols.size(); i++) {
+			if (cols[i].late)
+				continue;
 			const std::string ty = ctype_of(cols[i].type);
 			line("#pragma unroll");
 			snprintf(b, sizeof b, "\t\t\tfor (int p = 0; p < U / 2; p++) q%zu[p] = __builtin_nontemporal_load(&((%s)%s)[pb + p * 64]);",
@@ -282,6 +290,40 @@ struct Gen {
 			line(b);
 		}
 		line("\t\t}");
+	}
+
+	// Late columns of the vector loop: pair p of the lane is read only when
+	// one of its two rows is live; a dead pair's buffer offset is out of
+	// range, which returns zeros and reads no memory, so lines with no live
+	// row are never fetched.
+	void emit_late()
+	{
+		char b[512];
+		for (size_t i = 0; i < cols.size(); i++) {
+			if (!cols[i].late)
+				continue;
+			const int W = host_width(cols[i].type);
+			const char* ty = ctype_of(cols[i].type);
+			const char* fn = W == 1 ? "b16" : (W == 2 ? "b32" : (W == 4 ? "b64" : "b128"));
+			line("#pragma unroll");
+			line("\t\tfor (int p = 0; p < U / 2; p++) {");
+			snprintf(b, sizeof b, "\t\t\tconst u32 vo = ((live >> (2 * p)) & 3u) ? (cb + (u32)p * 128u + 2u * (u32)lane) * %du "
+				": 0x80000000u;", W);
+			line(b);
+			snprintf(b, sizeof b, "\t\t\tconst auto v = __builtin_amdgcn_raw_buffer_load_%s(rs%zu, vo, 0, 0);", fn, i);
+			line(b);
+			if (W == 1)
+				snprintf(b, sizeof b, "\t\t\tc%zu[2 * p] = (%s)(v & 0xFFu); c%zu[2 * p + 1] = (%s)(v >> 8);", i, ty, i, ty);
+			else if (W == 2)
+				snprintf(b, sizeof b, "\t\t\tc%zu[2 * p] = (%s)(v & 0xFFFFu); c%zu[2 * p + 1] = (%s)(v >> 16);", i, ty, i, ty);
+			else if (W == 4)
+				snprintf(b, sizeof b, "\t\t\tc%zu[2 * p] = (%s)v[0]; c%zu[2 * p + 1] = (%s)v[1];", i, ty, i, ty);
+			else
+				snprintf(b, sizeof b, "\t\t\tc%zu[2 * p] = (i64)(((u64)v[1] << 32) | v[0]); "
+					"c%zu[2 * p + 1] = (i64)(((u64)v[3] << 32) | v[2]);", i, i);
+			line(b);
+			line("\t\t}");
+		}
 	}
 
 	// Predicates, joins and the aggregation of the rows of one chunk whose
@@ -339,6 +381,8 @@ struct Gen {
 			for (size_t i = 0; i < cols.size(); i++)
 				if (!cols[i].pred)
 					load((int)i);
+		if (!scalar && late_stage == 1 && P.njoins == 0)
+			emit_late();
 		if (P.njoins == 0)
 			emit_next();
 
@@ -350,6 +394,8 @@ struct Gen {
 			const std::string kmin = uniform("i64", jp + ".kmin"), kmax = uniform("i64", jp + ".kmax");
 			const std::string mask = uniform("u64", jp + ".mask");
 			const std::string slots = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".slots");
+			if (!scalar && late_stage == j + 1)
+				emit_late();
 			snprintf(b, sizeof b, "\t\tu32 pay%d[U];", j);
 			line(b);
 			line("\t\t{");
@@ -360,6 +406,8 @@ struct Gen {
 			line("\t\t\t\tVH_KILL(u, key < " + kmin + " || key > " + kmax + ");");
 			line("\t\t\t\toff[u] = VH_LIVE(u) ? (u32)(key - " + kmin + ") : 0u;");
 			line("\t\t\t}");
+			if (!scalar && j == last && late_stage == P.njoins + 1)
+				emit_late();   // aggregation columns, before this join's lookups (and the next chunk's loads)
 			if (J.dir_w) {
 				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
 				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
@@ -483,29 +531,29 @@ struct Gen {
 		// group keys and sum arguments
 		for (int p = 0; p < P.npreds; p++) {
 			snprintf(b, sizeof b, "P.preds[%d].r", p);
-			note_role(P.preds[p].r, b, true);
+			note_role(P.preds[p].r, b, true, 0);
 		}
 		for (int j = 0; j < P.njoins; j++) {
 			snprintf(b, sizeof b, "P.joins[%d].key", j);
-			note_role(P.joins[j].key, b, false);
+			note_role(P.joins[j].key, b, false, j + 1);
 		}
 		for (int k = 0; k < P.nkeys; k++) {
 			snprintf(b, sizeof b, "P.keys[%d].r", k);
-			note_role(P.keys[k].r, b, false);
+			note_role(P.keys[k].r, b, false, P.njoins + 1);
 		}
 		for (int si = 0; si < P.nsums; si++)
 			for (int t = 0; t < P.sums[si].nterms; t++) {
 				const Term& T = P.sums[si].t[t];
 				snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
 				if (T.sa)
-					note_role(T.a, std::string(b) + ".a", false);
+					note_role(T.a, std::string(b) + ".a", false, P.njoins + 1);
 				if (T.sb)
-					note_role(T.b, std::string(b) + ".b", false);
+					note_role(T.b, std::string(b) + ".b", false, P.njoins + 1);
 			}
 		// The vector main loop needs every probe column aligned to a pair of
 		// its values, an even U and a grid-strided row mapping (not GB_PART's
 		// slab per workgroup); otherwise every row takes the scalar path.
-		vec = eager && U % 2 == 0 && mode != GB_PART;
+		vec = U % 2 == 0 && mode != GB_PART;
 		int row_bytes = 0;
 		for (const Col& c : cols) {
 			vec = vec && ((uintptr_t)c.ptr % (uintptr_t)(2 * host_width(c.type))) == 0;
@@ -513,8 +561,35 @@ struct Gen {
 		}
 		if (U == 0) {
 			// measured on MI355X: 8 rows per lane for 16 (c1 probe) and 32
-			// bytes (star fact) of columns per row, 2 for 36 (filter)
-			U = vec ? (row_bytes <= 32 ? 8 : 2) : 4;
+			// bytes (star fact) of columns per row, 2 for 36 (filter); 4 in
+			// 1024-thread workgroups (star: 344 us against 351 with 8)
+			U = vec ? (block > 256 ? 4 : (row_bytes <= 32 ? 8 : 2)) : 4;
+		}
+		// Late materialization in the vector loop: columns first used at or
+		// after late_stage are loaded there, for live rows only.  Asked for
+		// (not eager): after the last join whose build is filtered, else
+		// after the predicates.  By default, when two joins have filtered
+		// builds (a dimension filter keeping 1 in 5 twice leaves 4% of the
+		// rows, and about half of the 8-byte lines hold no live row): the
+		// aggregation's columns only, requested before the last join's
+		// lookups, since a late join key costs an exposed memory round trip
+		// (measured: star probe 465 us with late date/part keys, 458 us
+		// without late columns).
+		if (vec) {
+			int last_filtered = -1, nfiltered = 0;
+			for (int j = 0; j < P.njoins; j++)
+				if (P.joins[j].filtered) {
+					last_filtered = j;
+					nfiltered++;
+				}
+			if (!eager)
+				late_stage = last_filtered >= 0 ? last_filtered + 2 : (P.npreds > 0 ? 1 : 0);
+			else if (nfiltered >= 2)
+				late_stage = P.njoins + 1;   // aggregation columns only: their loads
+				                             // overlap the last join's lookups
+			for (Col& c : cols)
+				c.late = late_stage > 0 && c.stage >= late_stage &&
+					(double)P.rows * host_width(c.type) < 2147483000.0;   // buffer offsets are 32-bit
 		}
 
 		if (mode == GB_PART) {
@@ -537,14 +612,25 @@ struct Gen {
 			line("\tconst i64 nfull = rows / (64 * U);");
 			line("\tif (wv < nfull) {");
 			for (size_t i = 0; i < cols.size(); i++) {
+				if (cols[i].late)
+					continue;
 				snprintf(b, sizeof b, "\t\tvh2_%s q%zu[U / 2];", ctype_of(cols[i].type), i);
 				line(b);
 			}
-			load_pairs("wv");
+			// pipelined: the first chunk here, each next one behind the last
+			// join's lookups (body); otherwise every chunk at the loop top
+			if (pipe)
+				load_pairs("wv");
 			line("\tfor (i64 w = wv; w < nfull; w += nwv) {");
+			if (!pipe)
+				load_pairs("w");
+			line("\t\tconst u32 cb = (u32)(w * (64 * U));   // first row of the chunk");
+			line("\t\t(void)cb;");
 			for (size_t i = 0; i < cols.size(); i++) {
 				snprintf(b, sizeof b, "\t\t%s c%zu[U];", ctype_of(cols[i].type), i);
 				line(b);
+				if (cols[i].late)
+					continue;
 				line("#pragma unroll");
 				snprintf(b, sizeof b, "\t\tfor (int p = 0; p < U / 2; p++) { c%zu[2 * p] = q%zu[p].x; c%zu[2 * p + 1] = "
 					"q%zu[p].y; }", i, i, i, i);
@@ -570,7 +656,8 @@ struct Gen {
 		k += "#define VH_LIVE(u) ((live >> (u)) & 1u)\n";
 		k += "#define VH_KILL(u, c) (live &= ~((u32)(c) << (u)))\n";
 		k += "static constexpr int U = " + std::to_string(U) + ";\n";
-		k += "extern \"C\" __global__ __launch_bounds__(256) void vh_generated(const Plan* __restrict__ Pp)\n{\n";
+		k += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(block) +
+			") void vh_generated(const Plan* __restrict__ Pp)\n{\n";
 		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
 		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
@@ -604,6 +691,11 @@ struct Gen {
 				k += "\tif (gload(P.joins[" + std::to_string(j) + "].dups, 0)) dups_mask |= " + std::to_string(1u << j) + "u;\n";
 		k += "\t(void)dups_mask;\n";
 		k += decls;
+		for (size_t i = 0; i < cols.size(); i++)
+			if (cols[i].late)
+				k += "\tconst __amdgpu_buffer_rsrc_t rs" + std::to_string(i) +
+					" = __builtin_amdgcn_make_buffer_rsrc((void*)" + cols[i].path + ", (short)0, (int)(P.rows * " +
+					std::to_string(host_width(cols[i].type)) + "), 0x00020000);\n";
 		k += "\t__syncthreads();\n\tconst i64 rows = P.rows;\n\tconst i64 tile = (i64)U * blockDim.x;\n";
 		k += loop;
 		k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
@@ -632,9 +724,10 @@ std::string device_arch()
 
 } // namespace
 
-std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline)
+std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block)
 {
 	Gen g(P, eager, pipeline, unroll);
+	g.block = block;
 	return g.source(P.mode);
 }
 
@@ -683,11 +776,11 @@ int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log)
 	return VH_OK;
 }
 
-hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, hipStream_t s)
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s)
 {
 	const Plan* arg = dP;
 	void* args[] = {(void*)&arg};
-	return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, 256, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
+	return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
 		nullptr);
 }
 

@@ -27,7 +27,8 @@ inline int jit_unroll(const vh_config* cfg)
 // loop, columns no predicate reads are loaded for qualifying rows only.
 // pipeline: the vector loop requests the next chunk's columns right behind
 // the last join's table lookups.
-std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline);
+// block: threads per workgroup (the kernel's launch bound).
+std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block = 256);
 
 // Generated-kernel options of a configuration's vh_kernel value.
 inline bool jit_eager(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
@@ -36,6 +37,6 @@ inline bool jit_pipeline(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log);
 // Launch a compiled kernel over P (dP: its device copy).  GB_REG / GB_LDS
 // leave per-block partials for launch_merge.
-hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, hipStream_t s);
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s);
 
 } // namespace vh

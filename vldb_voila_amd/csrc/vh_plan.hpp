@@ -73,7 +73,7 @@ struct JoinT {
 	int npay;
 	int variant;         // enum vh_join_variant
 	int last;            // last join of the pipeline: JoinN may match many
-	int pad;
+	int filtered;        // the build side has predicates (probe keys often miss)
 	const u32* dups;     // set by the build when a key occurs twice
 	// generated probe kernels: a copy of the table in LDS at byte lds_off of
 	// the dynamic LDS, made by every workgroup at its start.  LDS_DIR: the
