@@ -186,11 +186,15 @@ enum vh_join_layout {
 enum vh_kernel {
 	VH_KERNEL_AUTO = 0,      /* generated, or generic if hipRTC fails       */
 	VH_KERNEL_GENERIC = 1,
-	VH_KERNEL_JIT = 2,       /* generated; vh_query_new fails without it    */
-	VH_KERNEL_JIT_LAZY = 3,  /* generated, columns after the predicates are
-	                            loaded only for qualifying rows             */
-	VH_KERNEL_JIT_PIPE = 4   /* generated, the next chunk's columns are
-	                            loaded while this chunk's joins resolve     */
+	VH_KERNEL_JIT = 2,       /* generated; vh_query_new fails without it.
+	                            The next chunk's columns are requested while
+	                            this chunk's joins resolve; columns first used
+	                            after two filtered joins are read for live
+	                            rows only                                   */
+	VH_KERNEL_JIT_LAZY = 3,  /* generated, no prefetch; columns first used
+	                            after the last filtered join (else after the
+	                            predicates) are read for live rows only      */
+	VH_KERNEL_JIT_PIPE = 4   /* same kernel as VH_KERNEL_JIT                */
 };
 
 typedef struct vh_config {
@@ -202,7 +206,9 @@ typedef struct vh_config {
 	int32_t kernel;        /* enum vh_kernel (single queries; batches run the
 	                          generic kernel)                                  */
 	int32_t unroll;        /* rows per lane per step of the generated kernel,
-	                          0 = 4; 1..16                                     */
+	                          1..16; 0 = by row width (8 for <= 32 bytes of
+	                          probe columns, else 2; 4 in 1024-thread
+	                          workgroups)                                      */
 	int32_t merge;         /* dense LDS group-by, how workgroups combine: 0 =
 	                          auto, 1 = partials + a merge launch, 2 = each
 	                          workgroup adds into the result with atomics     */
