@@ -360,6 +360,13 @@ int vh_batch_run_graph(vh_batch* b, void* stream);
 int vh_batch_result(vh_batch* b, int32_t i, void* stream, char* buf, size_t cap, size_t* len,
 	int64_t* nrows);
 int vh_batch_info(const vh_batch* b, int32_t* items, int32_t* grid, int64_t* lds_bytes);
+/* Source of the generated kernel a batch runs for this pipeline's probe
+ * items (vh_generated_batch: work items from a queue, each item's LDS
+ * group-by state flushed to the pipeline's result); no device needed, buffer
+ * convention of vh_generate_source.  Pipelines whose sources are equal share
+ * one kernel in a batch. */
+int vh_generate_batch_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap,
+	size_t* len);
 int vh_batch_free(vh_batch* b);
 
 /* ------------------------------------------------------------------------ */

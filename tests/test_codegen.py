@@ -107,10 +107,10 @@ def test_source_reflects_the_plan():
     # live rows only (buffer loads); the scalar tail loads them after the
     # predicate too
     lazy = source(d, kernel=abi.VH_KERNEL_JIT_LAZY)
-    assert "const i64 pb = w * (32 * U) + lane;" in lazy
+    assert "const i64 pb = (rb >> 1) + w * (32 * U) + lane;" in lazy
     assert "__builtin_amdgcn_raw_buffer_load_b128" in lazy   # int64 price, two rows a lane
     assert lazy.index("compare(") < lazy.index("__builtin_amdgcn_raw_buffer_load")
-    tail = lazy[lazy.index("for (i64 base = nfull"):]
+    tail = lazy[lazy.index("for (i64 base = rb + nfull"):]
     assert tail.index("compare(") < tail.index("c2[u] =")
 
 
@@ -129,3 +129,31 @@ def test_filtered_dimension_keeps_a_key_bitmap_in_lds():
     src = source(pl.desc())
     assert "tb0[off[u] >> 5]" in src and "P.joins[0].bits" in src
     assert "td0" not in src
+
+
+def batch_source(desc, **cfg):
+    L = runtime.lib()
+    L.vh_generate_batch_source.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    c = runtime.make_config(**cfg)
+    n = C.c_size_t()
+    runtime.check(L.vh_generate_batch_source(C.byref(desc), C.byref(c), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    runtime.check(L.vh_generate_batch_source(C.byref(desc), C.byref(c), buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("mode", [abi.VH_GROUP_AUTO, abi.VH_GROUP_REG, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL,
+                                  abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH, abi.VH_GROUP_PART])
+def test_batch_entry_point_compiles(mode):
+    """The batched launch's generated kernel: work items from a queue, the
+    item's rows through the same vector loop, its LDS state flushed."""
+    prm = datagen.batch_params(1, seed=3)[0]
+    prm["groups"] = 100
+    b, p = datagen.gen_batch_pipeline(prm, seed=3, index=0)
+    src = batch_source(workloads.batch_pipeline(prm, b, p).desc(), group_mode=mode)
+    assert "void vh_generated_batch(" in src and "gatomic_add(counter, 1u)" in src
+    assert "const i64 rb = I.r0, re = I.r1;" in src or "const i64 first = I.r0;" in src
+    if mode in (abi.VH_GROUP_LDS, abi.VH_GROUP_REG):
+        assert "lds_dense_flush(P, L);" in src
+    rc, log, size = compile_for_gfx950(src)
+    assert rc == 0 and size > 0, log[:2000]

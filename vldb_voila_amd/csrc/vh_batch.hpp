@@ -7,15 +7,6 @@
 
 namespace vh {
 
-struct BatchItem {
-	int plan;            // pipeline index
-	int phase;           // 0 = build rows of builds[build], 1 = probe rows of plans[plan]
-	int build;           // index into the build plan array (phase 0)
-	int reserved;
-	i64 r0;
-	i64 r1;
-};
-
 // p[0, n) = v; with only_if, only when *only_if != 0 when the launch reads it
 struct FillSeg {
 	u64* p;

@@ -1,17 +1,23 @@
 // Host side of the batched launch: lowers every pipeline like a single
 // query, lays out the work queue (all build items, then all probe items) and
-// enqueues one multi-buffer clear plus two persistent launches (builds, then
-// probes), or replays the three from a captured hipGraph.
+// enqueues one multi-buffer clear plus persistent launches (builds, then the
+// probe items of each generated kernel or generic group-by strategy, then
+// partition aggregations), or replays them from a captured hipGraph.  Probe
+// items run generated kernels (vh_generated_batch, one per distinct source:
+// pipelines that differ only in data share one) unless a pipeline's config
+// asks for the generic kernel or hipRTC is unavailable.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/voila_hip.h"
 #include "vh_batch.hpp"
 #include "vh_engine.hpp"
+#include "vh_jit.hpp"
 #include "vh_kernels.hpp"
 
 using namespace vh;
@@ -24,11 +30,16 @@ using namespace vh;
 	} while (0)
 
 struct BatchLaunch {
-	int kind;            // kBatchBuild, kBatchAgg or a GroupMode (probe items)
+	int kind;            // kBatchBuild, kBatchAgg, kBatchGenerated or a GroupMode (probe items)
 	u32 first;
 	u32 count;
 	size_t lds;
+	hipFunction_t fn;    // kBatchGenerated: the compiled vh_generated_batch
 };
+
+static constexpr int kBatchGenerated = -3;
+// probe items' grouping key (BatchItem::build) of generated kernel g
+static constexpr int kGenKey = 1000;
 
 struct vh_batch {
 	std::vector<Query*> q;
@@ -79,6 +90,30 @@ static int tile_rows_of(const vh_config& c)
 	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
 }
 
+// Lowering of one pipeline of a batch: lane-private group-by state cannot
+// outlive a work item (LDS instead), and a partitioned group-by's regions
+// are the work items.
+static int lower_for_batch(int i, const vh_pipeline_desc* d, const vh_config& cfg, Lowered& L)
+{
+	vh_config lc = cfg;
+	if (lc.group_mode == VH_GROUP_REG)
+		lc.group_mode = VH_GROUP_LDS;
+	int rc = lower(d, &lc, L);
+	if (rc)
+		return fail(rc, "pipeline %d: %s", i, last_error());
+	Plan& P = L.plan;
+	if (P.mode == GB_REG)
+		P.mode = GB_LDS;
+	const int T = tile_rows_of(cfg);
+	if (P.mode == GB_PART) {
+		P.gregion_rows = T;
+		if ((P.rows + T - 1) / T > kMaxGroupRegions)
+			return fail(VH_ERR_UNSUPPORTED, "pipeline %d: more than %d items for a partitioned group-by", i,
+				kMaxGroupRegions);
+	}
+	return VH_OK;
+}
+
 // clear, then every build item, then the probe items of each group-by
 // strategy (a launch each), then the partition aggregation items of
 // partitioned group-bys
@@ -87,8 +122,19 @@ static hipError_t enqueue(vh_batch* b, hipStream_t s)
 	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
 	for (size_t k = 0; k < b->launches.size() && e == hipSuccess; k++) {
 		const BatchLaunch& l = b->launches[k];
-		e = launch_batch(l.kind, b->d_plans, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k, b->grid, l.lds,
-			s);
+		if (l.kind == kBatchGenerated) {
+			if (!l.count)
+				continue;
+			const Plan* plans = b->d_plans;
+			const BatchItem* items = b->d_items;
+			u32 first = l.first, count = l.count;
+			u32* counter = b->d_ctl + k;
+			void* args[] = {(void*)&plans, (void*)&items, (void*)&first, (void*)&count, (void*)&counter};
+			e = hipModuleLaunchKernel(l.fn, (unsigned)b->grid, 1, 1, 256, 1, 1, (unsigned)l.lds, s, args, nullptr);
+		} else {
+			e = launch_batch(l.kind, b->d_plans, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k, b->grid,
+				l.lds, s);
+		}
 	}
 	return e;
 }
@@ -107,6 +153,9 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	std::vector<BuildPlan> builds;
 	std::vector<BatchItem> build_items, probe_items, agg_items;
 	std::vector<FillSeg> segs;
+	std::vector<std::string> gen_src;        // generated batch kernels, by index
+	std::vector<hipFunction_t> gen_fn;
+	std::vector<size_t> gen_lds;
 	int rc = VH_OK;
 	for (int i = 0; i < n && !rc; i++) {
 		Query* q = new (std::nothrow) Query();
@@ -119,26 +168,10 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 			memset(&q->cfg, 0, sizeof(q->cfg));
 			q->cfg.group_mode = VH_GROUP_AUTO;
 		}
-		vh_config lc = q->cfg;
-		if (lc.group_mode == VH_GROUP_REG)
-			lc.group_mode = VH_GROUP_LDS;      // register state cannot outlive an item
-		if ((rc = lower(&q->desc, &lc, q->L))) {
-			fail(rc, "pipeline %d: %s", i, last_error());
+		if ((rc = lower_for_batch(i, &q->desc, q->cfg, q->L)))
 			break;
-		}
 		Plan& P = q->L.plan;
-		if (P.mode == GB_REG)
-			P.mode = GB_LDS;            // register state cannot outlive an item
 		const int T = tile_rows_of(q->cfg);
-		if (P.mode == GB_PART) {
-			// an item is a region: the [region][partition] tables have a row per tile
-			P.gregion_rows = T;
-			if ((P.rows + T - 1) / T > kMaxGroupRegions) {
-				rc = fail(VH_ERR_UNSUPPORTED, "pipeline %d: more than %d items for a partitioned group-by", i,
-					kMaxGroupRegions);
-				break;
-			}
-		}
 		if ((rc = query_create(q, false))) {
 			fail(rc, "pipeline %d: %s", i, last_error());
 			break;
@@ -153,8 +186,35 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 			if (q->dir[j])
 				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
 		}
+		// the probe kernel: generated for this pipeline's plan (shared with
+		// every pipeline whose source is the same), or generic per strategy
+		int key = P.mode;
+		if (q->cfg.kernel != VH_KERNEL_GENERIC) {
+			const std::string src = jit_source(P, true, jit_unroll(&q->cfg), true, 256, true);
+			size_t g = 0;
+			while (g < gen_src.size() && gen_src[g] != src)
+				g++;
+			if (g == gen_src.size()) {
+				hipFunction_t fn = nullptr;
+				std::string log;
+				const int jrc = jit_compile(src, &fn, log, "vh_generated_batch");
+				if (jrc && q->cfg.kernel != VH_KERNEL_AUTO) {
+					rc = fail(jrc, "pipeline %d: %s", i, last_error());
+					break;
+				}
+				if (!jrc) {
+					gen_src.push_back(src);
+					gen_fn.push_back(fn);
+					gen_lds.push_back(0);
+				}
+			}
+			if (g < gen_src.size()) {
+				key = kGenKey + (int)g;
+				gen_lds[g] = std::max(gen_lds[g], pipeline_lds_bytes(P));
+			}
+		}
 		for (i64 r0 = 0; r0 < P.rows; r0 += T)
-			probe_items.push_back({i, 1, P.mode, 0, r0, std::min<i64>(r0 + T, P.rows)});
+			probe_items.push_back({i, 1, key, 0, r0, std::min<i64>(r0 + T, P.rows)});
 		if (P.mode == GB_PART) {
 			for (int p = 0; p < P.gparts; p++)
 				agg_items.push_back({i, 2, -1, 0, p, p + 1});
@@ -180,23 +240,27 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	std::stable_sort(probe_items.begin(), probe_items.end(),
 		[](const BatchItem& x, const BatchItem& y) { return x.build < y.build; });
 	std::vector<BatchItem> items = build_items;
-	b->launches.push_back({kBatchBuild, 0, (u32)build_items.size(), 0});
+	b->launches.push_back({kBatchBuild, 0, (u32)build_items.size(), 0, nullptr});
 	for (size_t k = 0; k < probe_items.size();) {
 		size_t e = k;
 		while (e < probe_items.size() && probe_items[e].build == probe_items[k].build)
 			e++;
-		const int mode = probe_items[k].build;
-		b->launches.push_back({mode, (u32)(build_items.size() + k), (u32)(e - k), b->mode_lds[mode]});
+		const int key = probe_items[k].build;
+		if (key >= kGenKey)
+			b->launches.push_back({kBatchGenerated, (u32)(build_items.size() + k), (u32)(e - k),
+				gen_lds[key - kGenKey], gen_fn[key - kGenKey]});
+		else
+			b->launches.push_back({key, (u32)(build_items.size() + k), (u32)(e - k), b->mode_lds[key], nullptr});
 		k = e;
 	}
 	items.insert(items.end(), probe_items.begin(), probe_items.end());
-	b->launches.push_back({kBatchAgg, (u32)items.size(), (u32)agg_items.size(), b->agg_lds});
+	b->launches.push_back({kBatchAgg, (u32)items.size(), (u32)agg_items.size(), b->agg_lds, nullptr});
 	items.insert(items.end(), agg_items.begin(), agg_items.end());
 	for (const BatchLaunch& l : b->launches)
 		if (l.kind != kBatchBuild)
 			b->lds = std::max(b->lds, l.lds);
 	b->nitems = (u32)items.size();
-	const size_t ctl_words = 4;   // queue counters (u32), one per launch (at most 7)
+	const size_t ctl_words = (b->launches.size() + 1) / 2 + 1;   // queue counters (u32), one per launch
 	if ((rc = b->upload((void**)&b->d_ctl, nullptr, ctl_words * 8)) ||
 		(rc = b->upload((void**)&b->d_plans, plans.data(), plans.size() * sizeof(Plan))) ||
 		(rc = b->upload((void**)&b->d_builds, builds.data(), builds.size() * sizeof(BuildPlan))) ||
@@ -215,6 +279,33 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	b->grid = grid > 0 ? grid : cus * 4;
 	*out = b;
+	return VH_OK;
+}
+
+int vh_generate_batch_source(const vh_pipeline_desc* desc, const vh_config* cfg, char* buf, size_t cap,
+	size_t* len)
+{
+	if (!desc || !len)
+		return fail(VH_ERR_INVALID, "vh_generate_batch_source arguments");
+	vh_config c;
+	if (cfg) {
+		c = *cfg;
+	} else {
+		memset(&c, 0, sizeof(c));
+		c.group_mode = VH_GROUP_AUTO;
+	}
+	Lowered L;
+	int rc = lower_for_batch(0, desc, c, L);
+	if (rc)
+		return rc;
+	const std::string src = jit_source(L.plan, true, jit_unroll(&c), true, 256, true);
+	*len = src.size();
+	if (buf) {
+		if (cap <= src.size())
+			return fail(VH_ERR_CAPACITY, "source needs %zu bytes", src.size() + 1);
+		memcpy(buf, src.data(), src.size());
+		buf[src.size()] = 0;
+	}
 	return VH_OK;
 }
 

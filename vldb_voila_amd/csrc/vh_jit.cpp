@@ -80,6 +80,7 @@ struct Gen {
 	bool vec = false;   // vector main loop (decided in source())
 	int late_stage = 0; // vector loop: stage before which late columns are loaded (0: none)
 	int block = 256;    // threads per workgroup
+	bool batch = false; // entry point of a batched launch (vh_generated_batch)
 	std::vector<Col> cols;
 	// plan fields the kernel reads, copied into locals before the row loop
 	// (the loop's atomics would otherwise make the compiler reload them)
@@ -279,7 +280,7 @@ struct Gen {
 	{
 		char b[512];
 		line("\t\t{");
-		line("\t\t\tconst i64 pb = " + chunk + " * (32 * U) + lane;");
+		line("\t\t\tconst i64 pb = (rb >> 1) + " + chunk + " * (32 * U) + lane;");
 		for (size_t i = 0; i < cols.size(); i++) {
 			if (cols[i].late)
 				continue;
@@ -593,23 +594,42 @@ struct Gen {
 		}
 
 		if (mode == GB_PART) {
-			// rows of this workgroup: one contiguous slab, the region its tuples are kept for
-			line("\tconst i64 first = (i64)blockIdx.x * P.gregion_rows;");
-			line("\tconst i64 end = first + P.gregion_rows < rows ? first + P.gregion_rows : rows;");
+			// rows of this workgroup: one contiguous slab (a batch item), the
+			// region its tuples are kept for
+			if (batch) {
+				line("\tconst i64 first = I.r0;");
+				line("\tconst i64 end = I.r1;");
+				line("\tL.region = (u32)(first / P.gregion_rows);");
+			} else {
+				line("\tconst i64 first = (i64)blockIdx.x * P.gregion_rows;");
+				line("\tconst i64 end = first + P.gregion_rows < rows ? first + P.gregion_rows : rows;");
+				line("\tL.region = blockIdx.x;");
+			}
 			line("\tL.rbase = first;");
 			line("\tL.rcap = end > first ? (u32)(end - first) : 0u;");
-			line("\tL.region = blockIdx.x;");
 			scalar_loop(mode, "first", "end", "tile");
 		} else if (!vec) {
-			scalar_loop(mode, "(i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
+			if (batch)
+				scalar_loop(mode, "I.r0", "I.r1", "tile");
+			else
+				scalar_loop(mode, "(i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
 		} else {
-			// full chunks of 64U rows, one per wave at a time, then the
-			// remaining rows (fewer than 64U) on the scalar path
+			// full chunks of 64U rows of [rb, re), one per wave at a time, then
+			// the remaining rows (fewer than 64U) on the scalar path.  A batch
+			// item's rows go to the waves of this workgroup (rb is even: items
+			// start at multiples of 1024 rows)
 			line("\tconst int lane = threadIdx.x & 63;");
-			line("\tconst i64 wv = (i64)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + "
-				"(threadIdx.x >> 6)));");
-			line("\tconst i64 nwv = (i64)gridDim.x * (blockDim.x >> 6);");
-			line("\tconst i64 nfull = rows / (64 * U);");
+			if (batch) {
+				line("\tconst i64 rb = I.r0, re = I.r1;");
+				line("\tconst i64 wv = (i64)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));");
+				line("\tconst i64 nwv = (i64)(blockDim.x >> 6);");
+			} else {
+				line("\tconst i64 rb = 0, re = rows;");
+				line("\tconst i64 wv = (i64)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + "
+					"(threadIdx.x >> 6)));");
+				line("\tconst i64 nwv = (i64)gridDim.x * (blockDim.x >> 6);");
+			}
+			line("\tconst i64 nfull = (re - rb) / (64 * U);");
 			line("\tif (wv < nfull) {");
 			for (size_t i = 0; i < cols.size(); i++) {
 				if (cols[i].late)
@@ -624,7 +644,7 @@ struct Gen {
 			line("\tfor (i64 w = wv; w < nfull; w += nwv) {");
 			if (!pipe)
 				load_pairs("w");
-			line("\t\tconst u32 cb = (u32)(w * (64 * U));   // first row of the chunk");
+			line("\t\tconst u32 cb = (u32)(rb + w * (64 * U));   // first row of the chunk");
 			line("\t\t(void)cb;");
 			for (size_t i = 0; i < cols.size(); i++) {
 				snprintf(b, sizeof b, "\t\t%s c%zu[U];", ctype_of(cols[i].type), i);
@@ -641,7 +661,10 @@ struct Gen {
 			body(mode, false);
 			line("\t}");
 			line("\t}");
-			scalar_loop(mode, "nfull * (64 * U) + (i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
+			if (batch)
+				scalar_loop(mode, "rb + nfull * (64 * U)", "re", "tile");
+			else
+				scalar_loop(mode, "rb + nfull * (64 * U) + (i64)blockIdx.x * tile", "re", "(i64)gridDim.x * tile");
 		}
 		const std::string loop = out;
 
@@ -656,9 +679,22 @@ struct Gen {
 		k += "#define VH_LIVE(u) ((live >> (u)) & 1u)\n";
 		k += "#define VH_KILL(u, c) (live &= ~((u32)(c) << (u)))\n";
 		k += "static constexpr int U = " + std::to_string(U) + ";\n";
-		k += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(block) +
-			") void vh_generated(const Plan* __restrict__ Pp)\n{\n";
-		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
+		if (batch) {
+			// persistent: work items from the launch's queue until it is empty
+			k += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(block) +
+				") void vh_generated_batch(const Plan* __restrict__ plans, const BatchItem* __restrict__ items, "
+				"u32 first_item, u32 count, u32* __restrict__ counter)\n{\n";
+			k += "\textern __shared__ u64 lds[];\n\t__shared__ u32 slot;\n";
+			k += "\tfor (;;) {\n";
+			k += "\tif (threadIdx.x == 0)\n\t\tslot = gatomic_add(counter, 1u);\n\t__syncthreads();\n";
+			k += "\tconst u32 item = slot;\n\t__syncthreads();\n\tif (item >= count)\n\t\tbreak;\n";
+			k += "\tconst BatchItem I = items[first_item + item];\n";
+			k += "\tconst Plan& P = plans[I.plan];\n";
+		} else {
+			k += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(block) +
+				") void vh_generated(const Plan* __restrict__ Pp)\n{\n";
+			k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
+		}
 		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
 		// join tables kept in LDS: copied by every workgroup (16-byte units;
@@ -698,7 +734,20 @@ struct Gen {
 					std::to_string(host_width(cols[i].type)) + "), 0x00020000);\n";
 		k += "\t__syncthreads();\n\tconst i64 rows = P.rows;\n\tconst i64 tile = (i64)U * blockDim.x;\n";
 		k += loop;
-		k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
+		if (batch) {
+			// the item's LDS group-by state goes to the pipeline's global
+			// arrays before the next item reuses the LDS
+			k += "\t__syncthreads();\n";
+			if (mode == GB_LDS)
+				k += "\tlds_dense_flush(P, L);\n";
+			else if (mode == GB_LDSHASH)
+				k += "\tldshash_flush(P, L);\n";
+			else if (mode == GB_PART)
+				k += "\tpart_finish(P, L);\n";
+			k += "\t__syncthreads();\n\t}\n}\n";
+		} else {
+			k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
+		}
 		return k;
 	}
 };
@@ -724,14 +773,15 @@ std::string device_arch()
 
 } // namespace
 
-std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block)
+std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block, bool batch)
 {
 	Gen g(P, eager, pipeline, unroll);
 	g.block = block;
+	g.batch = batch;
 	return g.source(P.mode);
 }
 
-int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log)
+int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, const char* name)
 {
 	const std::string arch = device_arch();
 	if (arch.empty())
@@ -768,7 +818,7 @@ int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log)
 	Compiled c;
 	hipError_t e = hipModuleLoadData(&c.mod, code.data());
 	if (e == hipSuccess)
-		e = hipModuleGetFunction(&c.fn, c.mod, "vh_generated");
+		e = hipModuleGetFunction(&c.fn, c.mod, name);
 	if (e != hipSuccess)
 		return fail(VH_ERR_HIP, "loading the generated kernel: %s", hipGetErrorString(e));
 	g_cache[key] = c;

@@ -28,13 +28,18 @@ inline int jit_unroll(const vh_config* cfg)
 // pipeline: the vector loop requests the next chunk's columns right behind
 // the last join's table lookups.
 // block: threads per workgroup (the kernel's launch bound).
-std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block = 256);
+// batch: the entry point of a batched launch, vh_generated_batch(plans,
+// items, first, count, counter), which takes work items (rows [r0, r1) of
+// plans[plan]) from the queue until it is empty and adds each item's LDS
+// group-by state to the pipeline's global arrays.
+std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int block = 256, bool batch = false);
 
 // Generated-kernel options of a configuration's vh_kernel value.
 inline bool jit_eager(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
 inline bool jit_pipeline(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
 // Compile (or fetch from the cache) `src` for the current device.
-int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log);
+// `name`: the kernel's symbol (vh_generated or vh_generated_batch).
+int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, const char* name = "vh_generated");
 // Launch a compiled kernel over P (dP: its device copy).  GB_REG / GB_LDS
 // leave per-block partials for launch_merge.
 hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s);

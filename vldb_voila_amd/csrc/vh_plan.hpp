@@ -158,6 +158,17 @@ struct Plan {
 	u32* error;          // bit 0: hash table full, bit 3: group key outside [min, max]
 };
 
+// A work item of a batched launch: rows [r0, r1) of one pipeline's build or
+// probe side.
+struct BatchItem {
+	int plan;            // pipeline index
+	int phase;           // 0 = build rows of builds[build], 1 = probe rows of plans[plan]
+	int build;           // index into the build plan array (phase 0)
+	int reserved;
+	i64 r0;
+	i64 r1;
+};
+
 struct BuildPlan {
 	i64 rows;
 	int npreds;
