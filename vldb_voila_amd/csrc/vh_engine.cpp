@@ -510,13 +510,21 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		// built-in heuristic; the autotuner and the learned selector override
 		// it.  A shared LDS table beats lane-private accumulators even at a
 		// handful of groups: the latter's LDS footprint (groups x sums x 12
-		// bytes per lane) caps the workgroups a CU can hold.
+		// bytes per lane) caps the workgroups a CU can hold.  When a group
+		// key is the payload of a filtered build, the groups present are
+		// usually far fewer than the domain of the payload columns (star:
+		// 175 of 4,375), so the LDS hash table (which adds the groups it
+		// cannot hold into global memory) is taken up to 16 K; otherwise
+		// only while every group fits it.
+		bool keys_filtered = false;
+		for (int k = 0; k < P.nkeys; k++)
+			if (P.keys[k].r.kind == ROLE_PAY && d->joins[P.keys[k].r.slot].build.npreds > 0)
+				keys_filtered = true;
+		const double hash_limit = (keys_filtered ? 16.0 : 1.0) * kLdsHashSlots;
 		if (lds_need <= 32 * 1024)
 			mode = GB_LDS;
-		else if (domain <= 16.0 * kLdsHashSlots && P.nsums <= 4)
-			mode = GB_LDSHASH;      // groups present are often far fewer than the
-			                        // domain (keys from filtered builds); those the
-			                        // LDS table cannot hold add into global memory
+		else if (domain <= hash_limit && P.nsums <= 4)
+			mode = GB_LDSHASH;
 		else if (part_fits(domain, P.nsums))
 			mode = GB_PART;
 		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
