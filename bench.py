@@ -22,8 +22,8 @@ select + probe + aggregate kernel, the merge of per-workgroup partials.
 Inputs rotate over enough copies that the working set exceeds the 256 MB
 Infinity Cache, so the figures are HBM figures.
 
-Prints one JSON line: `roofline` is the probe kernel (HIP events on its
-stream around its launch inside the timed region); `cpu_baseline` times the
+Prints one JSON line: `roofline` is the probe kernel (HIP events its launch
+stamps with the kernel's own start and end, inside the timed region); `cpu_baseline` times the
 oracle's vector(1024) restatement on the host cores over a bounded sample,
 and its rows double as the check of the GPU result (the run fails on a
 difference).
@@ -168,7 +168,7 @@ def main():
         queries[i % copies].run(stream)
     stream.synchronize()
 
-    # timed region: K steps; the probe kernel bracketed by events on its stream
+    # timed region: K steps; the probe kernel timed by events its launch stamps
     K = args.steps
     ev = [(runtime.Event(), runtime.Event()) for _ in range(K)]
     t_start, t_stop = runtime.Event(), runtime.Event()
@@ -179,13 +179,14 @@ def main():
         q = queries[i % copies]
         q.run_phase(0, stream)
         q.run_phase(1, stream)
-        ev[i][0].record(stream)
+        q.probe_events(*ev[i])      # stamped by the probe kernel's launch itself
         q.run_phase(2, stream)
-        ev[i][1].record(stream)
         q.run_phase(3, stream)
     t_stop.record(stream)
     stream.synchronize()
     wall = time.perf_counter() - wall0
+    for q in queries:
+        q.probe_events(None, None)
     ms_per_step = t_start.elapsed_ms(t_stop) / K
     probe_avg_ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
     achieved = probe_bytes / (probe_avg_ms / 1e3) / 1e9

@@ -271,6 +271,13 @@ int vh_query_info(const vh_query* q, int32_t* group_mode, int32_t* grid, int64_t
  * generates C++ per query and compiles it with g++ -O3 -shared
  * (codegen.cpp:211-395, compiler.cpp:19-55). */
 int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, size_t* len);
+/* Events (vh_event_create) that later runs stamp with the generated probe
+ * kernel's own start and end (the launch records them itself, so the time
+ * between them excludes the dispatch gap that events recorded on the stream
+ * around it include); NULL, NULL stops it.  VH_ERR_UNSUPPORTED when the
+ * query runs the generic kernel.  No counterpart in the reference, which
+ * times whole queries (compiler.cpp:576-588). */
+int vh_query_probe_events(vh_query* q, void* start, void* stop);
 /* Layout of join j's table: entry_bytes 1, 2 or 4 for a direct table
  * (entries = key range), 8 for hash slots (entries = slots). */
 int vh_query_join(const vh_query* q, int32_t j, int32_t* entry_bytes, int64_t* entries);

@@ -237,6 +237,17 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 	return VH_OK;
 }
 
+int vh_query_probe_events(vh_query* q, void* start, void* stop)
+{
+	if (!q)
+		return fail(VH_ERR_INVALID, "null query");
+	if (!q->q.jit_fn && (start || stop))
+		return fail(VH_ERR_UNSUPPORTED, "kernel timestamps are recorded for the generated probe kernel only");
+	q->q.probe_start = (hipEvent_t)start;
+	q->q.probe_stop = (hipEvent_t)stop;
+	return VH_OK;
+}
+
 int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, size_t* len)
 {
 	if (!q)

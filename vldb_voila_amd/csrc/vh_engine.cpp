@@ -864,7 +864,7 @@ int query_run(Query* q, hipStream_t s, int phase)
 	}
 	if (phase == -1 || phase == 2) {
 		if (q->jit_fn)
-			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s));
+			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s, q->probe_start, q->probe_stop));
 		else
 			HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));
 	}

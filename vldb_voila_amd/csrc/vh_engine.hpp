@@ -56,6 +56,8 @@ struct Query {
 	bool ran = false;
 	std::string result;
 	hipFunction_t jit_fn = nullptr;   // generated probe kernel (null: generic)
+	hipEvent_t probe_start = nullptr;  // timestamps of the generated kernel's own start and end
+	hipEvent_t probe_stop = nullptr;
 	std::string jit_src;
 	~Query();
 	int alloc(void** p, size_t bytes);

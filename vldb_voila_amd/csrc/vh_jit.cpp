@@ -17,6 +17,7 @@
 // before the predicates, so a chunk costs one round trip to memory for its
 // columns plus one per join, and register use is what this plan needs.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -826,12 +827,16 @@ int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, con
 	return VH_OK;
 }
 
-hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s)
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s,
+	hipEvent_t start, hipEvent_t stop)
 {
 	const Plan* arg = dP;
 	void* args[] = {(void*)&arg};
-	return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)pipeline_lds_bytes(P), s, args,
-		nullptr);
+	const unsigned lds = (unsigned)pipeline_lds_bytes(P);
+	if (start || stop)
+		return hipExtModuleLaunchKernel(fn, (uint32_t)grid * (uint32_t)block, 1, 1, (uint32_t)block, 1, 1, lds, s,
+			args, nullptr, start, stop, 0);
+	return hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, lds, s, args, nullptr);
 }
 
 } // namespace vh

@@ -42,6 +42,9 @@ inline bool jit_pipeline(int kernel) { return kernel != VH_KERNEL_JIT_LAZY; }
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, const char* name = "vh_generated");
 // Launch a compiled kernel over P (dP: its device copy).  GB_REG / GB_LDS
 // leave per-block partials for launch_merge.
-hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s);
+// start / stop (may be null): events the runtime stamps with the kernel's
+// own start and end (hipExtModuleLaunchKernel).
+hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s,
+	hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 } // namespace vh

@@ -222,6 +222,13 @@ class Query:
         check(lib().vh_query_kernel(self.ptr, None, buf, n.value + 1, C.byref(n)))
         return k.value, buf.value.decode()
 
+    def probe_events(self, start: "Event | None", stop: "Event | None"):
+        """Events later runs stamp with the generated probe kernel's own
+        start and end (None, None: stop)."""
+        L = lib()
+        L.vh_query_probe_events.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        check(L.vh_query_probe_events(self.ptr, start.ptr if start else None, stop.ptr if stop else None))
+
     def reset(self):
         check(lib().vh_query_reset(self.ptr))
 
