@@ -11,7 +11,12 @@
    and stores the timings in SQLite (explorer.cpp:33-86, compiler.cpp:124-255).
 2. Trains the selector MLP on the training pipelines with the HIP kernels.
 3. Evaluates on the held-out pipelines: the selector's choice (exhaustive
-   fallback below 0.5 confidence) against the exhaustive best and the default.
+   fallback below 0.5 confidence) against the exhaustive best, the built-in
+   default heuristic, and the best fixed blend: the one variant (applicable
+   to every pipeline) with the lowest total over the training pipelines,
+   used for all of them -- VOILA's default blend, one flavor for every
+   pipeline (QueryConfig::default_blend, runtime_framework.hpp:78), as the
+   explorer's base mode picks it (explorer.cpp:112-134).
 
 Writes timings.npz, model.npz and report.json into DIR, printing progress.
 """
@@ -56,9 +61,19 @@ def explore(count, seed, reps=2, log_every=100, out=None):
     return X, T, params
 
 
-def evaluate(net, mu, sd, X, T, params):
+def fixed_blend(T_train, T_eval):
+    """Index of the variant that applies to every pipeline of both sets and
+    has the lowest total time over the training pipelines."""
+    vt, ve = T_train[:, :selector.N_VARIANTS], T_eval[:, :selector.N_VARIANTS]
+    ok = (vt > 0).all(0) & (ve > 0).all(0)
+    tot = np.where(vt > 0, vt, np.inf).sum(0)
+    return int(np.argmin(np.where(ok, tot, np.inf)))
+
+
+def evaluate(net, mu, sd, X, T, params, fixed=None):
     """Selector decision per held-out pipeline, priced with the measured
-    timings: chosen variant (or the exhaustive best when it falls back)."""
+    timings: chosen variant (or the exhaustive best when it falls back).
+    fixed: the fixed blend's variant index (see fixed_blend)."""
     var = T[:, :selector.N_VARIANTS]
     default = T[:, selector.N_VARIANTS]
     mask = (var > 0).astype(np.uint8)
@@ -70,7 +85,14 @@ def evaluate(net, mu, sd, X, T, params):
     chosen = np.where(fallback, best, var[np.arange(len(p)), pick])
     theta = np.array([q["theta"] for q in params])
     skew = theta >= 1.0
-    return {
+    extra = {}
+    if fixed is not None:
+        f = var[:, fixed]
+        extra = {"fixed_blend": {"variant": fixed, "total_ms": float(f.sum()),
+                                 "over_selector_total": float(f.sum() / chosen.sum()),
+                                 "over_selector_skewed": float(f[skew].sum() / chosen[skew].sum()) if skew.any() else None,
+                                 "over_best_total": float(f.sum() / best.sum())}}
+    return dict(extra, **{
         "heldout": int(len(X)),
         "fallback_fraction": float(fallback.mean()),
         "chosen_over_best_mean": float(np.mean(chosen / best)),
@@ -82,7 +104,7 @@ def evaluate(net, mu, sd, X, T, params):
         "default_over_selector_skewed": float(default[skew].sum() / chosen[skew].sum()) if skew.any() else None,
         "default_over_best_skewed": float(default[skew].sum() / best[skew].sum()) if skew.any() else None,
         "top1_accuracy": float(np.mean(pick == np.where(var > 0, var, np.inf).argmin(1))),
-    }
+    })
 
 
 def main(argv=None):
@@ -112,7 +134,7 @@ def main(argv=None):
     train_s = time.time() - t0
     print(f"trained {a.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.3f} -> {losses[-1]:.3f}", flush=True)
     np.savez(os.path.join(a.out, "model.npz"), params=net.params(), mu=mu, sd=sd, losses=np.array(losses))
-    rep = evaluate(net, mu, sd, Xho, Tho, pho)
+    rep = evaluate(net, mu, sd, Xho, Tho, pho, fixed=fixed_blend(Ttr, Tho))
     rep["train"] = {"pipelines": int(len(Xtr)), "epochs": a.epochs, "seconds": train_s,
                     "loss_first": losses[0], "loss_last": losses[-1]}
     rep["train_fit"] = evaluate(net, mu, sd, Xtr, Ttr, ptr)
