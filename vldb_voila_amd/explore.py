@@ -30,14 +30,14 @@ import time
 
 import numpy as np
 
-from . import datagen, runtime, selector, workloads
+from . import datagen, runlog, runtime, selector, workloads
 
 PROBE_RANGE = (1e4, 5e5)
 BUILD_RANGE = (1e3, 1e5)
 GROUP_RANGE = (1, 1e6)
 
 
-def explore(count, seed, reps=2, log_every=100, out=None):
+def explore(count, seed, reps=2, log_every=100, out=None, log=None):
     params = datagen.batch_params(count, seed=seed, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
                                   group_range=GROUP_RANGE)
     cfgs = selector.variant_configs() + [runtime.make_config()]   # 48 variants + the default
@@ -53,6 +53,18 @@ def explore(count, seed, reps=2, log_every=100, out=None):
         d = workloads.batch_pipeline(prm, b, p, runtime.addrs(db), runtime.addrs(dp)).desc()
         X[i] = selector.features(prm)
         T[i] = selector.time_variants(d, reps=reps, stream=stream, configs=cfgs)
+        if log is not None:
+            # one row per variant in the reference's run log: the pipeline is
+            # the query, the variant its blend (the default last, with mode
+            # explore_default)
+            q = f"pipeline_s{seed}_{i}"
+            for v, c in enumerate(cfgs):
+                t = float(T[i, v])
+                runlog.record(log, query=q, blend=runlog.blend_string(c), time_ms=t if t > 0 else None,
+                              tuples=float(prm["probe_rows"] + prm["build_rows"]),
+                              result="OK" if t > 0 else "CRASH",
+                              mode="explore" if v < len(cfgs) - 1 else "explore_default")
+            log.commit()
         del db, dp
         if (i + 1) % log_every == 0:
             print(f"explore seed={seed}: {i + 1}/{count} pipelines, {time.time() - t0:.0f}s", flush=True)
@@ -115,6 +127,8 @@ def main(argv=None):
     ap.add_argument("--out", default="gpurun_out/selector")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--timings", default=None, help="reuse an existing timings.npz")
+    ap.add_argument("--runlog", default=None, help="also record every timing in this SQLite run log "
+                                                    "(the reference's runs table)")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     runtime.init(0)
@@ -126,8 +140,10 @@ def main(argv=None):
         ptr = datagen.batch_params(len(Xtr), seed=42, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
                                    group_range=GROUP_RANGE)
     else:
-        Xtr, Ttr, ptr = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"))
-        Xho, Tho, pho = explore(a.heldout, seed=4242, reps=a.reps, out=os.path.join(a.out, "partial_heldout.npz"))
+        log = runlog.open_log(a.runlog) if a.runlog else None
+        Xtr, Ttr, ptr = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"), log=log)
+        Xho, Tho, pho = explore(a.heldout, seed=4242, reps=a.reps, out=os.path.join(a.out, "partial_heldout.npz"),
+                                log=log)
         np.savez(os.path.join(a.out, "timings.npz"), Xtr=Xtr, Ttr=Ttr, Xho=Xho, Tho=Tho)
     t0 = time.time()
     net, mu, sd, losses = selector.train(Xtr, Ttr[:, :selector.N_VARIANTS], epochs=a.epochs)
