@@ -59,7 +59,7 @@ enum vh_aggr_fn { VH_A_SUM = 0, VH_A_COUNT };
 enum vh_join_variant { VH_JOIN01 = 0, VH_JOINN };
 
 #define VH_MAX_COLS     8
-#define VH_MAX_PREDS    4
+#define VH_MAX_PREDS    8
 #define VH_MAX_JOINS    4
 #define VH_MAX_PAYLOAD  2
 #define VH_MAX_KEYS     4

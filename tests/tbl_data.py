@@ -1,0 +1,99 @@
+"""Synthetic TPC-H-format text for the importer tests: a seeded lineitem.tbl
+with the reference schema's sixteen '|'-terminated fields, and the Q1 / Q6
+shaped pipelines (TPC-H's definitions, decimals scaled by 100 as the
+importer stores them)."""
+import datetime
+
+import numpy as np
+
+from vldb_voila_amd import tbl
+from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
+
+FLAGS = "RAN"
+STATUS = "OF"
+EPOCH = datetime.date(1992, 1, 1)
+SPAN_DAYS = 2526                       # 1992-01-01 .. 1998-12-01
+
+
+def write_lineitem(path, rows, seed=1):
+    rng = np.random.default_rng(seed)
+    qty = rng.integers(1, 51, rows)
+    price = rng.integers(90000, 10500000, rows)
+    disc = rng.integers(0, 11, rows)
+    tax = rng.integers(0, 9, rows)
+    flag = rng.integers(0, 3, rows)
+    status = rng.integers(0, 2, rows)
+    ship = rng.integers(0, SPAN_DAYS, rows)
+    lines = []
+    for i in range(rows):
+        d = EPOCH + datetime.timedelta(days=int(ship[i]))
+        fields = [str(i // 4 + 1), str(i % 997 + 1), str(i % 101 + 1), str(i % 4 + 1),
+                  f"{qty[i]}.00", f"{price[i] // 100}.{price[i] % 100:02d}", f"0.{disc[i]:02d}", f"0.{tax[i]:02d}",
+                  FLAGS[flag[i]], STATUS[status[i]], d.isoformat(), d.isoformat(), d.isoformat(),
+                  "DELIVER IN PERSON", "TRUCK", "a comment"]
+        lines.append("|".join(fields) + "|\n")
+    with open(path, "w") as f:
+        f.writelines(lines)
+
+
+Q1_COLS = ["l_quantity", "l_extendedprice", "l_discount", "l_tax", "l_returnflag", "l_linestatus", "l_shipdate"]
+Q1_CUTOFF = tbl.julian_day(1998, 9, 2)          # date '1998-12-01' - interval '90' day
+Q6_FROM, Q6_TO = tbl.julian_day(1994, 1, 1), tbl.julian_day(1995, 1, 1)
+
+
+def q1(arrays, bounds, addrs=None):
+    li = table_from_arrays("lineitem", arrays, addrs, bounds)
+    p = Pipeline(li)
+    p.select("l_shipdate", "<=", Q1_CUTOFF)
+    p.groupby(["l_returnflag", "l_linestatus"], [
+        ("sum", col("l_quantity")),
+        ("sum", col("l_extendedprice")),
+        ("sum", col("l_extendedprice") * (100 - col("l_discount"))),
+        ("sum", col("l_extendedprice") * (100 - col("l_discount")) * (100 + col("l_tax"))),
+        ("count",),
+    ])
+    return p
+
+
+def q6(arrays, bounds, addrs=None):
+    li = table_from_arrays("lineitem", arrays, addrs, bounds)
+    p = Pipeline(li)
+    p.select("l_shipdate", ">=", Q6_FROM)
+    p.select("l_shipdate", "<", Q6_TO)
+    p.select("l_discount", ">=", 5)
+    p.select("l_discount", "<=", 7)
+    p.select("l_quantity", "<", 2400)
+    p.groupby([], [("sum", col("l_extendedprice") * col("l_discount"))])
+    return p
+
+
+def q1_from_text(path):
+    """Q1 straight from the text with Python integers: sorted result rows."""
+    groups = {}
+    for line in open(path):
+        f = line.split("|")
+        q, p, d, t = (int(round(float(x) * 100)) for x in f[4:8])
+        y, m, dd = (int(x) for x in f[10].split("-"))
+        if tbl.julian_day(y, m, dd) > Q1_CUTOFF:
+            continue
+        key = (ord(f[8]), ord(f[9]))
+        g = groups.setdefault(key, [0, 0, 0, 0, 0])
+        g[0] += q
+        g[1] += p
+        g[2] += p * (100 - d)
+        g[3] += p * (100 - d) * (100 + t)
+        g[4] += 1
+    return sorted("|".join(str(v) for v in (*k, *g)) for k, g in groups.items())
+
+
+def q6_from_text(path):
+    s, n = 0, 0
+    for line in open(path):
+        f = line.split("|")
+        q, p, d = (int(round(float(x) * 100)) for x in f[4:7])
+        y, m, dd = (int(x) for x in f[10].split("-"))
+        j = tbl.julian_day(y, m, dd)
+        if Q6_FROM <= j < Q6_TO and 5 <= d <= 7 and q < 2400:
+            s += p * d
+            n += 1
+    return s, n

@@ -31,7 +31,7 @@ def test_descriptor_sizes():
     assert C.sizeof(abi.vh_column) == 32
     assert C.sizeof(abi.vh_pred) == 16
     assert C.sizeof(abi.vh_expr) == 24
-    assert C.sizeof(abi.vh_table_desc) == 16 + 8 * 32 + 4 * 16
+    assert C.sizeof(abi.vh_table_desc) == 16 + abi.VH_MAX_COLS * 32 + abi.VH_MAX_PREDS * 16
     assert C.sizeof(abi.vh_config) == 32
 
 

@@ -12,7 +12,7 @@ VH_A_SUM, VH_A_COUNT = range(2)
 VH_JOIN01, VH_JOINN = range(2)
 
 VH_MAX_COLS = 8
-VH_MAX_PREDS = 4
+VH_MAX_PREDS = 8
 VH_MAX_JOINS = 4
 VH_MAX_PAYLOAD = 2
 VH_MAX_KEYS = 4

@@ -38,9 +38,10 @@ class Table:
         raise KeyError(f"{self.name}: no column {name!r}")
 
 
-def table_from_arrays(name, arrays: dict, addrs: dict | None = None) -> Table:
+def table_from_arrays(name, arrays: dict, addrs: dict | None = None, bounds: dict | None = None) -> Table:
     """Table over numpy arrays; `addrs` overrides the data address per column
-    (device copies of the same arrays)."""
+    (device copies of the same arrays); `bounds` gives a column's [min, max]
+    metadata (as an importer records it) instead of the data's."""
     rows = None
     t = Table(name, 0)
     for cname, arr in arrays.items():
@@ -48,8 +49,11 @@ def table_from_arrays(name, arrays: dict, addrs: dict | None = None) -> Table:
             rows = len(arr)
         assert len(arr) == rows, "ragged table"
         addr = addrs[cname] if addrs is not None else arr.ctypes.data
-        vmin = int(arr.min()) if len(arr) else 0
-        vmax = int(arr.max()) if len(arr) else 0
+        if bounds is not None and cname in bounds:
+            vmin, vmax = bounds[cname]
+        else:
+            vmin = int(arr.min()) if len(arr) else 0
+            vmax = int(arr.max()) if len(arr) else 0
         t.add(cname, abi.vh_type_of(arr.dtype), addr, vmin, vmax)
     t.rows = rows or 0
     return t
