@@ -32,3 +32,20 @@ def test_tpch_shapes_on_imported_columns(lineitem, query, kernel):
     assert got == want
     if query == "q1":
         assert len(want) == 6
+
+
+def test_relalg_tpch_plans_on_the_gpu(lineitem):
+    """Q1 and Q6 as relalg plans over the imported catalog, run on the GPU."""
+    print("tbl: relalg Q1 / Q6", flush=True)
+    from vldb_voila_amd import relalg
+    from tbl_data import Q1_CUTOFF, Q6_FROM, Q6_TO
+    a, b = lineitem
+    dev = runtime.upload(a)
+    db = relalg.Database().add_table("lineitem", a, runtime.addrs(dev), bounds=b)
+    host = relalg.Database().add_table("lineitem", a, bounds=b)
+    for plan in (relalg.tpch_q1_plan(Q1_CUTOFF), relalg.tpch_q6_plan(Q6_FROM, Q6_TO)):
+        want = oracle.sorted_rows(oracle.bruteforce(host.lower(plan)))
+        q = db.query(plan)
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+        q.free()

@@ -83,3 +83,18 @@ def test_q1_q6_on_imported_columns_match_the_text(tmp_path):
     s, n = q6_from_text(path)
     assert n > 0
     assert oracle.sorted_rows(oracle.bruteforce(q6(a, b).desc())) == [str(s)]
+
+
+def test_relalg_q1_q6_plans_over_imported_columns(tmp_path):
+    """The same queries written as relalg plans over the imported catalog
+    lower to pipelines with the same answers."""
+    from vldb_voila_amd import relalg
+    from tbl_data import Q1_CUTOFF, Q6_FROM, Q6_TO
+    path = tmp_path / "lineitem.tbl"
+    write_lineitem(path, 2000, seed=9)
+    a, b = tbl.load_table(str(tmp_path), "lineitem", Q1_COLS)
+    db = relalg.Database().add_table("lineitem", a, bounds=b)
+    assert oracle.sorted_rows(oracle.bruteforce(db.lower(relalg.tpch_q1_plan(Q1_CUTOFF)))) == q1_from_text(path)
+    d6 = db.lower(relalg.tpch_q6_plan(Q6_FROM, Q6_TO))
+    assert d6.probe.npreds == 5
+    assert oracle.sorted_rows(oracle.bruteforce(d6)) == [str(q6_from_text(path)[0])]

@@ -38,11 +38,16 @@ class Database:
         check(self.L.vh_db_new(C.byref(p)))
         self.ptr = p.value
 
-    def add_table(self, name, arrays: dict, addrs: dict | None = None):
+    def add_table(self, name, arrays: dict, addrs: dict | None = None, bounds: dict | None = None):
+        """bounds: a column's [min, max] as an importer recorded it (default:
+        the data's)."""
         for col, arr in arrays.items():
             addr = addrs[col] if addrs is not None else arr.ctypes.data
-            lo = int(arr.min()) if len(arr) else 0
-            hi = int(arr.max()) if len(arr) else 0
+            if bounds is not None and col in bounds:
+                lo, hi = bounds[col]
+            else:
+                lo = int(arr.min()) if len(arr) else 0
+                hi = int(arr.max()) if len(arr) else 0
             check(self.L.vh_db_add_column(self.ptr, name.encode(), col.encode(), abi.vh_type_of(arr.dtype),
                                           addr, len(arr), lo, hi))
         return self
@@ -86,6 +91,25 @@ def join_aggregate_plan(cutoff: int, variant="joinn") -> str:
   (hashjoin {variant}
     (select (<= date {cutoff}) (scan probe fk date price)) (keys fk) (payl)
     (scan build key segment) (keys key) (payl segment)))"""
+
+
+def tpch_q1_plan(cutoff: int) -> str:
+    """TPC-H Q1 over imported lineitem columns (decimals scaled by 100, so
+    1 - l_discount is 100 - l_discount); the averages are the sums over the
+    count."""
+    return f"""(hashaggr hash (keys l_returnflag l_linestatus)
+  (aggrs (sum l_quantity) (sum l_extendedprice) (sum (* l_extendedprice (- 100 l_discount)))
+         (sum (* (* l_extendedprice (- 100 l_discount)) (+ 100 l_tax))) (count))
+  (select (<= l_shipdate {cutoff})
+    (scan lineitem l_returnflag l_linestatus l_quantity l_extendedprice l_discount l_tax l_shipdate)))"""
+
+
+def tpch_q6_plan(date_from: int, date_to: int) -> str:
+    """TPC-H Q6: discount between 0.05 and 0.07, quantity below 24."""
+    return f"""(hashaggr global (keys) (aggrs (sum (* l_extendedprice l_discount)))
+  (select (and (>= l_shipdate {date_from}) (and (< l_shipdate {date_to})
+          (and (>= l_discount 5) (and (<= l_discount 7) (< l_quantity 2400)))))
+    (scan lineitem l_shipdate l_discount l_quantity l_extendedprice)))"""
 
 
 def star_plan(region: int) -> str:
