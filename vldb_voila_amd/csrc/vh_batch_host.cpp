@@ -38,6 +38,8 @@ struct BatchLaunch {
 };
 
 static constexpr int kBatchGenerated = -3;
+static constexpr int kBatchScatter = -4;   // partitioned builds: chunk items
+static constexpr int kBatchSlice = -5;     // partitioned builds: partition items
 // probe items' grouping key (BatchItem::build) of generated kernel g
 static constexpr int kGenKey = 1000;
 
@@ -122,7 +124,10 @@ static hipError_t enqueue(vh_batch* b, hipStream_t s)
 	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
 	for (size_t k = 0; k < b->launches.size() && e == hipSuccess; k++) {
 		const BatchLaunch& l = b->launches[k];
-		if (l.kind == kBatchGenerated) {
+		if (l.kind == kBatchScatter || l.kind == kBatchSlice) {
+			e = launch_batch_part(l.kind == kBatchSlice, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k,
+				b->grid / 4, l.lds, s);
+		} else if (l.kind == kBatchGenerated) {
 			if (!l.count)
 				continue;
 			const Plan* plans = b->d_plans;
@@ -151,7 +156,8 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		return fail(VH_ERR_NOMEM, "host allocation");
 	std::vector<Plan> plans;
 	std::vector<BuildPlan> builds;
-	std::vector<BatchItem> build_items, probe_items, agg_items;
+	std::vector<BatchItem> build_items, probe_items, agg_items, scatter_items, slice_items;
+	size_t slice_lds = 0;
 	std::vector<FillSeg> segs;
 	std::vector<std::string> gen_src;        // generated batch kernels, by index
 	std::vector<hipFunction_t> gen_fn;
@@ -179,11 +185,21 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		b->mode_lds[P.mode] = std::max(b->mode_lds[P.mode], pipeline_lds_bytes(P));
 		const int first_build = (int)builds.size();
 		for (int j = 0; j < P.njoins; j++) {
-			builds.push_back(q->L.builds[j]);
-			for (i64 r0 = 0; r0 < q->L.builds[j].rows; r0 += T)
-				build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, q->L.builds[j].rows)});
+			const BuildPlan& B = q->L.builds[j];
+			builds.push_back(B);
+			if (B.nparts) {
+				// a direct table built in two LDS passes, as a single query's
+				for (int c = 0; c < B.nchunks; c++)
+					scatter_items.push_back({i, 0, first_build + j, 0, c, c + 1});
+				for (int p = 0; p < B.nparts; p++)
+					slice_items.push_back({i, 0, first_build + j, 0, p, p + 1});
+				slice_lds = std::max(slice_lds, part_slice_lds_bytes(B));
+			} else {
+				for (i64 r0 = 0; r0 < B.rows; r0 += T)
+					build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, B.rows)});
+			}
 			segs.push_back({q->slots[j], q->L.capacity[j], ~0ull});
-			if (q->dir[j])
+			if (q->dir[j] && !B.nparts)      // a partitioned build rewrites all of dir[]
 				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
 		}
 		// the probe kernel: generated for this pipeline's plan (shared with
@@ -241,16 +257,21 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		[](const BatchItem& x, const BatchItem& y) { return x.build < y.build; });
 	std::vector<BatchItem> items = build_items;
 	b->launches.push_back({kBatchBuild, 0, (u32)build_items.size(), 0, nullptr});
+	b->launches.push_back({kBatchScatter, (u32)items.size(), (u32)scatter_items.size(), 0, nullptr});
+	items.insert(items.end(), scatter_items.begin(), scatter_items.end());
+	b->launches.push_back({kBatchSlice, (u32)items.size(), (u32)slice_items.size(), slice_lds, nullptr});
+	items.insert(items.end(), slice_items.begin(), slice_items.end());
+	const size_t nbuild_items = items.size();
 	for (size_t k = 0; k < probe_items.size();) {
 		size_t e = k;
 		while (e < probe_items.size() && probe_items[e].build == probe_items[k].build)
 			e++;
 		const int key = probe_items[k].build;
 		if (key >= kGenKey)
-			b->launches.push_back({kBatchGenerated, (u32)(build_items.size() + k), (u32)(e - k),
+			b->launches.push_back({kBatchGenerated, (u32)(nbuild_items + k), (u32)(e - k),
 				gen_lds[key - kGenKey], gen_fn[key - kGenKey]});
 		else
-			b->launches.push_back({key, (u32)(build_items.size() + k), (u32)(e - k), b->mode_lds[key], nullptr});
+			b->launches.push_back({key, (u32)(nbuild_items + k), (u32)(e - k), b->mode_lds[key], nullptr});
 		k = e;
 	}
 	items.insert(items.end(), probe_items.begin(), probe_items.end());

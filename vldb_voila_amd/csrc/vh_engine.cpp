@@ -732,8 +732,6 @@ int query_create(Query* q, bool single)
 			L.builds[j].dir = q->dir[j];
 		}
 		BuildPlan& B = L.builds[j];
-		if (!single)
-			B.nparts = 0;   // batched builds claim entries in place
 		if (B.nparts) {
 			const size_t g = (size_t)std::max(B.nchunks, 1);
 			if ((rc = q->alloc((void**)&B.pbuf, g * kBuildChunkRows * sizeof(u64))) ||

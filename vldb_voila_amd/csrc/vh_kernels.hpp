@@ -13,6 +13,13 @@ hipError_t launch_build(const BuildPlan* dB, int grid, hipStream_t s);
 // Partitioned build of a direct table (B.nparts > 0): per-chunk LDS sort,
 // then one LDS-built slice per partition.  B: host copy; dB: device copy.
 hipError_t launch_part_build(const BuildPlan& B, const BuildPlan* dB, hipStream_t s);
+// The same two steps over batch work items: scatter items (BatchItem::build,
+// chunk r0) or, with `slices`, partition items (build, partition r0), taken
+// from a queue by `grid` persistent workgroups; slice_bytes = the largest
+// part_slice_lds_bytes of the builds involved.
+hipError_t launch_batch_part(bool slices, const BuildPlan* d_builds, const BatchItem* d_items, u32 first, u32 count,
+	u32* d_counter, int grid, size_t slice_bytes, hipStream_t s);
+size_t part_slice_lds_bytes(const BuildPlan& B);
 static constexpr int kMaxBuildParts = 4096;      // partitions of one build
 static constexpr int kMaxBuildChunks = 2048;     // 4096-row chunks of one build
 static constexpr int kBuildChunkRows = 4096;

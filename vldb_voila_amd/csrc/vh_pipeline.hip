@@ -170,21 +170,21 @@ __device__ __forceinline__ void overflow_insert(const BuildPlan& B, u32 off, u64
 	}
 }
 
-__global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPlan* __restrict__ Bp)
+// Chunk `chunk` of a partitioned build (the workgroup's LDS: hist, staged,
+// total): select its rows, count them per partition, store them sorted by
+// partition and record each partition's run.
+__device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk, u32* hist, u64* staged,
+	u32* total_slot)
 {
-	const BuildPlan& B = *Bp;
-	__shared__ u32 hist[kMaxParts];
-	__shared__ u64 staged[kPartChunk];
-	__shared__ u32 total;
 	const int P = B.nparts;
-	// the repeated-key flag of this build: set (if at all) by
-	// part_build_kernel, which runs after every workgroup of this launch
-	if (blockIdx.x == 0 && threadIdx.x == 0)
+	// the repeated-key flag of this build: set (if at all) by the slice
+	// builds, which run after every chunk of this launch
+	if (chunk == 0 && threadIdx.x == 0)
 		*B.dups = 0u;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
 		hist[p] = 0;
 	__syncthreads();
-	const i64 r0 = (i64)blockIdx.x * kPartChunk;
+	const i64 r0 = (i64)chunk * kPartChunk;
 	u32 off[kU], pay[kU], rank[kU];
 	bool ok[kU];
 	// every column of the kU rows is requested before any is tested (rows
@@ -228,11 +228,11 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	__syncthreads();
 	const u64 G = (u64)B.nchunks;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
-		((VH_GLOBAL u32*)B.pcnt)[(u64)p * G + blockIdx.x] = hist[p];
+		((VH_GLOBAL u32*)B.pcnt)[(u64)p * G + chunk] = hist[p];
 	__syncthreads();
-	const u32 n = block_exclusive_scan(hist, P, &total);
+	const u32 n = block_exclusive_scan(hist, P, total_slot);
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
-		((VH_GLOBAL u32*)B.poff)[(u64)p * G + blockIdx.x] = hist[p];
+		((VH_GLOBAL u32*)B.poff)[(u64)p * G + chunk] = hist[p];
 #pragma unroll
 	for (int u = 0; u < kU; u++)
 		if (ok[u])
@@ -241,18 +241,24 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	VH_GLOBAL u64* dst = (VH_GLOBAL u64*)B.pbuf + r0;
 	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
 		dst[i] = staged[i];
+	__syncthreads();   // the LDS is free for the workgroup's next chunk
 }
 
-__global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan* __restrict__ Bp)
+__global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPlan* __restrict__ Bp)
 {
-	const BuildPlan& B = *Bp;
-	extern __shared__ u64 lds_words[];
-	u32* slice = (u32*)lds_words;
-	__shared__ u32 pre[kMaxChunks];
-	__shared__ u32 base[kMaxChunks];
-	__shared__ u32 total_rows;
+	__shared__ u32 hist[kMaxParts];
+	__shared__ u64 staged[kPartChunk];
+	__shared__ u32 total;
+	part_scatter_chunk(*Bp, blockIdx.x, hist, staged, &total);
+}
+
+// Partition `p` of a partitioned build: gather its runs from every chunk,
+// claim entries in an LDS copy of its slice, store the slice (and its words
+// of the key bitmap).  LDS: the slice, pre and base (kMaxChunks each), total.
+__device__ __forceinline__ void part_build_slice(const BuildPlan& B, int p, u32* slice, u32* pre, u32* base,
+	u32* total_rows)
+{
 	constexpr int V = 2 * kU;   // rows gathered per thread per step
-	const int p = blockIdx.x;
 	const int P = B.nparts;
 	const int G = B.nchunks;
 	const int w = B.dir_w;
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 		base[b] = gload(B.poff, (i64)p * G + b);
 	}
 	__syncthreads();
-	const u32 total = block_exclusive_scan(pre, G, &total_rows);
+	const u32 total = block_exclusive_scan(pre, G, total_rows);
 	// row i of this partition (0 <= i < total) is pbuf[base[b] + i] for the
 	// last chunk b with pre[b] <= i
 	for (int b = threadIdx.x; b < G; b += blockDim.x)
@@ -325,6 +331,60 @@ __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan*
 			bdst[i] = word;
 		}
 	}
+	__syncthreads();   // the LDS is free for the workgroup's next partition
+}
+
+__global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan* __restrict__ Bp)
+{
+	extern __shared__ u64 lds_words[];
+	__shared__ u32 pre[kMaxChunks];
+	__shared__ u32 base[kMaxChunks];
+	__shared__ u32 total_rows;
+	part_build_slice(*Bp, blockIdx.x, (u32*)lds_words, pre, base, &total_rows);
+}
+
+// Batched partitioned builds: scatter items (build, chunk r0) and slice
+// items (build, partition r0) from a queue, as the one-query launches do
+// them by workgroup index; the slice launch runs after the scatter launch.
+__global__ __launch_bounds__(kPartBlock) void batch_part_scatter_kernel(const BuildPlan* __restrict__ builds,
+	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
+{
+	__shared__ u32 hist[kMaxParts];
+	__shared__ u64 staged[kPartChunk];
+	__shared__ u32 total;
+	__shared__ u32 slot;
+	for (;;) {
+		if (threadIdx.x == 0)
+			slot = gatomic_add(counter, 1u);
+		__syncthreads();
+		const u32 k = slot;
+		__syncthreads();
+		if (k >= count)
+			break;
+		const BatchItem I = items[first + k];
+		part_scatter_chunk(builds[I.build], (int)I.r0, hist, staged, &total);
+	}
+}
+
+__global__ __launch_bounds__(kPartBlock) void batch_part_slice_kernel(const BuildPlan* __restrict__ builds,
+	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
+{
+	extern __shared__ u64 lds_words[];
+	__shared__ u32 pre[kMaxChunks];
+	__shared__ u32 base[kMaxChunks];
+	__shared__ u32 total_rows;
+	__shared__ u32 slot;
+	for (;;) {
+		if (threadIdx.x == 0)
+			slot = gatomic_add(counter, 1u);
+		__syncthreads();
+		const u32 k = slot;
+		__syncthreads();
+		if (k >= count)
+			break;
+		const BatchItem I = items[first + k];
+		part_build_slice(builds[I.build], (int)I.r0, (u32*)lds_words, pre, base, &total_rows);
+	}
 }
 
 __global__ __launch_bounds__(kBlock) void fill_u64_kernel(u64* p, u64 n, u64 v, const u32* only_if)
@@ -347,6 +407,25 @@ hipError_t launch_fill(u64* p, u64 n, u64 v, hipStream_t s, const u32* only_if)
 		blocks = 4096;
 	hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n, v, only_if);
 	return hipGetLastError();
+}
+
+hipError_t launch_batch_part(bool slices, const BuildPlan* d_builds, const BatchItem* d_items, u32 first, u32 count,
+	u32* d_counter, int grid, size_t slice_bytes, hipStream_t s)
+{
+	if (!count)
+		return hipSuccess;
+	if (slices)
+		hipLaunchKernelGGL(batch_part_slice_kernel, dim3(grid), dim3(kPartBlock), slice_bytes, s, d_builds, d_items,
+			first, count, d_counter);
+	else
+		hipLaunchKernelGGL(batch_part_scatter_kernel, dim3(grid), dim3(kPartBlock), 0, s, d_builds, d_items, first,
+			count, d_counter);
+	return hipGetLastError();
+}
+
+size_t part_slice_lds_bytes(const BuildPlan& B)
+{
+	return (((size_t)1 << B.pshift) * (size_t)B.dir_w + 3) / 4 * 4;
 }
 
 hipError_t launch_part_build(const BuildPlan& B, const BuildPlan* dB, hipStream_t s)
