@@ -72,3 +72,43 @@ def test_batch_with_empty_pipelines():
     batch.run()
     for i in range(4):
         assert oracle.sorted_rows(batch.result(i)) == wants2[i]
+
+
+def test_batch_mixed_kernels_modes_and_repeated_keys():
+    """Generated and generic probe launches in one batch, every dense
+    group-by strategy, and builds with repeated keys (the partitioned builds'
+    overflow slots), over two runs and a graph replay."""
+    print("batch: mixed kernels, modes, repeated build keys", flush=True)
+    data, _ = small_batch(12, seed=21)
+    rng = np.random.default_rng(21)
+    cases, wants = [], []
+    for i, (prm, b, p) in enumerate(data):
+        if i % 3 == 0 and len(b["key"]) > 10:
+            b = dict(b)
+            k = b["key"].copy()
+            k[rng.integers(0, len(k), max(1, len(k) // 20))] = k[0]     # ~5% of rows repeat one key
+            b["key"] = k
+        cases.append((prm, b, p))
+        wants.append(oracle.sorted_rows(oracle.bruteforce(workloads.batch_pipeline(prm, b, p).desc())))
+    modes = [abi.VH_GROUP_LDS, abi.VH_GROUP_PART, abi.VH_GROUP_AUTO, abi.VH_GROUP_HASH]
+    devs, descs, cfgs = [], [], []
+    for i, (prm, b, p) in enumerate(cases):
+        db, dp = runtime.upload(b), runtime.upload(p)
+        devs.append((db, dp))
+        descs.append(workloads.batch_pipeline(prm, b, p, runtime.addrs(db), runtime.addrs(dp)).desc())
+        mode = modes[i % len(modes)]
+        if mode == abi.VH_GROUP_LDS and prm["groups"] > 1000:
+            mode = abi.VH_GROUP_AUTO       # the dense LDS table holds ~1,100 groups of these sums
+        kernel = abi.VH_KERNEL_GENERIC if i % 2 else abi.VH_KERNEL_AUTO
+        cfgs.append(runtime.make_config(group_mode=mode, kernel=kernel))
+    batch = runtime.Batch(descs, cfgs)
+    s = runtime.Stream()
+    for run in range(2):
+        batch.run(s)
+        for i in range(len(cases)):
+            assert oracle.sorted_rows(batch.result(i, s)) == wants[i], f"run {run} pipeline {i}"
+    batch.run_graph(s)
+    s.synchronize()
+    for i in range(len(cases)):
+        assert oracle.sorted_rows(batch.result(i, s)) == wants[i], f"graph pipeline {i}"
+    batch.free()
