@@ -350,6 +350,29 @@ int vh_vec_write(vh_table* t, int32_t col, int64_t base, const int32_t* sel, int
 /* bucket_build (ITable::build_index, runtime_struct.cpp:1283-1324) */
 int vh_vec_bucket_build(vh_table* t, void* stream);
 
+/* Varchar columns: one (1 + max_len)-byte slot per row, a length byte then
+ * the bytes (the binary column of Char<n> / Varchar<n>, db-engine-paradigms
+ * include/common/runtime/Types.hpp:125-130 and :243-250, as
+ * varchar(void* data, idx, max_len) reads it, runtime.hpp:192-205);
+ * 1 <= max_len <= 255.  Constants and needles are host memory of at most
+ * 255 bytes. */
+/* vec_hash on varchar: voila_hash<varchar>::hash, MurmurHash64A with seed 0
+ * (runtime.hpp:434-481; kernel_generator.py:465-468) */
+int vh_vec_hash_varchar(const int32_t* sel, int64_t inum, uint64_t* res, const uint8_t* col,
+	int32_t max_len, void* stream);
+/* vec_rehash on varchar: rehash(h, v) = hash(v, seed h) (runtime.hpp:482-484) */
+int vh_vec_rehash_varchar(const int32_t* sel, int64_t inum, uint64_t* res, const uint64_t* hash,
+	const uint8_t* col, int32_t max_len, void* stream);
+/* vec_contains: res[i] = col[i].contains(needle), memmem (runtime.hpp:206-208;
+ * kernel_generator.py:435-437) */
+int vh_vec_contains(const int32_t* sel, int64_t inum, uint8_t* res, const uint8_t* col, int32_t max_len,
+	const char* needle, int32_t needle_len, void* stream);
+/* vec_eq / vec_ne on varchar (kernel_generator.py:412-433): varchar::operator==
+ * (runtime.hpp:154-156) against col2 (max_len2) or, with col2 NULL, the constant */
+int vh_vec_cmp_varchar(int32_t cmp, const int32_t* sel, int64_t inum, uint8_t* res, const uint8_t* col1,
+	int32_t max_len1, const uint8_t* col2, int32_t max_len2, const char* constant, int32_t constant_len,
+	void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched pipelines: many independent pipelines in one persistent launch.   */
 /* The reference runs pipelines one at a time, each with its own flavor     */

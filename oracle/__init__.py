@@ -44,6 +44,13 @@ def lib():
         L.vo_type_from_minmax.argtypes = [C.c_double, C.c_double]
         L.vo_typing_of.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.vo_sizeof_pipeline_desc.restype = C.c_size_t
+        L.vo_murmur64a.restype = C.c_uint64
+        L.vo_murmur64a.argtypes = [C.c_char_p, C.c_size_t, C.c_uint64]
+        L.vo_vec_hash_varchar.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.vo_vec_contains.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int, C.c_char_p,
+                                      C.c_int]
+        L.vo_vec_cmp_varchar.argtypes = [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.c_void_p, C.c_int, C.c_char_p, C.c_int]
         _LIB = L
     return _LIB
 

@@ -53,5 +53,13 @@ def lib():
         L.ref_bucket_next.argtypes = [C.c_void_p, i32, C.c_void_p, C.c_void_p, u64]
         L.ref_check_u32.restype = i32
         L.ref_check_u32.argtypes = [C.c_void_p, i32, C.c_void_p, C.c_void_p, C.c_void_p, u64]
+        L.ref_hash_varchar.restype = u64
+        L.ref_hash_varchar.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.ref_rehash_varchar.restype = u64
+        L.ref_rehash_varchar.argtypes = [u64, C.c_void_p, C.c_size_t, C.c_size_t]
+        L.ref_varchar_contains.restype = C.c_int
+        L.ref_varchar_contains.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_char_p, C.c_uint32]
+        L.ref_varchar_eq.restype = C.c_int
+        L.ref_varchar_eq.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t]
         _LIB = L
     return _LIB

@@ -18,6 +18,36 @@ uint64_t ref_rehash_i32(uint64_t h, int32_t v) { return voila_hash<i32>::rehash(
 uint64_t ref_rehash_u32(uint64_t h, uint32_t v) { return voila_hash<u32>::rehash(h, v); }
 uint64_t ref_rehash_i64(uint64_t h, int64_t v) { return voila_hash<i64>::rehash(h, v); }
 
+// varchar(void* data, idx, max_len) over a column of (1 + max_len)-byte slots
+static varchar slot_of(const uint8_t* col, size_t idx, size_t max_len)
+{
+	return varchar((void*)col, idx, max_len);
+}
+
+uint64_t ref_hash_varchar(const uint8_t* col, size_t idx, size_t max_len)
+{
+	return voila_hash<varchar>::hash(slot_of(col, idx, max_len));
+}
+
+uint64_t ref_rehash_varchar(uint64_t h, const uint8_t* col, size_t idx, size_t max_len)
+{
+	return voila_hash<varchar>::rehash(h, slot_of(col, idx, max_len));
+}
+
+int ref_varchar_contains(const uint8_t* col, size_t idx, size_t max_len, const char* needle, uint32_t nlen)
+{
+	varchar n;
+	n.arr = needle;
+	n.len = (uint16_t)nlen;
+	return slot_of(col, idx, max_len).contains(n) ? 1 : 0;
+}
+
+int ref_varchar_eq(const uint8_t* col1, size_t idx1, size_t max_len1, const uint8_t* col2, size_t idx2,
+	size_t max_len2)
+{
+	return slot_of(col1, idx1, max_len1) == slot_of(col2, idx2, max_len2) ? 1 : 0;
+}
+
 size_t ref_to_cstr_i128(uint64_t lo, uint64_t hi, char* out)
 {
 	i128 v = (i128)(((u128)hi << 64) | lo);

@@ -28,6 +28,10 @@
  *   seltrue/selfalse/selunion against runtime.cpp (oracle/_ref) on random,
  *                   empty, full and sparse selections;
  *   to_cstr         against runtime.hpp (oracle/_ref) and Python str();
+ *   varchar hash    MurmurHash64A against SMHasher's published verification
+ *                   value and the reference's voila_hash<varchar> (oracle/_ref);
+ *   contains / eq   against Python's bytes.find / byte comparison and the
+ *                   reference's varchar::contains / operator== (oracle/_ref);
  *   calc_num_buckets against its closed form on small counts;
  *   vo_run_pipeline against vo_bruteforce_pipeline (different algorithm) and
  *                   hand-computed known answers on tiny inputs.
@@ -282,6 +286,92 @@ uint64_t vo_hash(int64_t canonical_value)
 uint64_t vo_rehash(uint64_t h, int64_t canonical_value)
 {
 	return vo_hash_u64((uint64_t)canonical_value) ^ (h * UINT64_C(2654435761));
+}
+
+/* ======================================================================== */
+/* Varchar columns (runtime.hpp:150-224)                                     */
+/* ======================================================================== */
+
+/* Slot idx of a varchar column as varchar(void* data, idx, max_len) reads it
+ * (runtime.hpp:192-205): (1 + max_len)-byte slots, a u8 length first.  The
+ * length is bounded by the slot, as the library bounds it (a well-formed
+ * column never exceeds it). */
+static const uint8_t* vo_varchar_at(const uint8_t* col, int max_len, int64_t idx, uint32_t* len)
+{
+	const uint8_t* p = col + (size_t)idx * (size_t)(max_len + 1);
+	*len = p[0] < (uint32_t)max_len ? p[0] : (uint32_t)max_len;
+	return p + 1;
+}
+
+/* voila_hash<varchar>::hash (runtime.hpp:434-481): MurmurHash64A. */
+uint64_t vo_murmur64a(const uint8_t* key, size_t len, uint64_t seed)
+{
+	const uint64_t m = UINT64_C(0xc6a4a7935bd1e995);
+	uint64_t h = seed ^ ((uint64_t)len * m);
+	size_t i = 0;
+	for (; i + 8 <= len; i += 8) {
+		uint64_t k;
+		memcpy(&k, key + i, 8);          /* little-endian host */
+		k *= m;
+		k ^= k >> 47;
+		k *= m;
+		h ^= k;
+		h *= m;
+	}
+	if (len - i) {
+		uint64_t t = 0;
+		memcpy(&t, key + i, len - i);
+		h ^= t;
+		h *= m;
+	}
+	h ^= h >> 47;
+	h *= m;
+	h ^= h >> 47;
+	return h;
+}
+
+/* vec_hash / vec_rehash on varchar: hash(v) = MurmurHash64A(v, 0),
+ * rehash(h, v) = MurmurHash64A(v, h) (runtime.hpp:482-484). */
+void vo_vec_hash_varchar(const sel_t* sel, int64_t inum, uint64_t* res, const uint64_t* h,
+	const uint8_t* col, int max_len)
+{
+	for (int64_t k = 0; k < inum; k++) {
+		const int64_t i = sel ? sel[k] : k;
+		uint32_t len;
+		const uint8_t* s = vo_varchar_at(col, max_len, i, &len);
+		res[i] = vo_murmur64a(s, len, h ? h[i] : 0);
+	}
+}
+
+/* vec_contains (kernel_generator.py:435-437): varchar::contains, memmem
+ * (runtime.hpp:206-208). */
+void vo_vec_contains(const sel_t* sel, int64_t inum, uint8_t* res, const uint8_t* col, int max_len,
+	const uint8_t* needle, int nlen)
+{
+	for (int64_t k = 0; k < inum; k++) {
+		const int64_t i = sel ? sel[k] : k;
+		uint32_t len;
+		const uint8_t* s = vo_varchar_at(col, max_len, i, &len);
+		res[i] = memmem(s, len, needle, (size_t)nlen) != NULL;
+	}
+}
+
+/* vec_eq / vec_ne on varchar (kernel_generator.py:412-433): varchar::operator==
+ * = equal lengths and strncmp(..., len) == 0 (runtime.hpp:154-156). */
+void vo_vec_cmp_varchar(int cmp, const sel_t* sel, int64_t inum, uint8_t* res, const uint8_t* col1,
+	int max_len1, const uint8_t* col2, int max_len2, const uint8_t* cst, int clen)
+{
+	for (int64_t k = 0; k < inum; k++) {
+		const int64_t i = sel ? sel[k] : k;
+		uint32_t la, lb;
+		const uint8_t* a = vo_varchar_at(col1, max_len1, i, &la);
+		const uint8_t* b = cst;
+		lb = (uint32_t)clen;
+		if (col2)
+			b = vo_varchar_at(col2, max_len2, i, &lb);
+		const int eq = la == lb && strncmp((const char*)a, (const char*)b, la) == 0;
+		res[i] = (uint8_t)(cmp == VH_EQ ? eq : !eq);
+	}
 }
 
 /* voila_cast<T>::to_cstr (runtime.hpp:348-387): magnitude digits, then '-'. */

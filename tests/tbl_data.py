@@ -97,3 +97,53 @@ def q6_from_text(path):
             s += p * d
             n += 1
     return s, n
+
+
+COLORS = ["almond", "antique", "aquamarine", "azure", "beige", "bisque", "black", "blanched", "blue", "blush",
+          "brown", "burlywood", "chartreuse", "chiffon", "chocolate", "coral", "forest", "green", "honeydew",
+          "khaki", "lavender", "lime", "navy", "olive", "orchid"]
+
+
+def write_part(path, rows, seed=1):
+    """part.tbl in the reference schema's nine fields: p_name five colour
+    words (as dbgen draws them), p_mfgr / p_brand with TPC-H's patterns."""
+    rng = np.random.default_rng(seed)
+    lines = []
+    for k in range(1, rows + 1):
+        name = " ".join(COLORS[c] for c in rng.choice(len(COLORS), 5, replace=False))
+        m = int(rng.integers(1, 6))
+        fields = [str(k), name, f"Manufacturer#{m}", f"Brand#{m}{int(rng.integers(1, 6))}",
+                  "STANDARD POLISHED TIN", str(int(rng.integers(1, 51))), "SM BOX",
+                  f"{900 + k % 1000}.{k % 100:02d}", "regular deposits"]
+        lines.append("|".join(fields) + "|\n")
+    with open(path, "w") as f:
+        f.writelines(lines)
+
+
+def green_revenue_from_text(lineitem_path, part_path, word="green"):
+    """sum(l_extendedprice * (100 - l_discount)) and count by l_returnflag
+    over lineitems whose part's p_name contains `word`, from the text."""
+    green = {int(f[0]) for f in (line.split("|") for line in open(part_path)) if word in f[1]}
+    groups = {}
+    for line in open(lineitem_path):
+        f = line.split("|")
+        if int(f[1]) not in green:
+            continue
+        p, d = (int(round(float(x) * 100)) for x in (f[5], f[6]))
+        g = groups.setdefault(ord(f[8]), [0, 0])
+        g[0] += p * (100 - d)
+        g[1] += 1
+    return sorted(f"{k}|{s}|{n}" for k, (s, n) in groups.items())
+
+
+def green_revenue(li, part_keys, green, li_bounds, addrs=None, green_addr=None):
+    """The same query as a pipeline: part filtered on a u8 predicate column
+    (p_green == 1, the vec_contains result) joined with lineitem."""
+    pt = table_from_arrays("part", {"p_partkey": part_keys, "p_green": green},
+                           {"p_partkey": addrs["p_partkey"], "p_green": green_addr} if addrs else None,
+                           {"p_green": (0, 1)})
+    lt = table_from_arrays("lineitem", li, addrs["lineitem"] if addrs else None, li_bounds)
+    p = Pipeline(lt)
+    p.join(pt, "p_partkey", "l_partkey", build_preds=[("p_green", "==", 1)])
+    p.groupby(["l_returnflag"], [("sum", col("l_extendedprice") * (100 - col("l_discount"))), ("count",)])
+    return p

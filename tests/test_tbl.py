@@ -71,7 +71,24 @@ def test_load_writes_the_cache_and_reads_it_back(tmp_path):
     for c in Q1_COLS:
         assert np.array_equal(a[c], a2[c])
     with pytest.raises(ValueError):
-        tbl.load_table(str(tmp_path), "lineitem", ["l_comment"])       # varchar: not on the path
+        tbl.load_table(str(tmp_path), "lineitem", ["l_nosuchcolumn"])
+
+
+def test_load_char_and_varchar_columns(tmp_path):
+    """char(n) / varchar(n) columns load as (1 + n)-byte slots, in the cache
+    too, and read back the text's fields (char: leading spaces dropped)."""
+    from vldb_voila_amd import varchar
+    write_lineitem(tmp_path / "lineitem.tbl", 200, seed=4)
+    cols = ["l_shipinstruct", "l_shipmode", "l_comment"]
+    a, b = tbl.load_table(str(tmp_path), "lineitem", cols)
+    assert a["l_shipinstruct"].shape == (200, 26) and a["l_comment"].shape == (200, 45)
+    assert b["l_comment"] is None
+    assert os.path.getsize(tmp_path / "cached" / "lineitem_l_comment") == 200 * 45
+    fields = [line.split("|") for line in open(tmp_path / "lineitem.tbl")]
+    assert varchar.unpack(a["l_shipinstruct"]) == [f[13].lstrip(" ").encode() for f in fields]
+    assert varchar.unpack(a["l_comment"]) == [f[15].encode() for f in fields]
+    a2, _ = tbl.load_table(str(tmp_path), "lineitem", cols)
+    assert all(np.array_equal(a[c], a2[c]) for c in cols)
 
 
 def test_q1_q6_on_imported_columns_match_the_text(tmp_path):

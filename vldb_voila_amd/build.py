@@ -8,7 +8,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoila_hip.so")
-SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_batch.hip", "vh_mlp.hip", "vh_engine.cpp",
+SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_varchar.hip", "vh_batch.hip", "vh_mlp.hip", "vh_engine.cpp",
            "vh_capi.cpp", "vh_capi_prims.cpp", "vh_batch_host.cpp", "vh_capi_tune.cpp", "vh_relalg.cpp",
            "vh_jit.cpp"]
 # Headers the generated kernels include, embedded in the library as text

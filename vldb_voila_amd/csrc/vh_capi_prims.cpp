@@ -11,6 +11,7 @@
 #include "vh_engine.hpp"
 #include "vh_kernels.hpp"
 #include "vh_prims.hpp"
+#include "vh_varchar.hpp"
 
 using namespace vh;
 
@@ -236,6 +237,51 @@ int vh_vec_rehash(int32_t type, const int32_t* sel, int64_t inum, uint64_t* res,
 	if (!valid_type(type) || inum < 0 || !hash)
 		return fail(VH_ERR_INVALID, "vh_vec_rehash arguments");
 	LAUNCH(launch_hash(type, sel, inum, (u64*)res, (const u64*)hash, col, (hipStream_t)stream));
+}
+
+// --- varchar statements --------------------------------------------------------
+
+static bool valid_max_len(int32_t n) { return n >= 1 && n <= kVcMaxLen; }
+
+int vh_vec_hash_varchar(const int32_t* sel, int64_t inum, uint64_t* res, const uint8_t* col, int32_t max_len,
+	void* stream)
+{
+	if (!valid_max_len(max_len) || inum < 0 || (inum && (!col || !res)))
+		return fail(VH_ERR_INVALID, "vh_vec_hash_varchar arguments");
+	LAUNCH(launch_varchar(VC_HASH, sel, inum, res, col, max_len, nullptr, nullptr, 0, nullptr, 0,
+		(hipStream_t)stream));
+}
+
+int vh_vec_rehash_varchar(const int32_t* sel, int64_t inum, uint64_t* res, const uint64_t* hash,
+	const uint8_t* col, int32_t max_len, void* stream)
+{
+	if (!valid_max_len(max_len) || inum < 0 || !hash || (inum && (!col || !res)))
+		return fail(VH_ERR_INVALID, "vh_vec_rehash_varchar arguments");
+	LAUNCH(launch_varchar(VC_HASH, sel, inum, res, col, max_len, (const u64*)hash, nullptr, 0, nullptr, 0,
+		(hipStream_t)stream));
+}
+
+int vh_vec_contains(const int32_t* sel, int64_t inum, uint8_t* res, const uint8_t* col, int32_t max_len,
+	const char* needle, int32_t needle_len, void* stream)
+{
+	if (!valid_max_len(max_len) || inum < 0 || needle_len < 0 || needle_len > kVcMaxLen ||
+		(needle_len && !needle) || (inum && (!col || !res)))
+		return fail(VH_ERR_INVALID, "vh_vec_contains arguments");
+	LAUNCH(launch_varchar(VC_CONTAINS, sel, inum, res, col, max_len, nullptr, nullptr, 0, (const uint8_t*)needle,
+		(unsigned)needle_len, (hipStream_t)stream));
+}
+
+int vh_vec_cmp_varchar(int32_t cmp, const int32_t* sel, int64_t inum, uint8_t* res, const uint8_t* col1,
+	int32_t max_len1, const uint8_t* col2, int32_t max_len2, const char* constant, int32_t constant_len,
+	void* stream)
+{
+	if ((cmp != VH_EQ && cmp != VH_NE) || !valid_max_len(max_len1) || inum < 0 || (inum && (!col1 || !res)))
+		return fail(VH_ERR_INVALID, "vh_vec_cmp_varchar arguments");
+	if (col2 ? !valid_max_len(max_len2)
+	         : (constant_len < 0 || constant_len > kVcMaxLen || (constant_len && !constant)))
+		return fail(VH_ERR_INVALID, "vh_vec_cmp_varchar operand");
+	LAUNCH(launch_varchar(cmp == VH_EQ ? VC_EQ : VC_NE, sel, inum, res, col1, max_len1, nullptr, col2, max_len2,
+		col2 ? nullptr : (const uint8_t*)constant, col2 ? 0u : (unsigned)constant_len, (hipStream_t)stream));
 }
 
 // --- table statements ----------------------------------------------------------

@@ -16,10 +16,17 @@ column types the operator path handles:
   date     Date::castString: YYYY-MM-DD -> Julian day number
            (mergeJulianDay, runtime/Types.cpp:108-117) -> int32
   char1    Char<1>::castString: the first byte -> uint8
+  char(n)  Char<n>::castString (Types.hpp:295-305): leading spaces dropped,
+           at most n bytes -> a (1 + n)-byte slot: length byte, then bytes
+  varchar(n) Varchar<n>::castString (Types.hpp:165-171): the bytes as they
+           are, at most n -> the same slot layout
 
-Longer chars and varchars are not on the path (they need MurmurHash64A keys)
-and are skipped.  Column types follow the reference's TPC-H schema
-(Import.cpp:262-392).
+Char and varchar columns load as uint8 arrays of shape (rows, 1 + n), the
+layout of the reference's binary Char<n> / Varchar<n> columns (a u8 length
+indicator, then `char value[n]`; Types.hpp:125-130, :243-250), with the
+bytes after the length zeroed.  The string statements of the path
+(vldb_voila_amd.varchar) read them on the GPU.  Column types follow the
+reference's TPC-H schema (Import.cpp:262-392).
 """
 from __future__ import annotations
 
@@ -32,34 +39,35 @@ DBL_MAX = sys.float_info.max
 DBL_MIN = sys.float_info.min   # MinMaxInfo starts hi at numeric_limits<double>::min()
 
 NUMPY_TYPE = {"integer": np.int32, "bigint": np.int64, "numeric": np.int64, "date": np.int32,
-              "char1": np.uint8}
+              "char1": np.uint8, "char": np.uint8, "varchar": np.uint8}
+STRING_TYPES = ("char", "varchar")
 
-# (column, type, numeric precision) of the reference's TPC-H tables; None:
-# not loadable here (char(n > 1) / varchar)
+# (column, type, numeric precision or string length) of the reference's
+# TPC-H tables
 TPCH = {
-    "part": [("p_partkey", "integer", 0), ("p_name", None, 0), ("p_mfgr", None, 0), ("p_brand", None, 0),
-             ("p_type", None, 0), ("p_size", "integer", 0), ("p_container", None, 0),
-             ("p_retailprice", "numeric", 2), ("p_comment", None, 0)],
-    "supplier": [("s_suppkey", "integer", 0), ("s_name", None, 0), ("s_address", None, 0),
-                 ("s_nationkey", "integer", 0), ("s_phone", None, 0), ("s_acctbal", "numeric", 2),
-                 ("s_comment", None, 0)],
+    "part": [("p_partkey", "integer", 0), ("p_name", "varchar", 55), ("p_mfgr", "char", 25),
+             ("p_brand", "char", 10), ("p_type", "varchar", 25), ("p_size", "integer", 0),
+             ("p_container", "char", 10), ("p_retailprice", "numeric", 2), ("p_comment", "varchar", 23)],
+    "supplier": [("s_suppkey", "integer", 0), ("s_name", "char", 25), ("s_address", "varchar", 40),
+                 ("s_nationkey", "integer", 0), ("s_phone", "char", 15), ("s_acctbal", "numeric", 2),
+                 ("s_comment", "varchar", 101)],
     "partsupp": [("ps_partkey", "integer", 0), ("ps_suppkey", "integer", 0), ("ps_availqty", "integer", 0),
-                 ("ps_supplycost", "numeric", 2), ("ps_comment", None, 0)],
-    "customer": [("c_custkey", "integer", 0), ("c_name", None, 0), ("c_address", None, 0),
-                 ("c_nationkey", "integer", 0), ("c_phone", None, 0), ("c_acctbal", "numeric", 2),
-                 ("c_mktsegment", None, 0), ("c_comment", None, 0)],
+                 ("ps_supplycost", "numeric", 2), ("ps_comment", "varchar", 199)],
+    "customer": [("c_custkey", "integer", 0), ("c_name", "char", 25), ("c_address", "varchar", 40),
+                 ("c_nationkey", "integer", 0), ("c_phone", "char", 15), ("c_acctbal", "numeric", 2),
+                 ("c_mktsegment", "char", 10), ("c_comment", "varchar", 117)],
     "orders": [("o_orderkey", "integer", 0), ("o_custkey", "integer", 0), ("o_orderstatus", "char1", 0),
-               ("o_totalprice", "numeric", 2), ("o_orderdate", "date", 0), ("o_orderpriority", None, 0),
-               ("o_clerk", None, 0), ("o_shippriority", "integer", 0), ("o_comment", None, 0)],
+               ("o_totalprice", "numeric", 2), ("o_orderdate", "date", 0), ("o_orderpriority", "char", 15),
+               ("o_clerk", "char", 15), ("o_shippriority", "integer", 0), ("o_comment", "varchar", 79)],
     "lineitem": [("l_orderkey", "integer", 0), ("l_partkey", "integer", 0), ("l_suppkey", "integer", 0),
                  ("l_linenumber", "integer", 0), ("l_quantity", "numeric", 2),
                  ("l_extendedprice", "numeric", 2), ("l_discount", "numeric", 2), ("l_tax", "numeric", 2),
                  ("l_returnflag", "char1", 0), ("l_linestatus", "char1", 0), ("l_shipdate", "date", 0),
-                 ("l_commitdate", "date", 0), ("l_receiptdate", "date", 0), ("l_shipinstruct", None, 0),
-                 ("l_shipmode", None, 0), ("l_comment", None, 0)],
-    "nation": [("n_nationkey", "integer", 0), ("n_name", None, 0), ("n_regionkey", "integer", 0),
-               ("n_comment", None, 0)],
-    "region": [("r_regionkey", "integer", 0), ("r_name", None, 0), ("r_comment", None, 0)],
+                 ("l_commitdate", "date", 0), ("l_receiptdate", "date", 0), ("l_shipinstruct", "char", 25),
+                 ("l_shipmode", "char", 10), ("l_comment", "varchar", 44)],
+    "nation": [("n_nationkey", "integer", 0), ("n_name", "char", 25), ("n_regionkey", "integer", 0),
+               ("n_comment", "varchar", 152)],
+    "region": [("r_regionkey", "integer", 0), ("r_name", "char", 25), ("r_comment", "varchar", 152)],
 }
 
 
@@ -157,7 +165,23 @@ def cast_char1(s: str) -> int:
     return b[0] if b else 0
 
 
-def cast(s: str, kind: str, precision: int = 0) -> int:
+def cast_char(s: str, n: int) -> bytes:
+    """Char<n>::castString: leading spaces dropped, at most n bytes."""
+    b = s.encode("latin-1").lstrip(b" ")
+    if len(b) > n:
+        raise FormatError(f"char({n}) value of {len(b)} bytes")
+    return b
+
+
+def cast_varchar(s: str, n: int) -> bytes:
+    """Varchar<n>::castString: the bytes as they are, at most n."""
+    b = s.encode("latin-1")
+    if len(b) > n:
+        raise FormatError(f"varchar({n}) value of {len(b)} bytes")
+    return b
+
+
+def cast(s: str, kind: str, precision: int = 0):
     if kind == "integer":
         return cast_integer(s)
     if kind == "bigint":
@@ -168,6 +192,10 @@ def cast(s: str, kind: str, precision: int = 0) -> int:
         return cast_date(s)
     if kind == "char1":
         return cast_char1(s)
+    if kind == "char":
+        return cast_char(s, precision)
+    if kind == "varchar":
+        return cast_varchar(s, precision)
     raise ValueError(f"column type {kind!r} is not loadable")
 
 
@@ -186,16 +214,26 @@ def minmax(values: np.ndarray):
     return lo, (hi if hi >= 1 else 0)
 
 
+def _slots(values, n: int) -> np.ndarray:
+    """(1 + n)-byte slots: the length byte, then the bytes, zero-padded."""
+    out = np.zeros((len(values), n + 1), dtype=np.uint8)
+    for i, b in enumerate(values):
+        out[i, 0] = len(b)
+        out[i, 1:1 + len(b)] = np.frombuffer(b, dtype=np.uint8)
+    return out
+
+
 def _cache_path(cachedir, table, col):
     return os.path.join(cachedir, f"{table}_{col}")
 
 
 def load_table(directory: str, table: str, columns=None, schema=None):
     """Columns of `<directory>/<table>.tbl` as numpy arrays, with their
-    [min, max].  The first load writes `<directory>/cached/<table>_<col>`
-    (raw values); later loads read those, as the reference's importer does.
-    `columns`: names to load (default: every loadable column of the schema);
-    `schema`: [(name, type, precision)] for tables outside TPC-H."""
+    [min, max] (None for char / varchar columns).  The first load writes
+    `<directory>/cached/<table>_<col>` (raw values; string columns as their
+    slots); later loads read those, as the reference's importer does.
+    `columns`: names to load (default: every column of the schema);
+    `schema`: [(name, type, precision or length)] for tables outside TPC-H."""
     spec = schema or TPCH[table]
     kinds = {name: (kind, prec) for name, kind, prec in spec}
     want = columns or [name for name, kind, _ in spec if kind]
@@ -214,13 +252,22 @@ def load_table(directory: str, table: str, columns=None, schema=None):
                     kind, prec = kinds[c]
                     vals[c].append(cast(fields[idx[c]], kind, prec))
         for c in want:
-            np.asarray(vals[c], dtype=NUMPY_TYPE[kinds[c][0]]).tofile(_cache_path(cachedir, table, c))
+            kind, n = kinds[c]
+            if kind in STRING_TYPES:
+                _slots(vals[c], n).tofile(_cache_path(cachedir, table, c))
+            else:
+                np.asarray(vals[c], dtype=NUMPY_TYPE[kind]).tofile(_cache_path(cachedir, table, c))
     out, bounds = {}, {}
     sizes = set()
     for c in want:
-        a = np.fromfile(_cache_path(cachedir, table, c), dtype=NUMPY_TYPE[kinds[c][0]])
+        kind, n = kinds[c]
+        a = np.fromfile(_cache_path(cachedir, table, c), dtype=NUMPY_TYPE[kind])
+        if kind in STRING_TYPES:
+            a = a.reshape(-1, n + 1)
+            bounds[c] = None
+        else:
+            bounds[c] = minmax(a)
         out[c] = a
-        bounds[c] = minmax(a)
         sizes.add(len(a))
     if len(sizes) > 1:
         raise ValueError(f"columns of {table} differ in size")
