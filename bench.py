@@ -1,6 +1,6 @@
 """Benchmark of the VOILA operator path on one MI355X.
 
-    python bench.py [--workload join|filter|star|batch] [--steps K] [--warmup W]
+    python bench.py [--workload join|filter|star|batch|contains] [--steps K] [--warmup W]
 
 join (default, the headline configuration): build 1,500,000 rows (unique
   int32 keys 1..1.5M permuted, int32 nation U[0,25), int32 segment U[0,5));
@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE + WRITE_SIZE runs scaled by a copy of known size; each file
 # gives the commands and the counter values
 TRAFFIC_FILES = {"join": "traffic_join_1.5Mx6M.md", "filter": "traffic_filter_groupby_6M.md",
-                 "star": "traffic_star_60M.md"}
+                 "star": "traffic_star_60M.md", "contains": "traffic_contains_10M.md"}
 METRIC = "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend"
 
 
@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["join", "filter", "star", "batch"], default="join")
+    ap.add_argument("--workload", choices=["join", "filter", "star", "batch", "contains"], default="join")
     ap.add_argument("--zipf", type=float, default=0.0, help="star: Zipf theta of the part FKs")
     ap.add_argument("--copies", type=int, default=0, help="input copies rotated (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
@@ -137,6 +137,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "contains":
+        return bench_contains(args)
 
     import numpy as np
 
@@ -256,6 +258,67 @@ def cpu_baseline(desc, tuples, args):
              "sample": f"full workload, {len(times)} hot runs of the vector(1024) restatement, "
                        f"min {best * 1e3:.1f} ms, avg {sum(times) / len(times) * 1e3:.1f} ms"},
             sorted(text.splitlines()))
+
+
+def bench_contains(args):
+    """vec_contains over a p_name-like varchar(55) column: 10M rows of
+    56-byte slots, needle 'green' (TPC-H Q9's predicate); two copies rotate
+    so the 560 MB column is read from HBM.  One kernel per step."""
+    import numpy as np
+
+    from vldb_voila_amd import datagen, runtime, varchar
+
+    rows, max_len, needle = 10_000_000, 55, b"green"
+    col = datagen.gen_part_names(rows, seed=42)
+    runtime.init(0)
+    stream = runtime.Stream()
+    copies = [runtime.DeviceBuffer.from_numpy(col) for _ in range(2)]
+    out = runtime.DeviceBuffer(rows)
+    for i in range(max(args.warmup, 1)):
+        varchar.contains(copies[i % 2].ptr, rows, max_len, needle, out=out, stream=stream)
+    stream.synchronize()
+    gpu = out.to_numpy(np.uint8, rows)
+    K = args.steps
+    t0, t1 = runtime.Event(), runtime.Event()
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for i in range(K):
+        varchar.contains(copies[i % 2].ptr, rows, max_len, needle, out=out, stream=stream)
+    t1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_ms(t1) / K
+    alg = rows * (max_len + 1) + rows          # every slot read once, one result byte written
+    achieved = alg / (ms / 1e3) / 1e9
+    line = {
+        "metric": METRIC, "value": rows / (ms / 1e3), "unit": "input tuples/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic p_name-like strings (seed 42)",
+        "config": {"workload": "vec_contains(p_name, 'green'): 10M rows of varchar(55) slots", "rows": rows},
+        "roofline": {"bound": "hbm", "kernel": "vh::varchar_kernel (contains)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("contains", 0.0), "avg_kernel_ms": ms, "algorithmic_bytes": alg},
+        "wall_s": wall,
+    }
+    if not args.no_cpu:
+        import oracle
+        sample = 2_000_000
+        want = np.zeros(sample, dtype=np.uint8)
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 5 and (len(times) < 2 or time.perf_counter() - t_start < args.cpu_seconds):
+            c0 = time.perf_counter()
+            oracle.lib().vo_vec_contains(None, sample, want.ctypes.data, col.ctypes.data, max_len, needle,
+                                         len(needle))
+            times.append(time.perf_counter() - c0)
+        best = min(times)
+        line["cpu_baseline"] = {"value": sample / best, "unit": "input tuples/s", "cores": 1, "kind": "port",
+                                "sample": f"first 2M rows, memmem per row (varchar::contains), {len(times)} "
+                                          f"runs, min {best * 1e3:.1f} ms"}
+        if not np.array_equal(gpu[:sample], want):
+            print(json.dumps({"error": "GPU contains differs from the CPU oracle"}), flush=True)
+            sys.exit(1)
+    print(json.dumps(line), flush=True)
 
 
 def bench_batch(args):

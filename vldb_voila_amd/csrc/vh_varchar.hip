@@ -98,27 +98,141 @@ __device__ __forceinline__ bool vc_equal(const uint8_t* a, u32 alen, const uint8
 	return true;
 }
 
-// Copy slots [r0, r0 + cnt) of a dense column of `total` bytes into LDS
-// (dword loads where the whole dword lies in the column, bytes at its two
-// ends); returns the LDS offset of row r0's slot.
+// ---- the same three operations on a row staged in LDS, four bytes at a time
+
+// Bytes i .. i + 3 of the LDS tile (any alignment): two aligned dwords and a
+// byte funnel shift.
+__device__ __forceinline__ u32 lds4(const u32* w, u32 i)
+{
+	return __builtin_amdgcn_alignbyte(w[(i >> 2) + 1], w[i >> 2], i & 3);
+}
+
+__device__ __forceinline__ u32 low_bytes(u32 n)   // mask of the low n bytes, n <= 4
+{
+	return n >= 4 ? 0xFFFFFFFFu : (1u << (8 * n)) - 1u;
+}
+
+// 0x80 in every byte of x that is zero (exact: no carry between bytes)
+__device__ __forceinline__ u32 zero_bytes(u32 x)
+{
+	return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+}
+
+__device__ __forceinline__ u64 murmur64a_lds(const u32* w, u32 i, u32 len, u64 seed)
+{
+	const u64 m = 0xc6a4a7935bd1e995ull;
+	const int r = 47;
+	u64 h = seed ^ ((u64)len * m);
+	const u32 nb = len >> 3;
+	for (u32 b = 0; b < nb; b++) {
+		u64 k = (u64)lds4(w, i + 8 * b) | ((u64)lds4(w, i + 8 * b + 4) << 32);
+		k *= m;
+		k ^= k >> r;
+		k *= m;
+		h ^= k;
+		h *= m;
+	}
+	const u32 rem = len & 7;
+	if (rem) {
+		const u32 t = i + 8 * nb;
+		u64 tail = lds4(w, t) & low_bytes(rem);
+		if (rem > 4)
+			tail |= (u64)(lds4(w, t + 4) & low_bytes(rem - 4)) << 32;
+		h ^= tail;
+		h *= m;
+	}
+	h ^= h >> r;
+	h *= m;
+	h ^= h >> r;
+	return h;
+}
+
+// memmem: candidate starts are the bytes equal to the needle's first byte
+// (found four at a time), each then compared four bytes at a time.
+__device__ __forceinline__ bool contains_lds(const u32* w, u32 i, u32 len, const u32* nw, u32 nlen)
+{
+	if (nlen > len)
+		return false;
+	if (nlen == 0)
+		return true;
+	const u32 last = len - nlen;                        // the last start
+	const u32 first = (lds4(nw, 0) & 0xFFu) * 0x01010101u;
+	for (u32 s0 = 0; s0 <= last; s0 += 8) {
+		const u32 w0 = lds4(w, i + s0);
+		const u32 w1 = lds4(w, i + s0 + 4);
+		u64 t = (u64)zero_bytes(w0 ^ first) | ((u64)zero_bytes(w1 ^ first) << 32);
+		if (last - s0 < 7)
+			t &= (1ull << (8 * (last - s0 + 1))) - 1ull;   // starts s0 .. last only
+		while (t) {
+			const u32 s = s0 + ((u32)__builtin_ctzll(t) >> 3);
+			bool ok = true;
+			for (u32 j = 1; j < nlen && ok; j += 4)
+				ok = ((lds4(w, i + s + j) ^ lds4(nw, j)) & low_bytes(nlen - j)) == 0;
+			if (ok)
+				return true;
+			t &= t - 1;
+		}
+	}
+	return false;
+}
+
+// varchar::operator== against the constant: equal lengths, then bytes up to
+// the first difference or a NUL both share.
+__device__ __forceinline__ bool equal_lds(const u32* w, u32 i, u32 len, const u32* cw, u32 clen)
+{
+	if (len != clen)
+		return false;
+	for (u32 j = 0; j < len; j += 4) {
+		const u32 a = lds4(w, i + j);
+		const u32 hb = low_bytes(len - j) & 0x80808080u;
+		const u32 ne = ~zero_bytes(a ^ lds4(cw, j)) & hb;   // bytes that differ
+		const u32 nul = zero_bytes(a) & hb;                  // NUL bytes of the row
+		if (ne | nul) {
+			const u32 fne = ne ? (u32)__builtin_ctz(ne) : 32u;
+			const u32 fnul = nul ? (u32)__builtin_ctz(nul) : 32u;
+			return fnul < fne;   // a NUL both share, before any difference: equal
+		}
+	}
+	return true;
+}
+
+// Copy slots [r0, r0 + cnt) of a dense column of `total` bytes into LDS in
+// 16-byte chunks, each thread's (up to) eight loads in flight together; a
+// chunk that reaches past either end of the column is read byte by byte.
+// Returns the LDS offset of row r0's slot.
 __device__ __forceinline__ u32 stage_slots(const uint8_t* col, u64 total, i64 r0, int cnt, int slot, u32* lds)
 {
 	const uintptr_t base = (uintptr_t)col;
 	const uintptr_t b0 = base + (u64)r0 * (u64)slot;
 	const uintptr_t b1 = b0 + (u64)cnt * (u64)slot;
-	const uintptr_t a0 = b0 & ~(uintptr_t)3;
-	const u32 words = (u32)(((b1 + 3) & ~(uintptr_t)3) - a0) / 4;
-	for (u32 w = threadIdx.x; w < words; w += blockDim.x) {
-		const uintptr_t a = a0 + 4 * (uintptr_t)w;
-		u32 v = 0;
-		if (a >= base && a + 4 <= base + total) {
-			v = *(const u32*)a;
-		} else {
-			for (int b = 0; b < 4; b++)
-				if (a + b >= base && a + b < base + total)
-					v |= (u32) * (const uint8_t*)(a + b) << (8 * b);
+	const uintptr_t a0 = b0 & ~(uintptr_t)15;
+	const u32 chunks = (u32)((((b1 + 15) & ~(uintptr_t)15) - a0) / 16);
+	uint4* dst = (uint4*)lds;
+	for (u32 c0 = 0; c0 < chunks; c0 += 8 * kVcThreads) {
+		uint4 v[8];
+#pragma unroll
+		for (int u = 0; u < 8; u++) {
+			const u32 c = c0 + u * kVcThreads + threadIdx.x;
+			const uintptr_t a = a0 + 16 * (uintptr_t)c;
+			v[u] = make_uint4(0u, 0u, 0u, 0u);
+			if (c < chunks && a >= base && a + 16 <= base + total)
+				v[u] = *(const uint4*)a;
 		}
-		lds[w] = v;
+#pragma unroll
+		for (int u = 0; u < 8; u++) {
+			const u32 c = c0 + u * kVcThreads + threadIdx.x;
+			if (c >= chunks)
+				continue;
+			const uintptr_t a = a0 + 16 * (uintptr_t)c;
+			if (!(a >= base && a + 16 <= base + total)) {
+				u32 wv[4] = {0u, 0u, 0u, 0u};
+				for (int b = 0; b < 16; b++)
+					if (a + b >= base && a + b < base + total)
+						wv[b >> 2] |= (u32) * (const uint8_t*)(a + b) << (8 * (b & 3));
+				v[u] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+			}
+			dst[c] = v[u];
+		}
 	}
 	__syncthreads();
 	return (u32)(b0 - a0);
@@ -126,10 +240,12 @@ __device__ __forceinline__ u32 stage_slots(const uint8_t* col, u64 total, i64 r0
 
 __global__ __launch_bounds__(kVcThreads) void varchar_kernel(VcArgs A)
 {
-	extern __shared__ u32 vc_lds[];
-	__shared__ uint8_t cst[kVcMaxConst];
-	for (u32 j = threadIdx.x; j < A.clen; j += blockDim.x)
-		cst[j] = A.cbytes[j];
+	extern __shared__ uint4 vc_lds16[];        // 16-byte aligned: stage_slots stores uint4
+	u32* vc_lds = (u32*)vc_lds16;
+	__shared__ u32 cst_words[kVcMaxConst / 4 + 2];    // + 2: lds4 reads a dword past the last byte
+	uint8_t* cst = (uint8_t*)cst_words;
+	for (u32 j = threadIdx.x; j < sizeof(cst_words); j += blockDim.x)
+		cst[j] = j < A.clen ? A.cbytes[j] : 0;
 	__syncthreads();
 	const int R = A.tile_rows;
 	const u32 maxlen = (u32)A.slot - 1;
@@ -146,7 +262,15 @@ __global__ __launch_bounds__(kVcThreads) void varchar_kernel(VcArgs A)
 			// the length byte, bounded by the slot (well-formed columns never exceed it)
 			const u32 len = p[0] < maxlen ? p[0] : maxlen;
 			const uint8_t* s = p + 1;
-			if (A.op == VC_HASH) {
+			const u32 li = off + (u32)threadIdx.x * (u32)A.slot + 1;   // the row's bytes in the tile
+			if (!A.sel && A.op == VC_HASH) {
+				((u64*)A.res)[i] = murmur64a_lds(vc_lds, li, len, A.h ? A.h[i] : 0ull);
+			} else if (!A.sel && A.op == VC_CONTAINS) {
+				((uint8_t*)A.res)[i] = contains_lds(vc_lds, li, len, cst_words, A.clen) ? 1 : 0;
+			} else if (!A.sel && !A.col2) {
+				const bool eq = equal_lds(vc_lds, li, len, cst_words, A.clen);
+				((uint8_t*)A.res)[i] = (eq == (A.op == VC_EQ)) ? 1 : 0;
+			} else if (A.op == VC_HASH) {
 				((u64*)A.res)[i] = murmur64a(s, len, A.h ? A.h[i] : 0ull);
 			} else if (A.op == VC_CONTAINS) {
 				((uint8_t*)A.res)[i] = vc_contains(s, len, cst, A.clen) ? 1 : 0;
@@ -188,8 +312,9 @@ hipError_t launch_varchar(int op, const int32_t* sel, i64 inum, void* res, const
 	i64 grid = (inum + A.tile_rows - 1) / A.tile_rows;
 	if (grid > 8192)
 		grid = 8192;
-	// staging: the tile's slots plus up to 3 bytes either side of dword alignment
-	const size_t lds = sel ? 0 : ((size_t)A.tile_rows * A.slot + 8 + 3) / 4 * 4;
+	// staging: the tile's slots, up to 15 bytes either side of 16-byte
+	// alignment, and the dword past the last byte that lds4 reads
+	const size_t lds = sel ? 0 : ((size_t)A.tile_rows * A.slot + 48 + 15) / 16 * 16;
 	hipLaunchKernelGGL(varchar_kernel, dim3((unsigned)grid), dim3(kVcThreads), lds, s, A);
 	return hipGetLastError();
 }

@@ -192,3 +192,25 @@ def gen_batch_pipeline(params: dict, seed=42, index=0):
              "g": uniform_int(s, 55, n, 0, params["groups"], np.int32),
              "v": uniform_int(s, 56, n, -10**9, 10**12, np.int64)}
     return build, probe
+
+
+# The colour words dbgen draws p_name from (TPC-H specification 4.2.3).
+P_NAME_WORDS = (
+    "almond antique aquamarine azure beige bisque black blanched blue blush brown burlywood burnished "
+    "chartreuse chiffon chocolate coral cornflower cornsilk cream cyan dark deep dim dodger drab firebrick "
+    "floral forest frosted gainsboro ghost goldenrod green grey honeydew hot indian ivory khaki lace "
+    "lavender lawn lemon light lime linen magenta maroon medium metallic midnight mint misty moccasin "
+    "navajo navy olive orange orchid pale papaya peach peru pink plum powder puff purple red rose rosy "
+    "royal saddle salmon sandy seashell sienna sky slate smoke snow spring steel tan thistle tomato "
+    "turquoise violet wheat white yellow").split()
+
+
+def gen_part_names(rows: int, seed=42, distinct=1 << 16) -> np.ndarray:
+    """p_name-like varchar(55) column, (rows, 56) uint8 slots: `distinct`
+    names of five different colour words, each row one of them uniformly."""
+    from .varchar import pack
+    rng = np.random.default_rng(seed)
+    names = [" ".join(P_NAME_WORDS[c] for c in rng.choice(len(P_NAME_WORDS), 5, replace=False))
+             for _ in range(distinct)]
+    base = pack(names, 55)
+    return base[(rand_u64(seed, 41, rows) % np.uint64(distinct)).astype(np.int64)]
