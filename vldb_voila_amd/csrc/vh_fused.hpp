@@ -421,25 +421,33 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 		off[p] = L.keys[p];
 	__syncthreads();
 	const u64 W = (u64)P.nsums + 1;
-	constexpr int V = 4;   // tuples per thread in flight
+	const int ns = P.nsums;
+	// every word of a thread's V tuples is loaded before any is stored, so
+	// the copy costs one memory round trip per V tuples
+	constexpr int V = 4;
 	for (u32 i0 = 0; i0 < n; i0 += V * blockDim.x) {
 		u64 g[V];
+		u64 x[V][VH_MAX_AGGRS];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			const u32 i = i0 + v * blockDim.x + threadIdx.x;
-			g[v] = i < n ? P.gtup0[(u64)(L.rbase + i) * W] : ~0ull;
+			const VH_GLOBAL u64* src = (const VH_GLOBAL u64*)P.gtup0 + (u64)(L.rbase + (i < n ? i : 0u)) * W;
+			g[v] = i < n ? src[0] : ~0ull;
+#pragma unroll
+			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
+				x[v][s2] = (i < n && s2 < ns) ? src[1 + s2] : 0ull;
 		}
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			if (g[v] == ~0ull)
 				continue;
-			const u32 i = i0 + v * blockDim.x + threadIdx.x;
-			const u64* src = P.gtup0 + (u64)(L.rbase + i) * W;
 			const u32 pos = atomicAdd(&L.keys[g[v] >> P.gshift], 1u);
-			u64* dst = P.gtup1 + (u64)(L.rbase + pos) * W;
+			VH_GLOBAL u64* dst = (VH_GLOBAL u64*)P.gtup1 + (u64)(L.rbase + pos) * W;
 			dst[0] = g[v];
-			for (u64 w = 1; w < W; w++)
-				dst[w] = src[w];
+#pragma unroll
+			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
+				if (s2 < ns)
+					dst[1 + s2] = x[v][s2];
 		}
 	}
 	__syncthreads();
@@ -477,38 +485,43 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 		base[r] = (u32)((i64)r * P.gregion_rows) + base[r] - pre[r];
 	__syncthreads();
 	const u64 W = (u64)ns + 1;
-	constexpr int V = 8;   // tuples per thread in flight
+	// every word of a thread's V tuples is loaded before the LDS adds
+	constexpr int V = 4;
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
-		const u64* t[V];
 		u64 g[V];
+		u64 x[V][VH_MAX_AGGRS];
+		bool has[V];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			const u32 i = i0 + v * blockDim.x + threadIdx.x;
-			t[v] = nullptr;
-			g[v] = 0;
-			if (i < total) {
-				int a = 0, b = R - 1;
-				while (a < b) {
-					const int mid = (a + b + 1) >> 1;
-					if (pre[mid] <= i)
-						a = mid;
-					else
-						b = mid - 1;
-				}
-				t[v] = P.gtup1 + (u64)(base[a] + i) * W;
-				g[v] = t[v][0];
+			has[v] = i < total;
+			int a = 0, b = R - 1;
+			while (a < b) {
+				const int mid = (a + b + 1) >> 1;
+				if (pre[mid] <= i)
+					a = mid;
+				else
+					b = mid - 1;
 			}
+			const VH_GLOBAL u64* t = (const VH_GLOBAL u64*)P.gtup1 + (u64)(has[v] ? base[a] + i : 0u) * W;
+			g[v] = has[v] ? t[0] : 0ull;
+#pragma unroll
+			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
+				x[v][s2] = (has[v] && s2 < ns) ? t[1 + s2] : 0ull;
 		}
 #pragma unroll
 		for (int v = 0; v < V; v++) {
-			if (!t[v])
+			if (!has[v])
 				continue;
 			const u64 li = g[v] - g0;
 			atomicAdd(&cnt[li], 1u);
-			for (int s2 = 0; s2 < ns; s2++) {
-				const SumArg& A = P.sums[s2];
-				add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], t[v][1 + s2], P.wide && A.wide,
-					A.arg_signed);
+#pragma unroll
+			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
+				if (s2 < ns) {
+					const SumArg& A = P.sums[s2];
+					add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], x[v][s2], P.wide && A.wide,
+						A.arg_signed);
+				}
 			}
 		}
 	}
@@ -518,15 +531,27 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 		if (!c)
 			continue;
 		const u64 g = g0 + li;
-		P.out_cnt[g] += c;
-		for (int s2 = 0; s2 < ns; s2++) {
-			const SumArg& A = P.sums[s2];
-			const u64 l = lo[(u64)s2 * S + li];
-			const i64 h = (P.wide && A.wide) ? (i64)hi[(u64)s2 * S + li] : (A.arg_signed ? (i64)l >> 63 : 0);
-			const u64 idx = (u64)s2 * P.domain + g;
-			const u64 o = P.out_lo[idx];
-			P.out_lo[idx] = o + l;
-			P.out_hi[idx] += h + ((o + l) < o ? 1 : 0);
+		// this workgroup is the only writer of group g here; the global words
+		// may hold fan-out rows added during the probe, so read them all first
+		u64 olo[VH_MAX_AGGRS];
+		i64 ohi[VH_MAX_AGGRS];
+		const u64 oc = P.out_cnt[g];
+#pragma unroll
+		for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
+			olo[s2] = s2 < ns ? P.out_lo[(u64)s2 * P.domain + g] : 0ull;
+			ohi[s2] = s2 < ns ? P.out_hi[(u64)s2 * P.domain + g] : 0;
+		}
+		P.out_cnt[g] = oc + c;
+#pragma unroll
+		for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
+			if (s2 < ns) {
+				const SumArg& A = P.sums[s2];
+				const u64 l = lo[(u64)s2 * S + li];
+				const i64 h = (P.wide && A.wide) ? (i64)hi[(u64)s2 * S + li] : (A.arg_signed ? (i64)l >> 63 : 0);
+				const u64 idx = (u64)s2 * P.domain + g;
+				P.out_lo[idx] = olo[s2] + l;
+				P.out_hi[idx] = ohi[s2] + h + ((olo[s2] + l) < olo[s2] ? 1 : 0);
+			}
 		}
 	}
 }
