@@ -778,8 +778,34 @@ int query_create(Query* q, bool single)
 	// whole direct table and clears its own repeated-key flag
 	// (part_scatter_kernel), so only its overflow slots are cleared here, and
 	// only after a run that used them; other joins clear their flag here.
+	// two or more partitioned builds: their chunks and partitions become the
+	// work items of one scatter and one slice launch
+	int npart = 0;
+	for (int j = 0; j < P.njoins; j++)
+		npart += L.builds[j].nparts > 0;
+	if (single && npart >= 2) {
+		if ((rc = q->alloc((void**)&q->dctl, 2 * sizeof(u32))))
+			return rc;
+		std::vector<BatchItem> items;
+		for (int j = 0; j < P.njoins; j++)
+			for (int c = 0; c < L.builds[j].nchunks && L.builds[j].nparts; c++)
+				items.push_back({0, 0, j, 0, c, 0});
+		q->nscatter = (u32)items.size();
+		for (int j = 0; j < P.njoins; j++) {
+			for (int p = 0; p < L.builds[j].nparts; p++)
+				items.push_back({0, 0, j, 0, p, 0});
+			if (L.builds[j].nparts)
+				q->slice_lds = std::max(q->slice_lds, part_slice_lds_bytes(L.builds[j]));
+		}
+		q->nslice = (u32)items.size() - q->nscatter;
+		if ((rc = q->alloc((void**)&q->dparts, items.size() * sizeof(BatchItem))))
+			return rc;
+		HIP_TRY(hipMemcpy(q->dparts, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+	}
 	std::vector<FillSeg> segs;
 	segs.push_back({(u64*)q->flags, 1, 0, nullptr});
+	if (q->dctl)
+		segs.push_back({(u64*)q->dctl, 1, 0, nullptr});
 	for (int j = 0; j < P.njoins; j++) {
 		if (L.builds[j].nparts) {
 			segs.push_back({q->slots[j], L.capacity[j], ~0ull, dups_flag(q, j)});
@@ -820,10 +846,12 @@ int query_create(Query* q, bool single)
 	if ((rc = q->alloc((void**)&q->dplan, sizeof(Plan))))
 		return rc;
 	HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
-	for (int j = 0; j < P.njoins; j++) {
-		if ((rc = q->alloc((void**)&q->dbuild[j], sizeof(BuildPlan))))
+	if (P.njoins) {
+		if ((rc = q->alloc((void**)&q->dbuilds, P.njoins * sizeof(BuildPlan))))
 			return rc;
-		HIP_TRY(hipMemcpy(q->dbuild[j], &L.builds[j], sizeof(BuildPlan), hipMemcpyHostToDevice));
+		HIP_TRY(hipMemcpy(q->dbuilds, L.builds.data(), P.njoins * sizeof(BuildPlan), hipMemcpyHostToDevice));
+		for (int j = 0; j < P.njoins; j++)
+			q->dbuild[j] = q->dbuilds + j;
 	}
 	// the generated probe kernel, unless the caller asked for the generic one
 	const int kernel = q->cfg.kernel;
@@ -852,10 +880,18 @@ int query_run(Query* q, hipStream_t s, int phase)
 	if (phase == -1 || phase == 0)
 		HIP_TRY(launch_multi_fill(q->dclear, q->nclear, s));
 	if (phase == -1 || phase == 1) {
+		if (q->dparts) {
+			// a persistent workgroup per item slot: 48 KiB of LDS, three per CU
+			const int cap = 3 * device_cus();
+			HIP_TRY(launch_batch_part(false, q->dbuilds, q->dparts, 0, q->nscatter, q->dctl,
+				(int)std::max<u32>(1, std::min<u32>(q->nscatter, (u32)cap)), 0, s));
+			HIP_TRY(launch_batch_part(true, q->dbuilds, q->dparts, q->nscatter, q->nslice, q->dctl + 1,
+				(int)std::max<u32>(1, std::min<u32>(q->nslice, (u32)cap)), q->slice_lds, s));
+		}
 		for (int j = 0; j < P.njoins; j++) {
-			if (L.builds[j].nparts)
+			if (L.builds[j].nparts && !q->dparts)
 				HIP_TRY(launch_part_build(L.builds[j], q->dbuild[j], s));
-			else if (L.builds[j].rows > 0)
+			else if (!L.builds[j].nparts && L.builds[j].rows > 0)
 				HIP_TRY(launch_build(q->dbuild[j], L.build_grid[j], s));
 		}
 		for (int j = 0; j < P.njoins; j++)

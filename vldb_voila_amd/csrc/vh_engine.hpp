@@ -51,7 +51,14 @@ struct Query {
 	struct FillSeg* dclear = nullptr;   // segments of the per-run clear pass
 	int nclear = 0;
 	Plan* dplan = nullptr;
-	BuildPlan* dbuild[VH_MAX_JOINS] = {};
+	BuildPlan* dbuild[VH_MAX_JOINS] = {};   // entries of dbuilds
+	BuildPlan* dbuilds = nullptr;
+	// two or more partitioned builds run as the work items of two launches
+	// (scatter chunks, then slices), as a batch runs them
+	struct BatchItem* dparts = nullptr;
+	u32* dctl = nullptr;                     // the two launches' queue counters
+	u32 nscatter = 0, nslice = 0;
+	size_t slice_lds = 0;
 	std::vector<void*> allocs;
 	bool ran = false;
 	std::string result;
