@@ -608,7 +608,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	if (merge < 0 || merge > 2)
 		return fail(VH_ERR_INVALID, "merge %d", merge);
 	if (P.mode == GB_REG || P.mode == GB_LDS)
-		P.flush = merge == 2 || (merge == 0 && (double)grid * domain * (P.nsums + 1) <= 8192.0);
+		P.flush = merge == 2 || (merge == 0 && (double)grid * domain * (P.nsums + 1) <= 16384.0);
 	L.build_grid.assign(d->njoins, 0);
 	for (int j = 0; j < d->njoins; j++)
 		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 1023) / 1024, 4096));
