@@ -402,3 +402,87 @@ def test_star_schema(theta):
     for kernel in KERNELS:
         got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(kernel=kernel)))
         assert got == want, kernel
+
+
+def voila_hash_np(x: np.ndarray) -> np.ndarray:
+    """voila_hash over uint64 (runtime.hpp:413-424), wrapping arithmetic."""
+    x = x.astype(np.uint64)
+    m = np.uint64(0x45d9f3b)
+    x = x ^ (x >> np.uint64(16))
+    x = x * m
+    x = x ^ (x >> np.uint64(16))
+    x = x * m
+    return x ^ (x >> np.uint64(16))
+
+
+def colliding_offsets(count, bits=16):
+    """Key offsets o (0 first) whose voila_hash(o) has `bits` low zero bits:
+    they share one slot of any table of up to 2^bits slots."""
+    out = [0]
+    start = 1
+    while len(out) < count:
+        o = np.arange(start, start + (1 << 22), dtype=np.uint64)
+        out.extend(o[(voila_hash_np(o) & np.uint64((1 << bits) - 1)) == 0].tolist())
+        start += 1 << 22
+    return np.array(out[:count], dtype=np.int64)
+
+
+def test_colliding_hash_keys_near_int64_max():
+    """400 build keys near 2^62 whose table offsets all hash to one slot:
+    every probe walks one long cluster (hits, and colliding misses that walk
+    it to the end); sums of values near +-2^61 need 128 bits.  A key range
+    wider than 32 bits is refused."""
+    print("colliding hash keys", flush=True)
+    rng = np.random.default_rng(77)
+    offs = colliding_offsets(700)
+    base = np.int64(2**62)
+    build = {"k": base + offs[:400], "p": rng.integers(0, 40, 400).astype(np.int64)}
+    n = 50_000
+    pick = rng.integers(0, 3, n)
+    fk = np.where(pick == 0, base + offs[rng.integers(0, 400, n)],
+                  np.where(pick == 1, base + offs[400 + rng.integers(0, 300, n)],
+                           base + rng.integers(0, int(offs.max()), n)))
+    probe = {"fk": fk.astype(np.int64),
+             "v": rng.integers(-(2**61), 2**61, n).astype(np.int64)}
+    for variant in (abi.VH_JOINN, abi.VH_JOIN01):
+        def make(bb, pp):
+            p = Pipeline(table_from_arrays("s", probe, pp))
+            p.join(table_from_arrays("r", build, bb), "k", "fk", payload=["p"], variant=variant, names=["p"])
+            p.groupby(["p"], [("sum", col("v")), ("count",)])
+            return p
+        want = oracle.sorted_rows(oracle.bruteforce(make(None, None).desc()))
+        assert len(want) == 40
+        p, devs = on_gpu([build, probe], make)
+        for kernel in KERNELS:
+            for layout in (abi.VH_LAYOUT_AUTO, abi.VH_LAYOUT_HASH):
+                got = oracle.sorted_rows(runtime.run_pipeline(
+                    p, runtime.make_config(kernel=kernel, join_layout=layout)))
+                assert got == want, (variant, kernel, layout)
+    wide = {"k": np.array([0, 2**40], dtype=np.int64), "p": np.array([1, 2], dtype=np.int64)}
+    p = Pipeline(table_from_arrays("s", probe))
+    p.join(table_from_arrays("r", wide), "k", "fk", payload=["p"], names=["p"])
+    p.groupby(["p"], [("count",)])
+    with pytest.raises(runtime.VoilaHipError):
+        runtime.Query(p.desc())
+
+
+def test_colliding_group_keys():
+    """600 group keys whose voila_hash shares its low 16 bits (so they share
+    one slot of the global hash table's first probe, and of the LDS hash
+    table's), 30 K rows: every hash group-by strategy gives the oracle's rows."""
+    print("colliding group keys", flush=True)
+    rng = np.random.default_rng(78)
+    keys = colliding_offsets(600)
+    t = {"g": keys[rng.integers(0, 600, 30_000)], "v": rng.integers(-10**12, 10**12, 30_000)}
+
+    def make(addrs):
+        p = Pipeline(table_from_arrays("t", t, addrs))
+        p.groupby(["g"], [("sum", col("v")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
+    assert len(want) == 600
+    p, devs = on_gpu([t], make)
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
+        for kernel in (abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT):
+            got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode, kernel=kernel)))
+            assert got == want, (mode, kernel)

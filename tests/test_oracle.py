@@ -337,3 +337,41 @@ def test_oracle_detects_a_dropped_term():
 
 def test_desc_layout_matches_c():
     assert oracle.lib().vo_sizeof_pipeline_desc() == C.sizeof(abi.vh_pipeline_desc)
+
+
+def test_vector_flavor_with_colliding_hash_keys():
+    """Build keys whose voila_hash shares its low 16 bits: every key lands in
+    one bucket chain of the vector flavor's table (and the insert-conflict
+    rule retries within it); its rows equal the brute force's."""
+    m64 = np.uint64(0x45d9f3b)
+    keys = [0]
+    start = 1
+    while len(keys) < 600:
+        o = np.arange(start, start + (1 << 22), dtype=np.uint64)
+        h = o ^ (o >> np.uint64(16))
+        h = h * m64
+        h = h ^ (h >> np.uint64(16))
+        h = h * m64
+        h = h ^ (h >> np.uint64(16))
+        keys.extend(o[(h & np.uint64(0xFFFF)) == 0].tolist())
+        start += 1 << 22
+    keys = np.array(keys[:600], dtype=np.int64)
+    assert all(oracle.lib().vo_hash(int(k)) & 0xFFFF == 0 for k in keys[:20])
+    rng = np.random.default_rng(9)
+    build = {"k": keys[:350], "p": rng.integers(0, 12, 350).astype(np.int32)}
+    probe = {"fk": keys[rng.integers(0, 600, 20_000)], "v": rng.integers(-10**12, 10**12, 20_000)}
+    for variant in (abi.VH_JOINN, abi.VH_JOIN01):
+        p = Pipeline(table_from_arrays("s", probe))
+        p.join(table_from_arrays("r", build), "k", "fk", payload=["p"], variant=variant, names=["p"])
+        p.groupby(["p"], [("sum", col("v")), ("count",)])
+        want = oracle.sorted_rows(oracle.bruteforce(p.desc()))
+        for threads in (1, 4):
+            got = oracle.sorted_rows(oracle.run_pipeline(p.desc(), threads=threads)[0])
+            assert got == want, (variant, threads)
+    # groups keyed by colliding values too: every group in one chain
+    t = {"g": keys[rng.integers(0, 600, 30_000)], "v": rng.integers(0, 1000, 30_000)}
+    p = Pipeline(table_from_arrays("t", t))
+    p.groupby(["g"], [("sum", col("v")), ("count",)])
+    want = oracle.sorted_rows(oracle.bruteforce(p.desc()))
+    assert len(want) == 600
+    assert oracle.sorted_rows(oracle.run_pipeline(p.desc(), threads=4)[0]) == want
