@@ -486,3 +486,25 @@ def test_colliding_group_keys():
         for kernel in (abi.VH_KERNEL_GENERIC, abi.VH_KERNEL_JIT):
             got = oracle.sorted_rows(runtime.run_pipeline(p, runtime.make_config(group_mode=mode, kernel=kernel)))
             assert got == want, (mode, kernel)
+
+
+def test_more_than_2_31_rows():
+    """2.3 G rows of a u8 column (row indices past 2^31): select x <= 100,
+    sum(x) and count(*), against the closed form of the repeating pattern
+    x = i mod 251."""
+    print("2.3G rows", flush=True)
+    n, period = 2_300_000_000, 251
+    pattern = (np.arange(period) % 256).astype(np.uint8)
+    x = np.tile(pattern, n // period + 1)[:n]
+    full, rest = divmod(n, period)
+    want_cnt = full * 101 + sum(1 for v in range(rest) if v <= 100)
+    want_sum = full * sum(range(101)) + sum(v for v in range(rest) if v <= 100)
+    dev = runtime.DeviceBuffer.from_numpy(x)
+    t = table_from_arrays("t", {"x": x}, {"x": dev.ptr}, {"x": (0, 250)})
+    del x
+    p = Pipeline(t)
+    p.select("x", "<=", 100)
+    p.groupby([], [("sum", col("x")), ("count",)])
+    for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_GENERIC):
+        got = runtime.run_pipeline(p, runtime.make_config(kernel=kernel)).strip()
+        assert got == f"{want_sum}|{want_cnt}", (kernel, got)
