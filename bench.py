@@ -198,7 +198,8 @@ def main():
     stream.synchronize()
     for q in queries:
         q.probe_events(None, None)
-    probe_avg_ms = float(np.mean([a.elapsed_ms(b) for a, b in ev]))
+    probe_ms = sorted(a.elapsed_ms(b) for a, b in ev)
+    probe_avg_ms = float(np.mean(probe_ms))
     achieved = probe_bytes / (probe_avg_ms / 1e3) / 1e9
 
     line = {
@@ -227,6 +228,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(args.workload, args.zipf),
             "avg_kernel_ms": probe_avg_ms,
+            "min_kernel_ms": probe_ms[0],
+            "median_kernel_ms": probe_ms[len(probe_ms) // 2],
             "algorithmic_bytes": probe_bytes,
         },
         "wall_s": wall,
