@@ -89,7 +89,11 @@ static int tile_rows_of(const vh_config& c)
 	int shift = c.tile_shift;
 	if (shift < -6) shift = -6;
 	if (shift > 6) shift = 6;
-	return shift >= 0 ? (65536 << shift) : (65536 >> (-shift));
+	// 16 K rows by default: a partitioned group-by's tuples of one item are
+	// re-read (sorted by partition) soon after they are written, and smaller
+	// items keep them cached (measured on the 256-pipeline batch: 5.52 ms at
+	// 16 K rows, 5.63 at 32 K, 5.86 at 64 K, 6.14 at 128 K)
+	return shift >= 0 ? (16384 << shift) : (16384 >> (-shift));
 }
 
 // Lowering of one pipeline of a batch: lane-private group-by state cannot
@@ -106,8 +110,12 @@ static int lower_for_batch(int i, const vh_pipeline_desc* d, const vh_config& cf
 	Plan& P = L.plan;
 	if (P.mode == GB_REG)
 		P.mode = GB_LDS;
-	const int T = tile_rows_of(cfg);
+	i64 T = tile_rows_of(cfg);
 	if (P.mode == GB_PART) {
+		// the default item size grows for a pipeline that would otherwise
+		// have more regions than a partition's run table holds
+		while (cfg.tile_shift == 0 && (P.rows + T - 1) / T > kMaxGroupRegions && T < ((i64)1 << 30))
+			T <<= 1;
 		P.gregion_rows = T;
 		if ((P.rows + T - 1) / T > kMaxGroupRegions)
 			return fail(VH_ERR_UNSUPPORTED, "pipeline %d: more than %d items for a partitioned group-by", i,
@@ -177,7 +185,8 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		if ((rc = lower_for_batch(i, &q->desc, q->cfg, q->L)))
 			break;
 		Plan& P = q->L.plan;
-		const int T = tile_rows_of(q->cfg);
+		// rows per work item (a partitioned group-by's regions, as lowered)
+		const i64 T = P.mode == GB_PART ? P.gregion_rows : (i64)tile_rows_of(q->cfg);
 		if ((rc = query_create(q, false))) {
 			fail(rc, "pipeline %d: %s", i, last_error());
 			break;
