@@ -381,6 +381,7 @@ int vh_vec_cmp_varchar(int32_t cmp, const int32_t* sel, int64_t inum, uint8_t* r
 /* ------------------------------------------------------------------------ */
 
 typedef struct vh_batch vh_batch;
+/* grid: workgroups of each persistent launch (0: eight per CU). */
 int vh_batch_new(vh_batch** b, int32_t n, const vh_pipeline_desc* descs, const vh_config* cfgs,
 	int32_t grid);
 /* Enqueue: one multi-buffer clear + one persistent kernel (asynchronous). */

@@ -307,7 +307,10 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	int dev = 0, cus = 256;
 	(void)hipGetDevice(&dev);
 	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	b->grid = grid > 0 ? grid : cus * 4;
+	// eight persistent workgroups per CU (measured on the 256-pipeline batch:
+	// 5.35 ms at 8 per CU, 5.53 at 4, 6.60 at 2, 5.47 at 24; a workgroup that
+	// finds its queue drained returns at once)
+	b->grid = grid > 0 ? grid : cus * 8;
 	*out = b;
 	return VH_OK;
 }
