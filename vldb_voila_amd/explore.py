@@ -124,7 +124,7 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=4000)
     ap.add_argument("--heldout", type=int, default=500)
     ap.add_argument("--epochs", type=int, default=300)
-    ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--out", default="selector_out")
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--timings", default=None, help="reuse an existing timings.npz")
     ap.add_argument("--runlog", default=None, help="also record every timing in this SQLite run log "
