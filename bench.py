@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--zipf", type=float, default=0.0, help="star: Zipf theta of the part FKs")
     ap.add_argument("--copies", type=int, default=0, help="input copies rotated (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores, at most 16")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every usable host core, at most 16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and so the result check)")
     return ap.parse_args()
 
@@ -77,13 +77,15 @@ def workload_join(datagen, workloads):
         return workloads.join_aggregate_pipeline(build, probe, a[0], a[1])
 
     def probe_bytes(q):
-        # fk 4 + date 4 + price 8 B per probe row, one join-table entry per
-        # row that passes the date predicate
-        entry, _ = q.join(0)
-        return 6_000_000 * 16 + sel * entry
+        # fk 4 + date 4 + price 8 B per probe row, one 8-byte hash-table read
+        # (key i32 + segment i32) per row that passes the date predicate
+        return 6_000_000 * 16 + sel * 8
+    # the whole step: build input 12 B per row (key, nation, segment), a
+    # hash-table write of 8 B per build row, then the probe's bytes
+    step_bytes = 1_500_000 * 12 + 1_500_000 * 8 + 6_000_000 * 16 + sel * 8
     return {"name": "hash join 1.5M build x 6M probe (50% date filter) + sum(price) by segment (5 groups)",
             "tables": [build, probe], "make": make, "tuples": 7_500_000, "probe_bytes": probe_bytes,
-            "config": {"build_rows": 1_500_000, "probe_rows": 6_000_000}}
+            "step_bytes": step_bytes, "config": {"build_rows": 1_500_000, "probe_rows": 6_000_000}}
 
 
 def workload_filter(datagen, workloads):
@@ -94,7 +96,7 @@ def workload_filter(datagen, workloads):
     # flag 4 + status 4 + qty 8 + price 8 + disc 8 + date 4 B per row
     return {"name": "filter (96%) + group by (flag, status): 3 sums (one int128) + count, 6M rows",
             "tables": [t], "make": make, "tuples": 6_000_000,
-            "probe_bytes": lambda q: 6_000_000 * 36, "config": {"rows": 6_000_000}}
+            "probe_bytes": lambda q: 6_000_000 * 36, "step_bytes": 6_000_000 * 36, "config": {"rows": 6_000_000}}
 
 
 def workload_star(datagen, workloads, theta):
@@ -106,11 +108,12 @@ def workload_star(datagen, workloads, theta):
         return workloads.star_pipeline(fact, date, cust, supp, part,
                                        {"fact": a[0], "date": a[1], "customer": a[2], "supplier": a[3],
                                         "part": a[4]})
-    # the four int32 FKs and two int64 measures of every fact row
+    # the four int32 FKs and two int64 measures of every fact row; the step
+    # also reads each dimension's key and used columns (12 B per row)
     return {"name": f"star join: fact 60M x date, customer, supplier (region filters), part; "
                     f"175 groups; part FKs Zipf theta={theta}",
             "tables": [fact, date, cust, supp, part], "make": make, "tuples": 60_000_000 + dims,
-            "probe_bytes": lambda q: 60_000_000 * 32,
+            "probe_bytes": lambda q: 60_000_000 * 32, "step_bytes": 60_000_000 * 32 + dims * 12,
             "config": {"fact_rows": 60_000_000, "dimension_rows": dims, "zipf_theta": theta}}
 
 
@@ -231,6 +234,9 @@ def main():
             "min_kernel_ms": probe_ms[0],
             "median_kernel_ms": probe_ms[len(probe_ms) // 2],
             "algorithmic_bytes": probe_bytes,
+            "step_algorithmic_bytes": W["step_bytes"],
+            "step_achieved": W["step_bytes"] / (ms_per_step / 1e3) / 1e9,
+            "step_frac": W["step_bytes"] / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
         "wall_s": wall,
     }
@@ -244,23 +250,58 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def host_cpu():
+    """(CPU model, nproc, cores this process may use)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, os.cpu_count() or 1, allowed
+
+
+def cpu_threads(args):
+    """Every core this process may use, at most 16 (a GPU box's CPU share
+    for one GPU, whatever nproc reports there)."""
+    _, _, allowed = host_cpu()
+    return args.cpu_threads or max(1, min(allowed, 16))
+
+
 def cpu_baseline(desc, tuples, args):
     """The oracle's vector(1024) restatement on the host cores: the full
-    workload, hot runs until the budget is spent (at least 2, at most 5).
-    Returns (baseline record, its sorted result rows)."""
+    workload, 5 hot runs (min and avg, as compiler.cpp:576-588 reports them)
+    on every usable core, and the same at one thread while the budget
+    lasts.  Returns (baseline record, its sorted result rows)."""
     import oracle
 
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-    times, text = [], ""
-    t0 = time.perf_counter()
-    while len(times) < 5 and (len(times) < 2 or time.perf_counter() - t0 < args.cpu_seconds):
-        text, ns = oracle.run_pipeline(desc, threads=threads)
-        times.append(ns / 1e9)
-    best = min(times)
-    return ({"value": tuples / best, "unit": "input tuples/s", "cores": threads, "kind": "port",
-             "sample": f"full workload, {len(times)} hot runs of the vector(1024) restatement, "
-                       f"min {best * 1e3:.1f} ms, avg {sum(times) / len(times) * 1e3:.1f} ms"},
-            sorted(text.splitlines()))
+    model, nproc, allowed = host_cpu()
+    threads = cpu_threads(args)
+
+    def hot_runs(t, budget):
+        times, text = [], ""
+        t0 = time.perf_counter()
+        oracle.run_pipeline(desc, threads=t)          # cold run, not counted
+        while len(times) < 5 and (len(times) < 1 or time.perf_counter() - t0 < budget):
+            text, ns = oracle.run_pipeline(desc, threads=t)
+            times.append(ns / 1e9)
+        return times, text
+
+    times, text = hot_runs(threads, args.cpu_seconds)
+    t1, _ = hot_runs(1, args.cpu_seconds)
+    best, best1 = min(times), min(t1)
+    rec = {"value": tuples / best, "unit": "input tuples/s", "cores": threads, "kind": "port",
+           "cpu_model": model, "nproc": nproc, "usable_cores": allowed,
+           "avg_value": tuples / (sum(times) / len(times)),
+           "t1_value": tuples / best1, "t1_avg_value": tuples / (sum(t1) / len(t1)),
+           "sample": f"full workload; vector(1024) restatement, {len(times)} hot runs at {threads} threads "
+                     f"(min {best * 1e3:.1f} ms, avg {sum(times) / len(times) * 1e3:.1f} ms) and {len(t1)} at "
+                     f"1 thread (min {best1 * 1e3:.1f} ms, avg {sum(t1) / len(t1) * 1e3:.1f} ms); "
+                     f"{model}, nproc {nproc}, {allowed} usable, pool capped at 16"}
+    return rec, sorted(text.splitlines())
 
 
 def bench_contains(args):
@@ -406,7 +447,7 @@ def bench_batch(args):
         # a bounded sample: the smallest pipelines until the budget is spent;
         # each one's rows are also the check of the batched GPU result
         import oracle
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = cpu_threads(args)
         order = sorted(range(len(hosts)), key=lambda i: hosts[i][0]["probe_rows"])
         done, ns_total, t_sample = 0, 0, 0
         t0 = time.perf_counter()

@@ -310,6 +310,9 @@ int vh_table_reset(vh_table* t, void* stream);               /* ITable::reset   
 int vh_table_rows(vh_table* t, void* stream, int64_t* rows);  /* synchronises   */
 int vh_table_column(vh_table* t, int32_t col, void** dev_ptr);
 
+/* Selections count set flags per tile into library-owned scratch, one buffer
+ * per (device, stream) of the calling thread: selections on different
+ * streams run concurrently; two on one stream are ordered by it. */
 int vh_vec_seltrue(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
 	int32_t* d_onum, void* stream);                           /* vec_seltrue  */
 int vh_vec_selfalse(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,

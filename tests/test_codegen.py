@@ -157,3 +157,32 @@ def test_batch_entry_point_compiles(mode):
         assert "lds_dense_flush(P, L);" in src
     rc, log, size = compile_for_gfx950(src)
     assert rc == 0 and size > 0, log[:2000]
+
+
+def test_mixed_sign_product_is_refused():
+    """The reference types a product by its operands' (min*min, max*max)
+    (typing_pass.cpp:164-176).  With one operand of mixed sign that interval
+    misses the true one: a in [-60, 1] times b in [1, 3] is typed i8 over
+    [-60, 3] while the values span [-180, 3], so the reference's vector
+    flavor would wrap.  The lowering refuses such a sum instead of producing
+    wrapped numbers; the same product with both operands non-negative is
+    accepted."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    n = 1000
+    a = rng.integers(-60, 2, n).astype(np.int32)
+    b = rng.integers(1, 4, n).astype(np.int32)
+    a[:2], b[:2] = (-60, 1), (1, 3)
+    g = rng.integers(0, 3, n).astype(np.int32)
+    t = table_from_arrays("t", {"a": a, "b": b, "g": g})
+    bad = Pipeline(t).groupby(["g"], [("sum", col("a") * col("b"))]).desc()
+    L = runtime.lib()
+    L.vh_generate_source.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    n_out = C.c_size_t()
+    cfg = runtime.make_config()
+    rc = L.vh_generate_source(C.byref(bad), C.byref(cfg), None, 0, C.byref(n_out))
+    assert rc == 2   # VH_ERR_UNSUPPORTED
+    assert b"may wrap its type" in L.vh_last_error()
+    t2 = table_from_arrays("t", {"a": a + 60, "b": b, "g": g})
+    good = Pipeline(t2).groupby(["g"], [("sum", col("a") * col("b"))]).desc()
+    assert "vh_generated" in source(good)

@@ -51,6 +51,35 @@ def test_seltrue_selfalse(n, density, with_sel):
             assert np.array_equal(o[:ok], exp[exp < 1024])
 
 
+def test_selections_on_two_streams_do_not_share_scratch():
+    """Two different selections enqueued back to back on two streams and
+    synchronised only at the end: each keeps its own tile-count scratch, and
+    both agree with the oracle's vec_seltrue / vec_selfalse."""
+    print("seltrue on two streams", flush=True)
+    L = statements._bind()
+    rng = np.random.default_rng(7)
+    n1, n2 = 3_000_017, 2_100_001
+    p1 = (rng.random(n1) < 0.3).astype(np.uint8)
+    p2 = (rng.random(n2) < 0.7).astype(np.uint8)
+    d1, d2 = dev(p1), dev(p2)
+    r1, r2 = DeviceBuffer(n1 * 4), DeviceBuffer(n2 * 4)
+    c1, c2 = DeviceBuffer(16), DeviceBuffer(16)
+    s1, s2 = runtime.Stream(), runtime.Stream()
+    for _ in range(3):
+        runtime.check(L.vh_vec_seltrue(None, n1, r1.ptr, d1.ptr, c1.ptr, s1.ptr))
+        runtime.check(L.vh_vec_selfalse(None, n2, r2.ptr, d2.ptr, c2.ptr, s2.ptr))
+    s1.synchronize()
+    s2.synchronize()
+    for n, pred, res, cnt, want in ((n1, p1, r1, c1, 1), (n2, p2, r2, c2, 0)):
+        k = int(cnt.to_numpy(np.int32, 1)[0])
+        got = res.to_numpy(np.int32, k)
+        exp = np.zeros(n, dtype=np.int32)
+        fn = oracle.lib().vo_seltrue if want else oracle.lib().vo_selfalse
+        m = fn(None, n, exp.ctypes.data, pred.ctypes.data)
+        assert k == m
+        assert np.array_equal(got, exp[:m])
+
+
 def test_hash_rehash_against_oracle():
     print("hash", flush=True)
     L = statements._bind()

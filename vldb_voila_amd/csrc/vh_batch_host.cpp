@@ -134,7 +134,7 @@ static hipError_t enqueue(vh_batch* b, hipStream_t s)
 		const BatchLaunch& l = b->launches[k];
 		if (l.kind == kBatchScatter || l.kind == kBatchSlice) {
 			e = launch_batch_part(l.kind == kBatchSlice, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k,
-				b->grid / 4, l.lds, s);
+				std::max(1, b->grid / 4), l.lds, s);
 		} else if (l.kind == kBatchGenerated) {
 			if (!l.count)
 				continue;

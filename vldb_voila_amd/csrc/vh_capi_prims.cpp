@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "../../include/voila_hip.h"
@@ -34,8 +36,6 @@ struct vh_table {
 	u32* overflow = nullptr;
 	u64 max_rows = 0;
 	int64_t host_rows = 0;          // rows appended by vh_vec_write (write_pos)
-	u32* tiles = nullptr;           // selection scratch
-	int64_t tiles_cap = 0;
 	std::vector<void*> allocs;
 	~vh_table()
 	{
@@ -62,23 +62,37 @@ static u64 num_buckets(u64 count)
 	return 1ull << exp;
 }
 
-static thread_local u32* g_tiles = nullptr;
-static thread_local int64_t g_tiles_cap = 0;
+// Tile-count scratch of the selections (vh_vec_seltrue / vh_vec_selfalse),
+// one buffer per (device, stream) of the calling thread, so selections
+// enqueued on different streams never share one.  A buffer that must grow is
+// not freed: work already enqueued (or captured in a graph) may still read
+// it, so it is retired and kept; growth at least doubles, so the retired
+// buffers of a stream hold at most as much as its current one.
+struct SelScratch {
+	u32* tiles = nullptr;
+	int64_t cap = 0;
+};
+static thread_local std::map<std::pair<int, hipStream_t>, SelScratch> g_sel_scratch;
+static thread_local std::vector<void*> g_sel_retired;
 
-static int sel_scratch(int64_t inum, u32** out)
+static int sel_scratch(int64_t inum, hipStream_t stream, u32** out)
 {
 	const int64_t need = sel_tiles(inum) + 1;
-	if (need > g_tiles_cap) {
-		if (g_tiles)
-			(void)hipFree(g_tiles);
-		g_tiles = nullptr;
-		g_tiles_cap = 0;
-		hipError_t e = hipMalloc((void**)&g_tiles, need * sizeof(u32));
+	int dev = 0;
+	HIP_CALL(hipGetDevice(&dev));
+	SelScratch& S = g_sel_scratch[std::make_pair(dev, stream)];
+	if (need > S.cap) {
+		const int64_t cap = std::max<int64_t>(need, 2 * S.cap);
+		u32* p = nullptr;
+		hipError_t e = hipMalloc((void**)&p, cap * sizeof(u32));
 		if (e != hipSuccess)
 			return fail(VH_ERR_NOMEM, "selection scratch: %s", hipGetErrorString(e));
-		g_tiles_cap = need;
+		if (S.tiles)
+			g_sel_retired.push_back(S.tiles);
+		S.tiles = p;
+		S.cap = cap;
 	}
-	*out = g_tiles;
+	*out = S.tiles;
 	return VH_OK;
 }
 
@@ -187,7 +201,7 @@ int vh_vec_seltrue(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t
 	if (inum < 0 || inum > INT32_MAX || !d_onum || (inum && (!res || !pred)))
 		return fail(VH_ERR_INVALID, "vh_vec_seltrue arguments");
 	u32* tiles;
-	int rc = sel_scratch(inum, &tiles);
+	int rc = sel_scratch(inum, (hipStream_t)stream, &tiles);
 	if (rc)
 		return rc;
 	LAUNCH(launch_select(sel, inum, pred, 1, res, tiles, d_onum, (hipStream_t)stream));
@@ -199,7 +213,7 @@ int vh_vec_selfalse(const int32_t* sel, int64_t inum, int32_t* res, const uint8_
 	if (inum < 0 || inum > INT32_MAX || !d_onum || (inum && (!res || !pred)))
 		return fail(VH_ERR_INVALID, "vh_vec_selfalse arguments");
 	u32* tiles;
-	int rc = sel_scratch(inum, &tiles);
+	int rc = sel_scratch(inum, (hipStream_t)stream, &tiles);
 	if (rc)
 		return rc;
 	LAUNCH(launch_select(sel, inum, pred, 0, res, tiles, d_onum, (hipStream_t)stream));

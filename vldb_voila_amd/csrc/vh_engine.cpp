@@ -528,7 +528,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 			mode = GB_LDS;
 		else if (domain <= hash_limit && P.nsums <= 4)
 			mode = GB_LDSHASH;
-		else if (part_fits(domain, P.nsums))
+		else if (part_fits(domain, P.nsums) && P.rows < ((int64_t)1 << 32))
 			mode = GB_PART;
 		else if (domain <= (double)(1 << 20) && P.nsums <= 4)
 			mode = GB_LDSHASH;
@@ -552,6 +552,11 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	if (mode == GB_PART && !part_fits(domain, P.nsums))
 		return fail(VH_ERR_UNSUPPORTED, "partitioned group-by needs <= %d partitions of the dense domain "
 			"and <= 4 sums", kMaxGroupParts);
+	// tuple positions of the partitioned group-by (region bases, run offsets,
+	// partition totals) are 32-bit
+	if (mode == GB_PART && P.rows >= ((int64_t)1 << 32))
+		return fail(VH_ERR_UNSUPPORTED, "partitioned group-by over %lld probe rows: at most 2^32 - 1",
+			(long long)P.rows);
 	if (mode < GB_REG || (mode > GB_LDSHASH && mode != GB_PART))
 		return fail(VH_ERR_INVALID, "group mode %d", mode);
 	P.mode = mode;
@@ -829,6 +834,7 @@ int query_create(Query* q, bool single)
 	HIP_TRY(hipMemcpy(q->dclear, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
 
 	// join tables the generated probe kernel keeps in LDS, its workgroup size
+	const int generic_grid = L.grid;
 	if (single && q->cfg.kernel != VH_KERNEL_GENERIC) {
 		choose_probe_shape(L, &q->desc, &q->cfg);
 		for (int j = 0; j < P.njoins; j++) {
@@ -863,9 +869,24 @@ int query_create(Query* q, bool single)
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
 		if (rc && kernel != VH_KERNEL_AUTO)
 			return rc;
-		if (rc) {   // AUTO: the generic kernel computes the same result
+		if (rc) {
+			// AUTO: the generic kernel computes the same result.  It reads no
+			// LDS join tables and runs 256-thread workgroups, so the probe
+			// shape goes back to what the lowering chose for it.
 			q->jit_fn = nullptr;
 			q->jit_src.clear();
+			for (int j = 0; j < P.njoins; j++) {
+				P.joins[j].lds = LDS_NONE;
+				P.joins[j].lds_off = 0;
+				P.joins[j].bits = nullptr;
+				L.builds[j].bits = nullptr;
+			}
+			L.block = 256;
+			L.grid = generic_grid;
+			HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
+			if (P.njoins)
+				HIP_TRY(hipMemcpy(q->dbuilds, L.builds.data(), P.njoins * sizeof(BuildPlan),
+					hipMemcpyHostToDevice));
 		}
 	}
 	return VH_OK;
