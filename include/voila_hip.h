@@ -317,6 +317,16 @@ int vh_vec_seltrue(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t
 	int32_t* d_onum, void* stream);                           /* vec_seltrue  */
 int vh_vec_selfalse(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
 	int32_t* d_onum, void* stream);                           /* vec_selfalse */
+/* vh_vec_seltrue without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its tile
+ * counts live in a static device array of 512 KiB per GPU (held until the
+ * process ends), so inum is at most 131,071 x 4,096 = 536,866,816 positions
+ * (VH_ERR_CAPACITY above that).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams, other threads) race; calls
+ * ordered by one stream do not.  Same results as vh_vec_seltrue. */
+int vh_vec_seltrue_noalloc(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
+	int32_t* d_onum, void* stream);
 /* vec_<cmp>: res[i] = col1[i] cmp (col2 ? col2[i] : constant), both typed `type` */
 int vh_vec_cmp(int32_t cmp, int32_t type, const int32_t* sel, int64_t inum, uint8_t* res,
 	const void* col1, const void* col2, int64_t constant, void* stream);

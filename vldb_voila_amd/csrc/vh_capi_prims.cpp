@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -93,6 +94,35 @@ static int sel_scratch(int64_t inum, hipStream_t stream, u32** out)
 		S.cap = cap;
 	}
 	*out = S.tiles;
+	return VH_OK;
+}
+
+// Tile-count scratch of vh_vec_seltrue_noalloc: a static device array, so the
+// call allocates nothing.  Each device holds its own copy (at its own
+// address) until the process ends.  In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress, looked up once
+// per device (with that device current) and kept in g_scratch_ptr.
+static constexpr int64_t kStaticTiles = 131072;   // 512 KiB of u32 tile counts
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) u32 scratch[kStaticTiles];
+static std::mutex g_scratch_mu;
+static u32* g_scratch_ptr[kMaxDevices];
+
+static int static_scratch(u32** out)
+{
+	int dev = 0;
+	HIP_CALL(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices)
+		return fail(VH_ERR_INVALID, "device %d: at most %d devices", dev, kMaxDevices);
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		void* p = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p)
+			return fail(VH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+		g_scratch_ptr[dev] = (u32*)p;
+	}
+	*out = g_scratch_ptr[dev];
 	return VH_OK;
 }
 
@@ -204,6 +234,23 @@ int vh_vec_seltrue(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t
 	int rc = sel_scratch(inum, (hipStream_t)stream, &tiles);
 	if (rc)
 		return rc;
+	LAUNCH(launch_select(sel, inum, pred, 1, res, tiles, d_onum, (hipStream_t)stream));
+}
+
+int vh_vec_seltrue_noalloc(const int32_t* sel, int64_t inum, int32_t* res, const uint8_t* pred,
+	int32_t* d_onum, void* stream)
+{
+	if (inum < 0 || inum > INT32_MAX || !d_onum || (inum && (!res || !pred)))
+		return fail(VH_ERR_INVALID, "vh_vec_seltrue_noalloc arguments");
+	if (sel_tiles(inum) + 1 > kStaticTiles)
+		return fail(VH_ERR_CAPACITY, "vh_vec_seltrue_noalloc: %lld positions exceed its static scratch "
+			"(at most %lld)", (long long)inum, (long long)((kStaticTiles - 1) * kSelTileRows));
+	u32* tiles;
+	int rc = static_scratch(&tiles);
+	if (rc)
+		return rc;
+	// every tile count the scan reads is written first by the count pass, so
+	// the array needs no clearing between calls
 	LAUNCH(launch_select(sel, inum, pred, 1, res, tiles, d_onum, (hipStream_t)stream));
 }
 

@@ -13,7 +13,7 @@
 
 namespace vh {
 
-static constexpr int kTile = 4096;          // positions per selection tile
+static constexpr int kTile = (int)kSelTileRows;   // positions per selection tile
 static constexpr int kThreads = 256;
 
 __device__ __forceinline__ i64 pos_of(const int32_t* sel, i64 k)

@@ -11,6 +11,7 @@ typedef unsigned long long u64;
 typedef long long i64;
 typedef unsigned int u32;
 
+static constexpr i64 kSelTileRows = 4096;   // positions per selection tile
 i64 sel_tiles(i64 inum);
 hipError_t launch_select(const int32_t* sel, i64 inum, const uint8_t* pred, int want, int32_t* res,
 	u32* tile_scratch, int32_t* d_count, hipStream_t s);
