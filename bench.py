@@ -54,7 +54,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["join", "filter", "star", "batch", "contains"], default="join")
+    ap.add_argument("--workload", choices=["join", "join_sparse", "filter", "star", "batch", "contains"],
+                    default="join")
+    ap.add_argument("--join-layout", type=int, default=0, help="vh_config.join_layout (0 = auto)")
+    ap.add_argument("--part-shift", type=int, default=0, help="vh_config.part_shift (0 = auto, -1 = none)")
     ap.add_argument("--zipf", type=float, default=0.0, help="star: Zipf theta of the part FKs")
     ap.add_argument("--copies", type=int, default=0, help="input copies rotated (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
@@ -86,6 +89,23 @@ def workload_join(datagen, workloads):
     return {"name": "hash join 1.5M build x 6M probe (50% date filter) + sum(price) by segment (5 groups)",
             "tables": [build, probe], "make": make, "tuples": 7_500_000, "probe_bytes": probe_bytes,
             "step_bytes": step_bytes, "config": {"build_rows": 1_500_000, "probe_rows": 6_000_000}}
+
+
+def workload_join_sparse(datagen, workloads):
+    build, probe = datagen.gen_join_sparse(1_500_000, 6_000_000, seed=42)
+    sel = int((probe["date"] <= workloads.JOIN_DATE_CUTOFF).sum())
+
+    def make(addrs=None):
+        a = addrs or [None, None]
+        return workloads.join_aggregate_pipeline(build, probe, a[0], a[1])
+
+    W = {"name": "hash join 1.5M build (unique keys over [0, 2^31)) x 6M probe (50% date filter) + "
+                 "sum(price) by segment (5 groups)",
+         "tables": [build, probe], "make": make, "tuples": 7_500_000,
+         "probe_bytes": lambda q: 6_000_000 * 16 + sel * 8,
+         "step_bytes": 1_500_000 * 12 + 1_500_000 * 8 + 6_000_000 * 16 + sel * 8,
+         "config": {"build_rows": 1_500_000, "probe_rows": 6_000_000, "key_range": "[0, 2^31)"}}
+    return W
 
 
 def workload_filter(datagen, workloads):
@@ -149,6 +169,8 @@ def main():
 
     if args.workload == "join":
         W = workload_join(datagen, workloads)
+    elif args.workload == "join_sparse":
+        W = workload_join_sparse(datagen, workloads)
     elif args.workload == "filter":
         W = workload_filter(datagen, workloads)
     else:
@@ -162,7 +184,9 @@ def main():
     for c in range(copies):
         dev = [runtime.upload(t) for t in W["tables"]]
         bufs.append(dev)
-        queries.append(runtime.Query(W["make"]([runtime.addrs(d) for d in dev]).desc()))
+        queries.append(runtime.Query(W["make"]([runtime.addrs(d) for d in dev]).desc(),
+                                     runtime.make_config(join_layout=args.join_layout,
+                                                         part_shift=args.part_shift)))
     q0 = queries[0]
     info = q0.info()
     kernel, _ = q0.kernel()
@@ -220,6 +244,8 @@ def main():
         "data": "synthetic (seed 42, counter-based splitmix64)",
         "config": dict({"workload": W["name"], "input_copies": copies, "group_mode": info["group_mode"],
                         "grid": info["grid"],
+                        "join_tables": [dict(zip(("entry_bytes", "entries"), q0.join(j)))
+                                        for j in range(q0.desc.njoins)],
                         "probe_kernel": "generated (hipRTC)" if kernel == 2 else "generic"}, **W["config"]),
         "roofline": {
             "bound": "hbm",

@@ -212,6 +212,17 @@ typedef struct vh_config {
 	int32_t merge;         /* dense LDS group-by, how workgroups combine: 0 =
 	                          auto, 1 = partials + a merge launch, 2 = each
 	                          workgroup adds into the result with atomics     */
+	int32_t part_shift;    /* partition fan-out of the join builds: 0 = auto,
+	                          -1 = none (global atomic builds), n > 0 =
+	                          partitions of 2^n keys (direct tables, clamped
+	                          to 32 KiB slices) or 2^n slots (hash tables,
+	                          10..14): fan-out = table entries / 2^n.  Auto
+	                          partitions direct tables, and hash tables
+	                          larger than one XCD's 4 MiB L2, in 2^13-slot
+	                          sub-tables                                     */
+	int32_t block;         /* threads per workgroup of the generated probe
+	                          kernel (waves per CU = block / 64 x workgroups
+	                          per CU): 0 = auto, 256, 512 or 1024            */
 } vh_config;
 
 /* ------------------------------------------------------------------------ */

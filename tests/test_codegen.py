@@ -83,6 +83,21 @@ def test_generated_source_compiles(name, desc):
                 assert rc == 0 and size > 0, (name, mode, layout, kernel, log[:2000])
 
 
+def test_partitioned_hash_table_source():
+    """A hash table above 4 MiB is built partitioned: the generated probe
+    wraps within the key's sub-table and consults the overflow table after a
+    spill; forcing part_shift -1 gives the single-table probe."""
+    b, p = datagen.gen_join_sparse(600_000, 6000, seed=5)
+    d = workloads.join_aggregate_pipeline(b, p).desc()
+    parted = source(d)
+    assert "hnext(h[u]" in parted and ".ovf" in parted and ".spill" in parted
+    flat = source(d, part_shift=-1)
+    assert "hnext(" not in flat and ".ovf" not in flat
+    for src in (parted, flat, source(d, part_shift=11, kernel=abi.VH_KERNEL_JIT_PIPE)):
+        rc, log, size = compile_for_gfx950(src)
+        assert rc == 0 and size > 0, log[:2000]
+
+
 def test_source_reflects_the_plan():
     b, p = datagen.gen_join_aggregate(2000, 6000, seed=5)
     d = workloads.join_aggregate_pipeline(b, p).desc()

@@ -1,0 +1,120 @@
+"""Hash-table joins (keys too sparse for a direct table) on the GPU against
+the oracle: the headline join with build keys spread over [0, 2^31) at full
+size, the same with generic and generated probe kernels, JoinN with heavily
+repeated keys, and a build with every key in one hash partition."""
+import numpy as np
+import pytest
+
+import oracle
+from vldb_voila_amd import abi, datagen, runtime, workloads
+from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    print("gpu hash join: init device 0", flush=True)
+    runtime.init(0)
+
+
+def on_gpu(arrays_list, make):
+    devs = [runtime.upload(a) for a in arrays_list]
+    return make(*[runtime.addrs(d) for d in devs]), devs
+
+
+@pytest.mark.parametrize("kernel,part", [(abi.VH_KERNEL_AUTO, 0), (abi.VH_KERNEL_GENERIC, 0),
+                                         (abi.VH_KERNEL_AUTO, -1)])
+def test_sparse_join_full_size(kernel, part):
+    print(f"sparse join 1.5M/6M seed 42 kernel {kernel} part_shift {part}", flush=True)
+    b, pr = datagen.gen_join_sparse(1_500_000, 6_000_000, seed=42)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part))
+    assert q.join(0)[0] == 8                 # a hash table, not a direct one
+    for rep in range(3):
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+    m = pr["date"] <= workloads.JOIN_DATE_CUTOFF
+    assert sum(int(r.split("|")[1]) for r in want) == int(pr["price"][m].sum())
+
+
+@pytest.mark.parametrize("variant", [abi.VH_JOINN, abi.VH_JOIN01])
+def test_sparse_join_repeated_keys(variant):
+    """200k build rows over 3,000 sparse keys, one key repeated 20,000 times:
+    JoinN multiplies each probe row by its key's rows."""
+    print(f"sparse join repeated keys variant {variant}", flush=True)
+    rng = np.random.default_rng(9)
+    keys = datagen.sparse_keys(9, 3, 3000).astype(np.int32)
+    kb = keys[rng.integers(0, 3000, 200_000)]
+    kb[:20_000] = keys[7]
+    build = {"key": kb, "pay": rng.integers(0, 40, 200_000).astype(np.int32)}
+    probe = {"fk": keys[rng.integers(0, 3000, 400_000)],
+             "v": rng.integers(-10**6, 10**6, 400_000).astype(np.int64)}
+    if variant == abi.VH_JOIN01:
+        # first-match joins over unique keys only (which duplicate a first
+        # match picks is not part of the result contract)
+        _, first = np.unique(kb, return_index=True)
+        build = {k: v[np.sort(first)] for k, v in build.items()}
+
+    def make(x=None, y=None):
+        r = table_from_arrays("r", build, x)
+        s = table_from_arrays("s", probe, y)
+        return Pipeline(s).join(r, "key", "fk", payload=["pay"], variant=variant, names=["pay"]) \
+            .groupby(["pay"], [("sum", col("v")), ("count",)])
+    want = oracle.sorted_rows(oracle.bruteforce(make().desc()))
+    p, devs = on_gpu([build, probe], make)
+    for kernel in (abi.VH_KERNEL_AUTO, abi.VH_KERNEL_GENERIC):
+        q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel))
+        assert q.join(0)[0] == 8
+        for rep in range(2):
+            q.run()
+            assert oracle.sorted_rows(q.result()) == want
+
+
+def test_hash_layout_on_dense_keys_matches_direct():
+    """The headline join forced onto the hash layout gives the direct
+    layout's rows."""
+    print("dense keys, hash layout", flush=True)
+    b, pr = datagen.gen_join_aggregate(300_000, 1_000_000, seed=4)
+    p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
+    for layout in (abi.VH_LAYOUT_HASH, abi.VH_LAYOUT_DIRECT):
+        q = runtime.Query(p.desc(), runtime.make_config(join_layout=layout))
+        assert q.join(0)[0] == (8 if layout == abi.VH_LAYOUT_HASH else 1)
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want
+
+
+@pytest.mark.parametrize("part", [0, 10])
+def test_partitioned_hash_table_spills(part):
+    """800k build rows over sparse keys (a 16 MiB table, built partitioned),
+    30,000 of them with one key: more than a 2^13-slot sub-table holds, so
+    rows spill to the overflow table, and JoinN must find them there.  With
+    2^10-slot sub-tables (part_shift 10) many keys spill.  Three runs in a
+    row: the overflow table is refilled after a run that used it."""
+    print(f"partitioned hash spill part_shift {part}", flush=True)
+    rng = np.random.default_rng(21)
+    keys = datagen.sparse_keys(21, 5, 500_000).astype(np.int32)
+    kb = keys[rng.integers(0, 500_000, 800_000)]
+    kb[:30_000] = keys[3]
+    kb[30_000:33_000] = keys[rng.integers(0, 500_000, 300)].repeat(10)
+    build = {"key": kb, "pay": rng.integers(0, 7, 800_000).astype(np.int32)}
+    fk = keys[rng.integers(0, 500_000, 300_000)]
+    fk[:2000] = keys[3]
+    probe = {"fk": fk, "v": rng.integers(-10**6, 10**6, 300_000).astype(np.int64)}
+
+    def make(x=None, y=None):
+        r = table_from_arrays("r", build, x)
+        s = table_from_arrays("s", probe, y)
+        return Pipeline(s).join(r, "key", "fk", payload=["pay"], variant=abi.VH_JOINN, names=["pay"]) \
+            .groupby(["pay"], [("sum", col("v")), ("count",)])
+    want = oracle.sorted_rows(oracle.bruteforce(make().desc()))
+    p, devs = on_gpu([build, probe], make)
+    for kernel in (abi.VH_KERNEL_AUTO, abi.VH_KERNEL_GENERIC):
+        q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part))
+        w, n = q.join(0)
+        assert w == 8 and n * 8 > 4 << 20
+        for rep in range(3):
+            q.run()
+            assert oracle.sorted_rows(q.result()) == want

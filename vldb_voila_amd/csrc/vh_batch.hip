@@ -80,10 +80,7 @@ __global__ __launch_bounds__(256) void batch_probe_kernel(const Plan* __restrict
 	BatchItem I;
 	while (next_item(items, first, count, counter, &slot, I)) {
 		const Plan& P = plans[I.plan];
-		u32 dups_mask = 0;
-		for (int j = 0; j < P.njoins; j++)
-			if (*P.joins[j].dups)
-				dups_mask |= 1u << j;
+		const u32 dups_mask = join_flags(P);
 		probe_item_m<MODE>(P, I.r0, I.r1, lds, dups_mask);
 	}
 }

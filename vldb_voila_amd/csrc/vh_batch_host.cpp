@@ -207,7 +207,9 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 				for (i64 r0 = 0; r0 < B.rows; r0 += T)
 					build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, B.rows)});
 			}
-			segs.push_back({q->slots[j], q->L.capacity[j], ~0ull});
+			// a partitioned hash table's slices store every slot: its
+			// overflow table is what needs refilling
+			segs.push_back({B.hashed ? q->ovf[j] : q->slots[j], q->L.capacity[j], ~0ull});
 			if (q->dir[j] && !B.nparts)      // a partitioned build rewrites all of dir[]
 				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
 		}

@@ -407,6 +407,11 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		const double prange = jd.npayload
 			? (double)jd.build.cols[jd.payload[0]].max - (double)jd.build.cols[jd.payload[0]].min : 0;
 		const bool dense = krange <= 4.0 * (double)std::max<int64_t>(jd.build.rows, 0) + 65536.0;
+		const int pcfg = cfg ? cfg->part_shift : 0;
+		if (pcfg < -1 || pcfg > 30)
+			return fail(VH_ERR_INVALID, "part_shift %d", pcfg);
+		const uint64_t brows = (uint64_t)std::max<int64_t>(jd.build.rows, 1);
+		const uint64_t chunks = (brows + kBuildChunkRows - 1) / kBuildChunkRows;
 		if (layout != VH_LAYOUT_HASH && krange <= (double)(1u << 30) && (layout == VH_LAYOUT_DIRECT || dense)) {
 			B.dir_w = prange <= 254 ? 1 : (prange <= 65534 ? 2 : 4);
 			// whole 16-byte units, so that an LDS copy may read it 16 bytes at a time
@@ -414,23 +419,42 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 			// partitioned build: slices of about 8192 rows (one workgroup
 			// each), at most kMaxSliceBytes of LDS and kMaxBuildParts of them
 			const uint64_t R = (uint64_t)krange;
-			const uint64_t brows = (uint64_t)std::max<int64_t>(jd.build.rows, 1);
 			int shift = 10;
-			while (shift < 30 && ((uint64_t)1 << shift) * brows < R * 8192)
-				shift++;
-			while (shift > 2 && ((uint64_t)1 << shift) * (uint64_t)B.dir_w > (uint64_t)kMaxSliceBytes)
+			if (pcfg > 0)
+				shift = std::max(pcfg, 5);   // the key bitmap takes whole 32-bit words of a slice
+			else
+				while (shift < 30 && ((uint64_t)1 << shift) * brows < R * 8192)
+					shift++;
+			while (shift > 5 && ((uint64_t)1 << shift) * (uint64_t)B.dir_w > (uint64_t)kMaxSliceBytes)
 				shift--;
 			const uint64_t parts = (R + ((uint64_t)1 << shift) - 1) >> shift;
-			const uint64_t chunks = (brows + kBuildChunkRows - 1) / kBuildChunkRows;
-			if (parts <= (uint64_t)kMaxBuildParts && chunks <= (uint64_t)kMaxBuildChunks) {
+			if (pcfg >= 0 && parts <= (uint64_t)kMaxBuildParts && chunks <= (uint64_t)kMaxBuildChunks) {
 				B.pshift = shift;
 				B.nparts = (int)parts;
+				B.nchunks = jd.build.rows > 0 ? (int)chunks : 0;
+			}
+		} else if (pcfg >= 0) {
+			// hash table: partitioned build when it exceeds one XCD's 4 MiB
+			// of L2 (or when asked): blocks of 2^13 slots (64 KiB of LDS),
+			// larger when there would be more than kMaxBuildParts of them
+			const int cbits = 63 - __builtin_clzll(cap);
+			int shift = pcfg > 0 ? std::min(std::max(pcfg, 10), 14) : 13;
+			while (shift < 14 && (cap >> shift) > (uint64_t)kMaxBuildParts)
+				shift++;
+			shift = std::min(shift, cbits);
+			if ((pcfg > 0 || cap * sizeof(u64) > ((uint64_t)4 << 20)) && (cap >> shift) <= (uint64_t)kMaxBuildParts &&
+				chunks <= (uint64_t)kMaxBuildChunks) {
+				B.hashed = 1;
+				B.pshift = shift;
+				B.nparts = (int)(cap >> shift);
 				B.nchunks = jd.build.rows > 0 ? (int)chunks : 0;
 			}
 		}
 
 		JoinT& J = P.joins[j];
 		J.mask = cap - 1;
+		J.smask = B.hashed ? ((1ull << B.pshift) - 1) : cap - 1;
+		J.hashed = B.hashed;
 		if ((rc = role_of_vcol(d, jd.probe_key, J.key)))
 			return rc;
 		J.kmin = kc.min;
@@ -740,6 +764,15 @@ int query_create(Query* q, bool single)
 			L.builds[j].dir = q->dir[j];
 		}
 		BuildPlan& B = L.builds[j];
+		if (B.hashed) {
+			if ((rc = q->alloc((void**)&q->ovf[j], L.capacity[j] * sizeof(u64))))
+				return rc;
+			HIP_TRY(hipMemset(q->ovf[j], 0xFF, L.capacity[j] * sizeof(u64)));
+			B.ovf = q->ovf[j];
+			B.spill = spill_flag(q, j);
+			P.joins[j].ovf = q->ovf[j];
+			P.joins[j].spill = spill_flag(q, j);
+		}
 		if (B.nparts) {
 			const size_t g = (size_t)std::max(B.nchunks, 1);
 			if ((rc = q->alloc((void**)&B.pbuf, g * kBuildChunkRows * sizeof(u64))) ||
@@ -812,7 +845,11 @@ int query_create(Query* q, bool single)
 	if (q->dctl)
 		segs.push_back({(u64*)q->dctl, 1, 0, nullptr});
 	for (int j = 0; j < P.njoins; j++) {
-		if (L.builds[j].nparts) {
+		if (L.builds[j].hashed) {
+			// the slices store every slot; only the overflow table is
+			// refilled, after a run that spilled into it
+			segs.push_back({q->ovf[j], L.capacity[j], ~0ull, spill_flag(q, j)});
+		} else if (L.builds[j].nparts) {
 			segs.push_back({q->slots[j], L.capacity[j], ~0ull, dups_flag(q, j)});
 		} else {
 			segs.push_back({(u64*)dups_flag(q, j), 1, 0, nullptr});

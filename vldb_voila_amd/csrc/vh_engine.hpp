@@ -47,6 +47,7 @@ struct Query {
 	u32* flags = nullptr;
 	u64* slots[VH_MAX_JOINS] = {};
 	u64* dir[VH_MAX_JOINS] = {};
+	u64* ovf[VH_MAX_JOINS] = {};       // overflow table of a partitioned hash table
 	u32* bits[VH_MAX_JOINS] = {};      // JoinT::lds == LDS_BITS: key bitmap of dir
 	struct FillSeg* dclear = nullptr;   // segments of the per-run clear pass
 	int nclear = 0;
@@ -70,8 +71,11 @@ struct Query {
 	int alloc(void** p, size_t bytes);
 };
 
-static constexpr int kFlagWords = 2 + 2 * VH_MAX_JOINS;   // error word, then a dups word per join
+// error word, then per join a 64-bit word: its repeated-key flag, then its
+// spill flag (a partitioned hash build put rows in the overflow table)
+static constexpr int kFlagWords = 2 + 2 * VH_MAX_JOINS;
 inline u32* dups_flag(Query* q, int j) { return q->flags + 2 + 2 * j; }
+inline u32* spill_flag(Query* q, int j) { return q->flags + 3 + 2 * j; }
 
 int fail(int code, const char* fmt, ...);
 const char* last_error();

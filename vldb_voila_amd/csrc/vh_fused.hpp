@@ -23,6 +23,26 @@ static constexpr int kU = 4;              // rows per lane per chunk
 // means a damaged table and is reported (error bit 32) instead of followed.
 static constexpr u64 kMaxProbe = 1ull << 16;
 
+// Next slot of a linear probe sequence that wraps within aligned blocks of
+// smask + 1 slots (the whole table, or one partition's sub-table).
+__device__ __forceinline__ u64 hnext(u64 h, u64 smask)
+{
+	return (h & ~smask) | ((h + 1) & smask);
+}
+
+// The join flags word of a launch, read once from the builds' flags.
+__device__ __forceinline__ u32 join_flags(const Plan& P)
+{
+	u32 m = 0;
+	for (int j = 0; j < P.njoins; j++) {
+		if (*P.joins[j].dups)
+			m |= 1u << j;
+		if (P.joins[j].hashed && *P.joins[j].spill)
+			m |= 1u << (kSpillBit + j);
+	}
+	return m;
+}
+
 // --------------------------------------------------------------------------
 // direct join tables: dir[key - kmin] = payload - pmin in w = 1, 2 or 4 bytes
 // --------------------------------------------------------------------------
@@ -641,6 +661,7 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 	// joins: first slot of every live row in flight, then resolve
 	u64 h_last[kU];
 	u32 off_last[kU];
+	u32 ovf_first = 0;   // rows whose last-join match came from the overflow table
 #pragma unroll
 	for (int j = 0; j < VH_MAX_JOINS; j++) {
 		if (j < P.njoins) {
@@ -681,14 +702,31 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 			for (int u = 0; u < kU; u++) {
 				if (!C.alive[u])
 					continue;
+				// a sub-table may be full (its other rows spilled): one full
+				// cycle of it, smask + 1 slots, ends the walk
+				const u64 lim = J.smask < kMaxProbe ? J.smask + 1 : kMaxProbe;
 				u64 n = 0;
-				while (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {
+				while (s[u] != kEmpty && (u32)s[u] != off[u] && n < lim) {
 					n++;
-					h[u] = (h[u] + 1) & J.mask;
+					h[u] = hnext(h[u], J.smask);
 					s[u] = gload(J.slots, (i64)h[u]);
 				}
 				if (n >= kMaxProbe)
 					gatomic_or(P.error, 32u);
+				if ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> (kSpillBit + j)) & 1u)) {
+					// not in its sub-table: rows of a full sub-table are in
+					// the overflow table
+					if (J.last)
+						ovf_first |= 1u << u;
+					h[u] = voila_hash((u64)off[u]) & J.mask;
+					s[u] = gload(J.ovf, (i64)h[u]);
+					for (n = 0; s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe; n++) {
+						h[u] = (h[u] + 1) & J.mask;
+						s[u] = gload(J.ovf, (i64)h[u]);
+					}
+					if (n >= kMaxProbe)
+						gatomic_or(P.error, 32u);
+				}
 				if (s[u] == kEmpty || (u32)s[u] != off[u])
 					C.alive[u] = false;
 				else
@@ -725,15 +763,27 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 			for (int u = 0; u < kU; u++) {
 				if (!C.alive[u])
 					continue;
-				u64 h = (h_last[u] + 1) & J.mask;
+				// the rest of the match's sequence: its sub-table (or the
+				// whole table), then the overflow table if the build spilled
+				bool ov = (ovf_first >> u) & 1u;
+				const u64 h0 = voila_hash((u64)off_last[u]) & J.mask;   // where the sequence began
+				u64 h = ov ? (h_last[u] + 1) & J.mask : hnext(h_last[u], J.smask);
 				for (u64 n = 0;; n++) {
 					if (n >= kMaxProbe) {
 						gatomic_or(P.error, 32u);
 						break;
 					}
-					const u64 s = gload(J.slots, (i64)h);
-					if (s == kEmpty)
+					// a full sub-table has no empty slot: its sequence ends
+					// where it began
+					const u64 s = (!ov && J.hashed && h == h0) ? kEmpty : gload(ov ? J.ovf : J.slots, (i64)h);
+					if (s == kEmpty) {
+						if (!ov && ((dups_mask >> (kSpillBit + last)) & 1u)) {
+							ov = true;
+							h = voila_hash((u64)off_last[u]) & J.mask;
+							continue;
+						}
 						break;
+					}
 					if ((u32)s == off_last[u]) {
 						Chunk one = C;
 #pragma unroll
@@ -747,7 +797,7 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 						}
 						consume_u<MODE>(P, L, one);
 					}
-					h = (h + 1) & J.mask;
+					h = ov ? (h + 1) & J.mask : hnext(h, J.smask);
 				}
 			}
 		}

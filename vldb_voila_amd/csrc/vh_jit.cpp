@@ -434,7 +434,12 @@ struct Gen {
 				snprintf(b, sizeof b, "\t\t\tfor (int u = 0; u < U; u++) VH_KILL(u, pay%d[u] == %s);", j, ev);
 				line(b);
 			} else {
+				// next slot: within the key's sub-table for a partitioned table
+				const std::string step = J.hashed
+					? "hnext(h[u], " + uniform("u64", jp + ".smask") + ")" : "(h[u] + 1) & " + mask;
 				line("\t\t\tu64 h[U], s[U];");
+				if (J.hashed && j == last)
+					line("\t\t\tu32 ovf_first = 0;");
 				line("#pragma unroll");
 				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + mask + ";");
 				line("#pragma unroll");
@@ -445,12 +450,33 @@ struct Gen {
 				line(b);
 				line("\t\t\t\tif (!VH_LIVE(u)) continue;");
 				line("\t\t\t\tu64 n = 0;");
-				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe) {");
+				// a partitioned table's sub-table may be full (its other rows
+				// spilled): one full cycle of it ends the walk
+				const std::string lim = J.hashed ? "(" + uniform("u64", jp + ".smask") + " + 1)" : "kMaxProbe";
+				line("\t\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n < " + lim + ") {");
 				line("\t\t\t\t\tn++;");
-				line("\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
+				line("\t\t\t\t\th[u] = " + step + ";");
 				line("\t\t\t\t\ts[u] = " + slots + "[h[u]];");
 				line("\t\t\t\t}");
 				line("\t\t\t\tif (n >= kMaxProbe) gatomic_or(" + err + ", 32u);");
+				if (J.hashed) {
+					// not in its sub-table: a full sub-table's rows are in the
+					// overflow table (whole-table wrap)
+					const std::string ovf = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".ovf");
+					snprintf(b, sizeof b, "\t\t\t\tif ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> %d) & 1u)) {",
+						kSpillBit + j);
+					line(b);
+					if (j == last)
+						line("\t\t\t\t\tovf_first |= 1u << u;");
+					line("\t\t\t\t\th[u] = voila_hash((u64)off[u]) & " + mask + ";");
+					line("\t\t\t\t\ts[u] = " + ovf + "[h[u]];");
+					line("\t\t\t\t\tfor (n = 0; s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe; n++) {");
+					line("\t\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
+					line("\t\t\t\t\t\ts[u] = " + ovf + "[h[u]];");
+					line("\t\t\t\t\t}");
+					line("\t\t\t\t\tif (n >= kMaxProbe) gatomic_or(" + err + ", 32u);");
+					line("\t\t\t\t}");
+				}
 				line("\t\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) VH_KILL(u, 1);");
 				snprintf(b, sizeof b, "\t\t\t\telse pay%d[u] = (u32)(s[u] >> 32);", j);
 				line(b);
@@ -482,6 +508,32 @@ struct Gen {
 					line("#pragma unroll");
 					line("\t\t\t\tfor (int u = 0; u < U; u++) {");
 					line("\t\t\t\t\tif (!VH_LIVE(u)) continue;");
+					if (J.hashed) {
+						// the rest of the match's sub-table sequence, then
+						// the overflow table if the build spilled
+						const std::string sm = uniform("u64", jp + ".smask");
+						const std::string ovf = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".ovf");
+						line("\t\t\t\t\tbool ov = (ovf_first >> u) & 1u;");
+						line("\t\t\t\t\tconst u64 h0 = voila_hash((u64)off[u]) & " + mask + ";");
+						line("\t\t\t\t\tu64 hh = ov ? (h[u] + 1) & " + mask + " : hnext(h[u], " + sm + ");");
+						line("\t\t\t\t\tfor (u64 n = 0;; n++) {");
+						line("\t\t\t\t\t\tif (n >= kMaxProbe) { gatomic_or(" + err + ", 32u); break; }");
+						// a full sub-table has no empty slot: its sequence ends where it began
+						line("\t\t\t\t\t\tconst u64 sv = ov ? " + ovf + "[hh] : (hh == h0 ? kEmpty : " + slots + "[hh]);");
+						line("\t\t\t\t\t\tif (sv == kEmpty) {");
+						snprintf(b, sizeof b, "\t\t\t\t\t\t\tif (!ov && ((dups_mask >> %d) & 1u)) {", kSpillBit + j);
+						line(b);
+						line("\t\t\t\t\t\t\t\tov = true;");
+						line("\t\t\t\t\t\t\t\thh = voila_hash((u64)off[u]) & " + mask + ";");
+						line("\t\t\t\t\t\t\t\tcontinue;");
+						line("\t\t\t\t\t\t\t}");
+						line("\t\t\t\t\t\t\tbreak;");
+						line("\t\t\t\t\t\t}");
+						line("\t\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
+						line("\t\t\t\t\t\thh = ov ? (hh + 1) & " + mask + " : hnext(hh, " + sm + ");");
+						line("\t\t\t\t\t}");
+						line("\t\t\t\t}");
+					} else {
 					if (J.dir_w)
 						line("\t\t\t\t\tu64 hh = voila_hash((u64)off[u]) & " + mask + ";");
 					else
@@ -494,6 +546,7 @@ struct Gen {
 					line("\t\t\t\t\t\thh = (hh + 1) & " + mask + ";");
 					line("\t\t\t\t\t}");
 					line("\t\t\t\t}");
+					}
 					line("\t\t\t}");
 				}
 			}
@@ -723,9 +776,13 @@ struct Gen {
 					".lds_off);\n";
 		}
 		k += "\tu32 dups_mask = 0;\n";
-		for (int j = 0; j < P.njoins; j++)
+		for (int j = 0; j < P.njoins; j++) {
 			if (P.joins[j].variant == VH_JOINN)
 				k += "\tif (gload(P.joins[" + std::to_string(j) + "].dups, 0)) dups_mask |= " + std::to_string(1u << j) + "u;\n";
+			if (P.joins[j].hashed)
+				k += "\tif (gload(P.joins[" + std::to_string(j) + "].spill, 0)) dups_mask |= " +
+					std::to_string(1u << (kSpillBit + j)) + "u;\n";
+		}
 		k += "\t(void)dups_mask;\n";
 		k += decls;
 		for (size_t i = 0; i < cols.size(); i++)

@@ -91,10 +91,7 @@ __global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict
 
 	Lds L = lds_layout<MODE>(P, lds);
 	lds_init<MODE>(P, L);
-	u32 dups_mask = 0;
-	for (int j = 0; j < P.njoins; j++)
-		if (*P.joins[j].dups)
-			dups_mask |= 1u << j;
+	const u32 dups_mask = join_flags(P);
 	__syncthreads();
 
 	const i64 tile = (i64)kU * blockDim.x;
@@ -179,8 +176,11 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 	const int P = B.nparts;
 	// the repeated-key flag of this build: set (if at all) by the slice
 	// builds, which run after every chunk of this launch
-	if (chunk == 0 && threadIdx.x == 0)
+	if (chunk == 0 && threadIdx.x == 0) {
 		*B.dups = 0u;
+		if (B.hashed)
+			*B.spill = 0u;
+	}
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
 		hist[p] = 0;
 	__syncthreads();
@@ -222,9 +222,15 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 		off[u] = (u32)(key[u] - B.kmin);
 		pay[u] = (u32)(pv[u] - B.pmin);
 	}
+	// partition of each row: its key range (direct table) or the block of
+	// 2^pshift slots its hash falls in (partitioned hash table)
+	u32 part[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++)
-		rank[u] = ok[u] ? atomicAdd(&hist[off[u] >> B.pshift], 1u) : 0u;
+		part[u] = B.hashed ? (u32)((voila_hash((u64)off[u]) & B.mask) >> B.pshift) : off[u] >> B.pshift;
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		rank[u] = ok[u] ? atomicAdd(&hist[part[u]], 1u) : 0u;
 	__syncthreads();
 	const u64 G = (u64)B.nchunks;
 	for (int p = threadIdx.x; p < P; p += blockDim.x)
@@ -236,7 +242,7 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 #pragma unroll
 	for (int u = 0; u < kU; u++)
 		if (ok[u])
-			staged[hist[off[u] >> B.pshift] + rank[u]] = (u64)off[u] | ((u64)pay[u] << 32);
+			staged[hist[part[u]] + rank[u]] = (u64)off[u] | ((u64)pay[u] << 32);
 	__syncthreads();
 	VH_GLOBAL u64* dst = (VH_GLOBAL u64*)B.pbuf + r0;
 	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
@@ -250,6 +256,103 @@ __global__ __launch_bounds__(kPartBlock) void part_scatter_kernel(const BuildPla
 	__shared__ u64 staged[kPartChunk];
 	__shared__ u32 total;
 	part_scatter_chunk(*Bp, blockIdx.x, hist, staged, &total);
+}
+
+// Linear-probing insert into the overflow table of a partitioned hash table
+// (a row whose sub-table is full); a slot holding the same key marks the
+// build as having repeated keys.
+__device__ __forceinline__ void spill_insert(const BuildPlan& B, u32 off, u64 val)
+{
+	*B.spill = 1u;
+	u64 h = voila_hash((u64)off) & B.mask;
+	for (u64 n = 0;; n++) {
+		if (n >= kMaxProbe || n > B.mask) {
+			gatomic_or(B.error, 32u);
+			return;
+		}
+		const u64 old = gatomic_cas(&B.ovf[h], kEmpty, val);
+		if (old == kEmpty)
+			return;
+		if ((u32)old == off)
+			*B.dups = 1u;
+		h = (h + 1) & B.mask;
+	}
+}
+
+// Where row i of partition p lies in pbuf: the last chunk b with pre[b] <= i
+// (pre = exclusive prefix of the partition's run lengths, base = the run's
+// start minus pre[b]).
+__device__ __forceinline__ u32 part_row_index(const u32* pre, const u32* base, int G, u32 i)
+{
+	int lo = 0, hi = G - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (pre[mid] <= i)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	return base[lo] + i;
+}
+
+// Partition `p` of a partitioned hash table: its rows go into an LDS copy of
+// its block of S = 2^pshift slots with LDS compare-and-swap and linear
+// probing that wraps within the block (the probe kernels follow the same
+// sequence); then the whole block is stored, so the table needs no clearing
+// between runs.  A row that finds the block full goes to the overflow table.
+__device__ __forceinline__ void part_build_hash_slice(const BuildPlan& B, int p, u64* table, u32* pre, u32* base,
+	u32* total_rows)
+{
+	constexpr int V = 2 * kU;
+	const int G = B.nchunks;
+	const u32 S = 1u << B.pshift;
+	for (u32 i = threadIdx.x; i < S; i += blockDim.x)
+		table[i] = kEmpty;
+	for (int b = threadIdx.x; b < G; b += blockDim.x) {
+		pre[b] = gload(B.pcnt, (i64)p * G + b);
+		base[b] = gload(B.poff, (i64)p * G + b);
+	}
+	__syncthreads();
+	const u32 total = block_exclusive_scan(pre, G, total_rows);
+	for (int b = threadIdx.x; b < G; b += blockDim.x)
+		base[b] = (u32)b * kPartChunk + base[b] - pre[b];
+	__syncthreads();
+	bool repeated = false;
+	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
+		u64 v[V];
+#pragma unroll
+		for (int u = 0; u < V; u++) {
+			const u32 i = i0 + u * blockDim.x + threadIdx.x;
+			v[u] = i < total ? gload(B.pbuf, (i64)part_row_index(pre, base, G, i)) : kEmpty;
+		}
+#pragma unroll
+		for (int u = 0; u < V; u++) {
+			if (v[u] == kEmpty)
+				continue;
+			const u32 off = (u32)v[u];
+			u32 h = (u32)voila_hash((u64)off) & (S - 1);
+			bool placed = false;
+			for (u32 n = 0; n < S; n++) {
+				const u64 old = atomicCAS(&table[h], kEmpty, v[u]);
+				if (old == kEmpty) {
+					placed = true;
+					break;
+				}
+				if ((u32)old == off)
+					repeated = true;
+				h = (h + 1) & (S - 1);
+			}
+			if (!placed)
+				spill_insert(B, off, v[u]);
+		}
+	}
+	if (repeated)
+		*B.dups = 1u;
+	__syncthreads();
+	VH_GLOBAL u64* dst = (VH_GLOBAL u64*)B.slots + (u64)p * S;
+	for (u32 i = threadIdx.x; i < S; i += blockDim.x)
+		dst[i] = table[i];
+	__syncthreads();   // the LDS is free for the workgroup's next partition
 }
 
 // Partition `p` of a partitioned build: gather its runs from every chunk,
@@ -334,13 +437,22 @@ __device__ __forceinline__ void part_build_slice(const BuildPlan& B, int p, u32*
 	__syncthreads();   // the LDS is free for the workgroup's next partition
 }
 
+__device__ __forceinline__ void part_slice(const BuildPlan& B, int p, u64* lds_words, u32* pre, u32* base,
+	u32* total_rows)
+{
+	if (B.hashed)
+		part_build_hash_slice(B, p, lds_words, pre, base, total_rows);
+	else
+		part_build_slice(B, p, (u32*)lds_words, pre, base, total_rows);
+}
+
 __global__ __launch_bounds__(kPartBlock) void part_build_kernel(const BuildPlan* __restrict__ Bp)
 {
 	extern __shared__ u64 lds_words[];
 	__shared__ u32 pre[kMaxChunks];
 	__shared__ u32 base[kMaxChunks];
 	__shared__ u32 total_rows;
-	part_build_slice(*Bp, blockIdx.x, (u32*)lds_words, pre, base, &total_rows);
+	part_slice(*Bp, blockIdx.x, lds_words, pre, base, &total_rows);
 }
 
 // Batched partitioned builds: scatter items (build, chunk r0) and slice
@@ -383,7 +495,7 @@ __global__ __launch_bounds__(kPartBlock) void batch_part_slice_kernel(const Buil
 		if (k >= count)
 			break;
 		const BatchItem I = items[first + k];
-		part_build_slice(builds[I.build], (int)I.r0, (u32*)lds_words, pre, base, &total_rows);
+		part_slice(builds[I.build], (int)I.r0, lds_words, pre, base, &total_rows);
 	}
 }
 
@@ -425,6 +537,8 @@ hipError_t launch_batch_part(bool slices, const BuildPlan* d_builds, const Batch
 
 size_t part_slice_lds_bytes(const BuildPlan& B)
 {
+	if (B.hashed)
+		return ((size_t)1 << B.pshift) * sizeof(u64);
 	return (((size_t)1 << B.pshift) * (size_t)B.dir_w + 3) / 4 * 4;
 }
 
@@ -432,7 +546,7 @@ hipError_t launch_part_build(const BuildPlan& B, const BuildPlan* dB, hipStream_
 {
 	if (B.nchunks > 0)
 		hipLaunchKernelGGL(part_scatter_kernel, dim3((unsigned)B.nchunks), dim3(kPartBlock), 0, s, dB);
-	const size_t slice = (((size_t)1 << B.pshift) * (size_t)B.dir_w + 3) / 4 * 4;
+	const size_t slice = part_slice_lds_bytes(B);
 	hipLaunchKernelGGL(part_build_kernel, dim3((unsigned)B.nparts), dim3(kPartBlock), slice, s, dB);
 	return hipGetLastError();
 }

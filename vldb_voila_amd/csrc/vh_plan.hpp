@@ -56,16 +56,24 @@ struct Pred {
 
 // Join table.  Hash layout: open addressing over 64-bit slots, low word =
 // key - kmin, high word = payload - pmin (both < 2^32 - 1), all-ones = empty,
-// linear probing from voila_hash(key - kmin).  Direct layout (dir_w != 0):
+// linear probing from voila_hash(key - kmin) & mask.  The probe sequence
+// wraps within an aligned block of smask + 1 slots: the whole table
+// (smask == mask), or, for a table built partitioned, the sub-table of the
+// key's partition (see BuildPlan); rows that found their sub-table full are
+// in the overflow table `ovf` (capacity mask + 1, whole-table wrap), used
+// only when the build set `spill`.  Direct layout (dir_w != 0):
 // dir[key - kmin] holds payload - pmin in dir_w bytes (all-ones = no row);
 // a key's second and later rows go to the hash slots, which then serve only
 // as its overflow (dups set).
 struct JoinT {
 	u64* slots;
 	u64 mask;
+	u64 smask;
+	u64* ovf;
+	const u32* spill;
 	const void* dir;
 	int dir_w;
-	int pad0;
+	int hashed;          // partitioned hash table (smask < mask, ovf and spill set)
 	Role key;            // probe-side key
 	i64 kmin;
 	i64 kmax;
@@ -86,6 +94,11 @@ struct JoinT {
 };
 
 enum JoinLds { LDS_NONE = 0, LDS_DIR = 1, LDS_BITS = 2 };
+
+// Bits of a probe launch's join flags word: bit j = join j's build saw a
+// repeated key, bit kSpillBit + j = join j's partitioned hash build spilled
+// rows to its overflow table.
+static constexpr int kSpillBit = 16;
 
 struct KeyT {
 	Role r;
@@ -203,7 +216,14 @@ struct BuildPlan {
 	int nchunks;
 	int nparts;
 	int pshift;
-	int pad1;
+	// partitioned hash table (hashed != 0): partition p is the aligned block
+	// of 2^pshift slots a key's voila_hash(key - kmin) & mask falls in; the
+	// slice launch builds each block as an LDS sub-table and stores all of
+	// it; a row that finds its sub-table full goes to `ovf` with a global CAS
+	// and sets *spill
+	int hashed;
+	u64* ovf;
+	u32* spill;
 };
 
 } // namespace vh

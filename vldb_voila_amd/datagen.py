@@ -113,6 +113,42 @@ def gen_join_aggregate(build_rows=1_500_000, probe_rows=6_000_000, seed=42):
     return build, probe
 
 
+def sparse_keys(seed, column, n, bits=31):
+    """n distinct keys over [0, 2^bits): a bijection of [0, 2^bits) (odd
+    multiplies and xorshifts modulo 2^bits) applied to a seeded offset plus
+    0..n-1."""
+    m = np.uint64((1 << bits) - 1)
+    c = np.uint64(mix64_int((seed * GOLDEN + column) & M64) & ((1 << bits) - 1))
+    x = (np.arange(n, dtype=np.uint64) + c) & m
+    with np.errstate(over="ignore"):
+        x = (x * np.uint64(0x2545F491)) & m
+        x ^= x >> np.uint64(bits // 2)
+        x = (x * np.uint64(0x9E3779B1)) & m
+        x ^= x >> np.uint64(bits // 3)
+        x = (x * np.uint64(0x85EBCA6B)) & m
+    return x.astype(np.int64)
+
+
+def gen_join_sparse(build_rows=1_500_000, probe_rows=6_000_000, seed=42):
+    """The join of gen_join_aggregate with build keys drawn without
+    repetition over [0, 2^31) instead of 1..build_rows (a key range far too
+    wide for a direct table: the join takes a hash table); probe fks are
+    uniform over the build keys."""
+    keys = sparse_keys(seed, 17, build_rows).astype(np.int32)
+    build = {
+        "key": keys,
+        "nation": uniform_int(seed, 12, build_rows, 0, 25, np.int32),
+        "segment": uniform_int(seed, 13, build_rows, 0, 5, np.int32),
+    }
+    pick = uniform_int(seed, 14, probe_rows, 0, build_rows, np.int64)
+    probe = {
+        "fk": keys[pick],
+        "date": uniform_int(seed, 15, probe_rows, SPAN_START, SPAN_START + SPAN_DAYS, np.int32),
+        "price": uniform_int(seed, 16, probe_rows, 90000, 10500000, np.int64),
+    }
+    return build, probe
+
+
 def _years_of_days(days: np.ndarray) -> np.ndarray:
     """Calendar year of Julian day numbers (proleptic Gregorian)."""
     a = days.astype(np.int64) + 32044

@@ -162,8 +162,11 @@ class Event:
             pass
 
 
-def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0, unroll=0, merge=0):
+def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0, unroll=0, merge=0,
+                part_shift=0, block=0):
     c = abi.vh_config()
+    c.part_shift = part_shift
+    c.block = block
     c.group_mode = group_mode
     c.grid = grid
     c.table_shift = table_shift
