@@ -58,6 +58,8 @@ def parse():
                     default="join")
     ap.add_argument("--join-layout", type=int, default=0, help="vh_config.join_layout (0 = auto)")
     ap.add_argument("--part-shift", type=int, default=0, help="vh_config.part_shift (0 = auto, -1 = none)")
+    ap.add_argument("--cfg", action="append", default=[],
+                    help="another vh_config field as name=value (grid, unroll, block, merge, table_shift, ...)")
     ap.add_argument("--zipf", type=float, default=0.0, help="star: Zipf theta of the part FKs")
     ap.add_argument("--copies", type=int, default=0, help="input copies rotated (0 = auto)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
@@ -184,9 +186,10 @@ def main():
     for c in range(copies):
         dev = [runtime.upload(t) for t in W["tables"]]
         bufs.append(dev)
+        extra = {k: int(v) for k, v in (kv.split("=", 1) for kv in args.cfg)}
         queries.append(runtime.Query(W["make"]([runtime.addrs(d) for d in dev]).desc(),
                                      runtime.make_config(join_layout=args.join_layout,
-                                                         part_shift=args.part_shift)))
+                                                         part_shift=args.part_shift, **extra)))
     q0 = queries[0]
     info = q0.info()
     kernel, _ = q0.kernel()
@@ -246,7 +249,8 @@ def main():
                         "grid": info["grid"],
                         "join_tables": [dict(zip(("entry_bytes", "entries"), q0.join(j)))
                                         for j in range(q0.desc.njoins)],
-                        "probe_kernel": "generated (hipRTC)" if kernel == 2 else "generic"}, **W["config"]),
+                        "probe_kernel": "generated (hipRTC)" if kernel == 2 else "generic",
+                        "radix_probe": "vh_generated_join" in q0.kernel()[1]}, **W["config"]),
         "roofline": {
             "bound": "hbm",
             "kernel": ("vh_generated" if kernel == 2 else "vh::pipeline_kernel") +

@@ -223,6 +223,15 @@ typedef struct vh_config {
 	int32_t block;         /* threads per workgroup of the generated probe
 	                          kernel (waves per CU = block / 64 x workgroups
 	                          per CU): 0 = auto, 256, 512 or 1024            */
+	int32_t radix;         /* radix-partitioned probe of a last join whose
+	                          hash table is built partitioned: the probe
+	                          writes its rows' tuples partitioned, then one
+	                          workgroup per partition joins them against the
+	                          partition's sub-table in LDS.  1 = on when
+	                          it applies; 0 (auto) and -1 = off: the
+	                          one-pass probe of the partitioned table was
+	                          faster on every shape measured                */
+	int32_t reserved;
 } vh_config;
 
 /* ------------------------------------------------------------------------ */

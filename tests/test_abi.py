@@ -32,7 +32,7 @@ def test_descriptor_sizes():
     assert C.sizeof(abi.vh_pred) == 16
     assert C.sizeof(abi.vh_expr) == 24
     assert C.sizeof(abi.vh_table_desc) == 16 + abi.VH_MAX_COLS * 32 + abi.VH_MAX_PREDS * 16
-    assert C.sizeof(abi.vh_config) == 40
+    assert C.sizeof(abi.vh_config) == 48
 
 
 def test_type_from_minmax_matches_oracle():

@@ -98,6 +98,42 @@ def test_partitioned_hash_table_source():
         assert rc == 0 and size > 0, log[:2000]
 
 
+def split_radix(src):
+    i = src.index("// generated join kernel")
+    return src[:i], src[i:]
+
+
+def test_radix_probe_sources():
+    """A last join over a partitioned hash table with radix = 1: a probe
+    kernel that writes partitioned tuples and a join kernel over LDS
+    sub-tables, for every group-by strategy it takes, with the carried values
+    (a probe column, an earlier join's payload)."""
+    import numpy as np
+    b, p = datagen.gen_join_sparse(600_000, 6000, seed=5)
+    d = workloads.join_aggregate_pipeline(b, p).desc()
+    assert "vh_generated_join" not in source(d)
+    probe, join = split_radix(source(d, radix=1))
+    assert "radix_slot<2>" in probe and "radix_finish<2>" in probe
+    assert "vh_generated_join" in join and "tab[h[u]]" in join
+    rng = np.random.default_rng(2)
+    dim = {"k": np.arange(1, 101, dtype=np.int32), "n": rng.integers(0, 7, 100).astype(np.int16)}
+    fact = {"a": rng.integers(1, 101, 500).astype(np.int32), "fk": p["fk"][:500],
+            "v": rng.integers(-5, 5, 500).astype(np.int8)}
+    pl = Pipeline(table_from_arrays("f", fact))
+    pl.join(table_from_arrays("d1", dim), "k", "a", payload=["n"], names=["n1"])
+    pl.join(table_from_arrays("r", b), "key", "fk", payload=["segment"], names=["seg"])
+    pl.groupby(["n1", "seg"], [("sum", col("v") * (col("n1") + 3)), ("count",)])
+    d2 = pl.desc()
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
+        for src in (source(d, group_mode=mode, radix=1), source(d2, group_mode=mode, radix=1)):
+            probe, join = split_radix(src)
+            for part in (probe, join):
+                rc, log, size = compile_for_gfx950(part)
+                assert rc == 0 and size > 0, (mode, log[:2000])
+    probe, join = split_radix(source(d2, radix=1))
+    assert "radix_slot<3>" in probe and "pay0[u]" in join
+
+
 def test_source_reflects_the_plan():
     b, p = datagen.gen_join_aggregate(2000, 6000, seed=5)
     d = workloads.join_aggregate_pipeline(b, p).desc()

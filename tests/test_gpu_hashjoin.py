@@ -23,15 +23,17 @@ def on_gpu(arrays_list, make):
     return make(*[runtime.addrs(d) for d in devs]), devs
 
 
-@pytest.mark.parametrize("kernel,part", [(abi.VH_KERNEL_AUTO, 0), (abi.VH_KERNEL_GENERIC, 0),
-                                         (abi.VH_KERNEL_AUTO, -1)])
-def test_sparse_join_full_size(kernel, part):
-    print(f"sparse join 1.5M/6M seed 42 kernel {kernel} part_shift {part}", flush=True)
+@pytest.mark.parametrize("kernel,part,radix", [(abi.VH_KERNEL_AUTO, 0, 0), (abi.VH_KERNEL_AUTO, 0, 1),
+                                               (abi.VH_KERNEL_GENERIC, 0, 1), (abi.VH_KERNEL_AUTO, -1, 1)])
+def test_sparse_join_full_size(kernel, part, radix):
+    print(f"sparse join 1.5M/6M seed 42 kernel {kernel} part_shift {part} radix {radix}", flush=True)
     b, pr = datagen.gen_join_sparse(1_500_000, 6_000_000, seed=42)
     want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, pr).desc()))
     p, devs = on_gpu([b, pr], lambda x, y: workloads.join_aggregate_pipeline(b, pr, x, y))
-    q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part))
+    q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part, radix=radix))
     assert q.join(0)[0] == 8                 # a hash table, not a direct one
+    src = q.kernel()[1]
+    assert ("vh_generated_join" in src) == (kernel == abi.VH_KERNEL_AUTO and part == 0 and radix == 1)
     for rep in range(3):
         q.run()
         assert oracle.sorted_rows(q.result()) == want
@@ -111,10 +113,39 @@ def test_partitioned_hash_table_spills(part):
             .groupby(["pay"], [("sum", col("v")), ("count",)])
     want = oracle.sorted_rows(oracle.bruteforce(make().desc()))
     p, devs = on_gpu([build, probe], make)
-    for kernel in (abi.VH_KERNEL_AUTO, abi.VH_KERNEL_GENERIC):
-        q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part))
+    for kernel, radix in ((abi.VH_KERNEL_AUTO, 0), (abi.VH_KERNEL_AUTO, 1), (abi.VH_KERNEL_GENERIC, 0)):
+        q = runtime.Query(p.desc(), runtime.make_config(kernel=kernel, part_shift=part, radix=radix))
         w, n = q.join(0)
         assert w == 8 and n * 8 > 4 << 20
         for rep in range(3):
             q.run()
             assert oracle.sorted_rows(q.result()) == want
+
+
+@pytest.mark.parametrize("mode", [abi.VH_GROUP_AUTO, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH,
+                                  abi.VH_GROUP_LDSHASH])
+def test_radix_probe_after_a_direct_join(mode):
+    """select -> direct join (payload used by the group-by) -> sparse hash
+    join (radix-partitioned probe): the tuples carry a probe column and the
+    first join's payload to the join kernel."""
+    print(f"radix probe after a direct join, mode {mode}", flush=True)
+    rng = np.random.default_rng(33)
+    b, pr = datagen.gen_join_sparse(700_000, 2_000_000, seed=33)
+    dim = {"k": np.arange(1, 1001, dtype=np.int32), "n": rng.integers(0, 9, 1000).astype(np.int32)}
+    fact = {"a": rng.integers(1, 1101, 2_000_000).astype(np.int32), "fk": pr["fk"], "date": pr["date"],
+            "v": rng.integers(-10**5, 10**5, 2_000_000).astype(np.int64)}
+
+    def make(x=None, y=None, z=None):
+        f = table_from_arrays("f", fact, z)
+        pl = Pipeline(f).select("date", "<=", workloads.JOIN_DATE_CUTOFF)
+        pl.join(table_from_arrays("d", dim, x), "k", "a", payload=["n"], names=["n"])
+        pl.join(table_from_arrays("r", b, y), "key", "fk", payload=["segment"], names=["seg"])
+        return pl.groupby(["n", "seg"], [("sum", col("v") * (col("n") + 1)), ("sum", col("v")), ("count",)])
+    want = oracle.sorted_rows(oracle.bruteforce(make().desc()))
+    devs = [runtime.upload(t) for t in (dim, b, fact)]
+    p = make(*[runtime.addrs(d) for d in devs])
+    q = runtime.Query(p.desc(), runtime.make_config(group_mode=mode, radix=1))
+    assert "vh_generated_join" in q.kernel()[1]
+    for rep in range(2):
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want

@@ -99,4 +99,4 @@ class vh_config(C.Structure):
     _fields_ = [("group_mode", C.c_int32), ("grid", C.c_int32), ("table_shift", C.c_int32),
                 ("tile_shift", C.c_int32), ("join_layout", C.c_int32), ("kernel", C.c_int32),
                 ("unroll", C.c_int32), ("merge", C.c_int32), ("part_shift", C.c_int32),
-                ("block", C.c_int32)]
+                ("block", C.c_int32), ("radix", C.c_int32), ("reserved", C.c_int32)]

@@ -104,6 +104,7 @@ static int lower_for_batch(int i, const vh_pipeline_desc* d, const vh_config& cf
 	vh_config lc = cfg;
 	if (lc.group_mode == VH_GROUP_REG)
 		lc.group_mode = VH_GROUP_LDS;
+	lc.radix = -1;   // batched probe items run the one-pass kernels
 	int rc = lower(d, &lc, L);
 	if (rc)
 		return fail(rc, "pipeline %d: %s", i, last_error());

@@ -226,7 +226,9 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 	const int kernel = cfg ? cfg->kernel : VH_KERNEL_AUTO;
 	if (kernel != VH_KERNEL_GENERIC)
 		choose_probe_shape(L, desc, cfg);
-	const std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel), L.block);
+	std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel), L.block);
+	if (L.plan.radix)
+		src += jit_join_source(L.plan, jit_unroll(cfg), 256);
 	*len = src.size();
 	if (buf) {
 		if (cap <= src.size())
@@ -252,7 +254,8 @@ int vh_query_kernel(const vh_query* q, int32_t* kernel, char* buf, size_t cap, s
 {
 	if (!q)
 		return fail(VH_ERR_INVALID, "null query");
-	const std::string& src = q->q.jit_src;
+	// a radix-partitioned probe runs two generated kernels: both sources
+	const std::string src = q->q.jit_src + q->q.jit_join_src;
 	if (kernel)
 		*kernel = q->q.jit_fn ? VH_KERNEL_JIT : VH_KERNEL_GENERIC;
 	if (len)

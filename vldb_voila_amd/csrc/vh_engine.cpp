@@ -583,6 +583,18 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 			(long long)P.rows);
 	if (mode < GB_REG || (mode > GB_LDSHASH && mode != GB_PART))
 		return fail(VH_ERR_INVALID, "group mode %d", mode);
+	// radix-partitioned probe: the last join's table is built partitioned
+	// and the group-by keeps its state per workgroup or in global memory
+	// (the join kernel's workgroups are partitions, not row slabs)
+	const int rcfg = cfg ? cfg->radix : 0;
+	if (rcfg < -1 || rcfg > 1)
+		return fail(VH_ERR_INVALID, "radix %d", rcfg);
+	P.radix = 0;
+	if (rcfg == 1 && P.njoins > 0 && L.builds[P.njoins - 1].hashed && mode != GB_PART) {
+		P.radix = 1;
+		if (mode == GB_REG)
+			mode = GB_LDS;
+	}
 	P.mode = mode;
 	if (mode == GB_HASH) {
 		// groups <= min(domain, tuples); tuples <= probe rows unless a JoinN fans out
@@ -610,9 +622,16 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	// MI355X; each workgroup pays a fixed start-up and partials write), the
 	// generic kernel with 4.
 	int grid = cfg && cfg->grid > 0 ? cfg->grid : 0;
+	bool hash_join = false;
+	for (int j = 0; j < P.njoins; j++)
+		hash_join = hash_join || !L.builds[j].dir_w;
 	if (!grid) {
 		int64_t want = (P.rows + 1023) / 1024;
-		const int per_cu = (cfg && cfg->kernel == VH_KERNEL_GENERIC) ? 4 : 2;
+		// lookups in a hash table too large for L2 wait on the Infinity
+		// Cache: twice the workgroups hide more of that latency (sparse-key
+		// join, 1.5M x 6M: probe 86 us at 2 per CU, 79 at 4, both 4 rows
+		// per lane)
+		const int per_cu = (cfg && cfg->kernel == VH_KERNEL_GENERIC) || hash_join ? 4 : 2;
 		grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * device_cus()));
 	}
 	if (P.mode == GB_PART) {
@@ -637,7 +656,7 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	if (merge < 0 || merge > 2)
 		return fail(VH_ERR_INVALID, "merge %d", merge);
 	if (P.mode == GB_REG || P.mode == GB_LDS)
-		P.flush = merge == 2 || (merge == 0 && (double)grid * domain * (P.nsums + 1) <= 16384.0);
+		P.flush = merge == 2 || (merge == 0 && (double)grid * domain * (P.nsums + 1) <= 16384.0) || P.radix;
 	L.build_grid.assign(d->njoins, 0);
 	for (int j = 0; j < d->njoins; j++)
 		L.build_grid[j] = (int)std::max<int64_t>(1, std::min<int64_t>((d->joins[j].build.rows + 1023) / 1024, 4096));
@@ -885,6 +904,41 @@ int query_create(Query* q, bool single)
 		}
 	}
 
+	// radix-partitioned probe: generated kernels only; its tuple space, run
+	// tables and both launches' LDS
+	if (P.radix && !(single && q->cfg.kernel != VH_KERNEL_GENERIC))
+		P.radix = 0;
+	if (P.radix) {
+		const JoinT& J = P.joins[P.njoins - 1];
+		const u64 S = J.smask + 1;
+		const u64 NP = (J.mask + 1) / S;
+		const u64 R = (u64)L.grid;
+		const size_t lds_max = (size_t)std::min(device_lds_per_block(), 160 * 1024);
+		P.rw = jit_radix_words(P);
+		P.rregions = (int)R;
+		P.rlds_a = (u32)((pipeline_lds_bytes(P) + 15) & ~(size_t)15);
+		P.rlds_b = (u32)((groupby_lds_bytes(P) + 15) & ~(size_t)15);
+		q->radix_lds_a = P.rlds_a + (2 * NP + 2) * sizeof(u32);
+		q->radix_lds_b = P.rlds_b + S * sizeof(u64) + (2 * R + 2) * sizeof(u32);
+		// a workgroup's rows: at most one tile per grid stride of the
+		// generated loops plus the last partial chunk (tile bound: 16 rows
+		// per lane)
+		const u64 tile = 16 * (u64)L.block;
+		const u64 rows = (u64)std::max<int64_t>(P.rows, 1);
+		P.rcap = (i64)(((rows + R * tile - 1) / (R * tile) + 2) * tile);
+		if (P.rw > kRadixMaxWords || q->radix_lds_a > lds_max || q->radix_lds_b > lds_max || R > 4096 ||
+			NP > (u64)kMaxBuildParts || R * (u64)P.rcap >= ((u64)1 << 32)) {
+			P.radix = 0;
+		} else {
+			const size_t tup = (size_t)(R * (u64)P.rcap * (u64)P.rw * sizeof(u64));
+			if ((rc = q->alloc((void**)&P.rtup0, tup)) || (rc = q->alloc((void**)&P.rtup1, tup)) ||
+				(rc = q->alloc((void**)&P.rcnt, NP * R * sizeof(u32))) ||
+				(rc = q->alloc((void**)&P.roff, NP * R * sizeof(u32))))
+				return rc;
+			q->radix_parts = (int)NP;
+		}
+	}
+
 	// device copies of the plans
 	if ((rc = q->alloc((void**)&q->dplan, sizeof(Plan))))
 		return rc;
@@ -904,6 +958,20 @@ int query_create(Query* q, bool single)
 		q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel), L.block);
 		std::string log;
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
+		if (!rc && P.radix) {
+			q->jit_join_src = jit_join_source(P, jit_unroll(&q->cfg), 256);
+			rc = jit_compile(q->jit_join_src, &q->jit_join_fn, log, "vh_generated_join");
+			if (rc && kernel == VH_KERNEL_AUTO) {
+				// the one-pass probe computes the same result
+				P.radix = 0;
+				q->jit_join_fn = nullptr;
+				q->jit_join_src.clear();
+				q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel), L.block);
+				rc = jit_compile(q->jit_src, &q->jit_fn, log);
+				if (!rc)
+					HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
+			}
+		}
 		if (rc && kernel != VH_KERNEL_AUTO)
 			return rc;
 		if (rc) {
@@ -912,6 +980,9 @@ int query_create(Query* q, bool single)
 			// shape goes back to what the lowering chose for it.
 			q->jit_fn = nullptr;
 			q->jit_src.clear();
+			q->jit_join_fn = nullptr;
+			q->jit_join_src.clear();
+			P.radix = 0;
 			for (int j = 0; j < P.njoins; j++) {
 				P.joins[j].lds = LDS_NONE;
 				P.joins[j].lds_off = 0;
@@ -958,7 +1029,14 @@ int query_run(Query* q, hipStream_t s, int phase)
 					q->bits[j], s));
 	}
 	if (phase == -1 || phase == 2) {
-		if (q->jit_fn)
+		if (q->jit_join_fn) {
+			// radix-partitioned probe: partitioned tuples, then one join
+			// workgroup per partition (the stamps span both launches)
+			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s, q->probe_start, nullptr,
+				(long)q->radix_lds_a));
+			HIP_TRY(launch_generated(q->jit_join_fn, P, q->dplan, q->radix_parts, 256, s, nullptr, q->probe_stop,
+				(long)q->radix_lds_b));
+		} else if (q->jit_fn)
 			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s, q->probe_start, q->probe_stop));
 		else
 			HIP_TRY(launch_pipeline(P, q->dplan, L.grid, s));

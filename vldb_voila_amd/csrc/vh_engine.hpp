@@ -67,6 +67,12 @@ struct Query {
 	hipEvent_t probe_start = nullptr;  // timestamps of the generated kernel's own start and end
 	hipEvent_t probe_stop = nullptr;
 	std::string jit_src;
+	// radix-partitioned probe (Plan::radix): the join kernel after the probe
+	// kernel, its source, and both launches' LDS bytes and the join grid
+	hipFunction_t jit_join_fn = nullptr;
+	std::string jit_join_src;
+	size_t radix_lds_a = 0, radix_lds_b = 0;
+	int radix_parts = 0;
 	~Query();
 	int alloc(void** p, size_t bytes);
 };

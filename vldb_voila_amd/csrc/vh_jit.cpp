@@ -82,6 +82,9 @@ struct Gen {
 	int late_stage = 0; // vector loop: stage before which late columns are loaded (0: none)
 	int block = 256;    // threads per workgroup
 	bool batch = false; // entry point of a batched launch (vh_generated_batch)
+	bool radix = false; // P.radix: the probe kernel ends at the last join with
+	                    // partitioned tuples (vh_generated), the join kernel
+	                    // (vh_generated_join) takes them from there
 	std::vector<Col> cols;
 	// plan fields the kernel reads, copied into locals before the row loop
 	// (the loop's atomics would otherwise make the compiler reload them)
@@ -110,6 +113,42 @@ struct Gen {
 		uni[expr] = name;
 		decls += "\t" + type + " const " + name + " = " + expr + ";\n";
 		return name;
+	}
+
+	// Values a radix probe tuple carries after its key offset: every probe
+	// column and earlier join payload the group keys and sum arguments read
+	// (the last join's payload comes from the join kernel's lookup).  The
+	// tuple's word k + 1 holds value k.
+	std::vector<std::pair<Role, std::string>> radix_values() const
+	{
+		std::vector<std::pair<Role, std::string>> v;
+		const int last = P.njoins - 1;
+		auto add = [&](const Role& r, const std::string& path) {
+			if (r.kind == ROLE_PAY && r.slot == last)
+				return;
+			if (r.kind != ROLE_COL && r.kind != ROLE_PAY)
+				return;
+			for (auto& x : v)
+				if (x.first.kind == r.kind && (r.kind == ROLE_PAY ? x.first.slot == r.slot
+					: (x.first.ptr == r.ptr && x.first.type == r.type)))
+					return;
+			v.push_back({r, path});
+		};
+		char b[96];
+		for (int k = 0; k < P.nkeys; k++) {
+			snprintf(b, sizeof b, "P.keys[%d].r", k);
+			add(P.keys[k].r, b);
+		}
+		for (int si = 0; si < P.nsums; si++)
+			for (int t = 0; t < P.sums[si].nterms; t++) {
+				const Term& T = P.sums[si].t[t];
+				snprintf(b, sizeof b, "P.sums[%d].t[%d]", si, t);
+				if (T.sa)
+					add(T.a, std::string(b) + ".a");
+				if (T.sb)
+					add(T.b, std::string(b) + ".b");
+			}
+		return v;
 	}
 
 	int col_index(const Role& r, const std::string& path, bool pred, int stage = 1 << 20)
@@ -410,6 +449,32 @@ struct Gen {
 			line("\t\t\t}");
 			if (!scalar && j == last && late_stage == P.njoins + 1)
 				emit_late();   // aggregation columns, before this join's lookups (and the next chunk's loads)
+			if (radix && j == last) {
+				// radix-partitioned probe: the row's tuple goes to its
+				// partition; the join kernel does the lookup and the group-by
+				emit_next();
+				const std::string qs = uniform("int", "(31 - __builtin_clz((u32)(" + jp + ".smask + 1)))");
+				const auto vals = radix_values();
+				snprintf(b, sizeof b, "%d", (int)vals.size() + 1);
+				const std::string W = b;
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) {");
+				line("\t\t\t\tif (!VH_LIVE(u)) continue;");
+				line("\t\t\t\tconst u32 q = (u32)((voila_hash((u64)off[u]) & " + mask + ") >> " + qs + ");");
+				line("\t\t\t\tVH_GLOBAL u64* t = radix_slot<" + W + ">(P, rc, rmeta, q);");
+				line("\t\t\t\tif (!t) continue;");
+				line("\t\t\t\tt[0] = (u64)off[u] | ((u64)q << 32);");
+				for (size_t k = 0; k < vals.size(); k++) {
+					if (vals[k].first.kind == ROLE_PAY)
+						snprintf(b, sizeof b, "\t\t\t\tt[%zu] = (u64)pay%d[u];", k + 1, vals[k].first.slot);
+					else
+						snprintf(b, sizeof b, "\t\t\t\tt[%zu] = (u64)%s;", k + 1, role(vals[k].first, vals[k].second).c_str());
+					line(b);
+				}
+				line("\t\t\t}");
+				line("\t\t}");
+				continue;
+			}
 			if (J.dir_w) {
 				const char* et = J.dir_w == 1 ? "uint8_t" : (J.dir_w == 2 ? "uint16_t" : "uint32_t");
 				const char* ev = J.dir_w == 1 ? "0xFFu" : (J.dir_w == 2 ? "0xFFFFu" : "0xFFFFFFFFu");
@@ -444,12 +509,20 @@ struct Gen {
 				line("\t\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + mask + ";");
 				line("#pragma unroll");
 				line("\t\t\tfor (int u = 0; u < U; u++) s[u] = " + slots + "[h[u]];");
+				// the next slot of each sequence is requested with the first:
+				// a collision then costs no second round trip (the two slots
+				// share a line but in one case of 16)
+				line("\t\t\tu64 s2[U];");
+				line("#pragma unroll");
+				line("\t\t\tfor (int u = 0; u < U; u++) { const u64 h2 = " + std::string(J.hashed ? "hnext(h[u], " +
+					uniform("u64", jp + ".smask") + ")" : "(h[u] + 1) & " + mask) + "; s2[u] = " + slots + "[h2]; }");
 				line("#pragma unroll");
 				line("\t\t\tfor (int u = 0; u < U; u++) {");
 				snprintf(b, sizeof b, "\t\t\t\tpay%d[u] = 0;", j);
 				line(b);
 				line("\t\t\t\tif (!VH_LIVE(u)) continue;");
 				line("\t\t\t\tu64 n = 0;");
+				line("\t\t\t\tif (s[u] != kEmpty && (u32)s[u] != off[u]) { n = 1; h[u] = " + step + "; s[u] = s2[u]; }");
 				// a partitioned table's sub-table may be full (its other rows
 				// spilled): one full cycle of it ends the walk
 				const std::string lim = J.hashed ? "(" + uniform("u64", jp + ".smask") + " + 1)" : "kMaxProbe";
@@ -614,11 +687,16 @@ struct Gen {
 			vec = vec && ((uintptr_t)c.ptr % (uintptr_t)(2 * host_width(c.type))) == 0;
 			row_bytes += host_width(c.type);
 		}
+		bool hash_join = false;
+		for (int j = 0; j < P.njoins; j++)
+			hash_join = hash_join || !P.joins[j].dir_w;
 		if (U == 0) {
 			// measured on MI355X: 8 rows per lane for 16 (c1 probe) and 32
 			// bytes (star fact) of columns per row, 2 for 36 (filter); 4 in
-			// 1024-thread workgroups (star: 344 us against 351 with 8)
-			U = vec ? (block > 256 ? 4 : (row_bytes <= 32 ? 8 : 2)) : 4;
+			// 1024-thread workgroups (star: 344 us against 351 with 8) and
+			// with hash-table lookups, which keep more registers per row
+			// (sparse-key join: 86 us at 4 against 98 at 8)
+			U = vec ? (block > 256 || hash_join ? 4 : (row_bytes <= 32 ? 8 : 2)) : 4;
 		}
 		// Late materialization in the vector loop: columns first used at or
 		// after late_stage are loaded there, for live rows only.  Asked for
@@ -749,8 +827,19 @@ struct Gen {
 				") void vh_generated(const Plan* __restrict__ Pp)\n{\n";
 			k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
 		}
-		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
-		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
+		if (radix) {
+			// partition counts (NP), cursors (NP), the region's tuple count
+			// and a scan total, after any join tables in LDS
+			const std::string jl = "P.joins[" + std::to_string(P.njoins - 1) + "]";
+			k += "\tconst int NP = (int)((" + jl + ".mask + 1) / (" + jl + ".smask + 1));\n";
+			k += "\tu32* const rc = (u32*)((char*)lds + P.rlds_a);\n";
+			k += "\tu32* const rcur = rc + NP;\n\tu32* const rmeta = rcur + NP;\n";
+			k += "\tfor (int i = threadIdx.x; i < NP; i += blockDim.x) rc[i] = 0;\n";
+			k += "\tif (threadIdx.x == 0) rmeta[0] = 0;\n";
+		} else {
+			k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
+			k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
+		}
 		// join tables kept in LDS: copied by every workgroup (16-byte units;
 		// dir[] and the bitmap are allocated in whole units)
 		for (int j = 0; j < P.njoins; j++) {
@@ -803,9 +892,169 @@ struct Gen {
 			else if (mode == GB_PART)
 				k += "\tpart_finish(P, L);\n";
 			k += "\t__syncthreads();\n\t}\n}\n";
+		} else if (radix) {
+			k += "\tradix_finish<" + std::to_string(radix_values().size() + 1) + ">(P, rc, rcur, rmeta, NP);\n}\n";
 		} else {
 			k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
 		}
+		return k;
+	}
+
+	// The join kernel of a radix-partitioned probe: workgroup p copies
+	// partition p's sub-table of the last join into LDS, then takes the
+	// partition's tuples (U per lane, from every region's run), looks each
+	// key up in LDS (rows of a full sub-table: in the overflow table) and
+	// aggregates it as the one-pass kernel would; at its end the LDS
+	// group-by state is added to the global arrays.
+	std::string join_source(int mode)
+	{
+		char b[1024];
+		const int last = P.njoins - 1;
+		const JoinT& J = P.joins[last];
+		snprintf(b, sizeof b, "P.joins[%d]", last);
+		const std::string jp = b;
+		const std::string err = uniform("u32*", "P.error");
+		const std::string mask = uniform("u64", jp + ".mask");
+		const std::string sm = uniform("u64", jp + ".smask");
+		const std::string ovf = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".ovf");
+		const auto vals = radix_values();
+		const int W = (int)vals.size() + 1;
+		if (U <= 0)
+			U = 4;
+		std::vector<int> ci(vals.size(), -1);
+		for (size_t k = 0; k < vals.size(); k++)
+			if (vals[k].first.kind == ROLE_COL)
+				ci[k] = col_index(vals[k].first, vals[k].second, false);
+		line("\tfor (u32 i0 = 0; i0 < rtotal; i0 += U * blockDim.x) {");
+		line("\t\tu32 live = 0;");
+		line("\t\tu32 off[U];");
+		for (size_t k = 0; k < vals.size(); k++) {
+			if (vals[k].first.kind == ROLE_PAY)
+				snprintf(b, sizeof b, "\t\tu32 pay%d[U];", vals[k].first.slot);
+			else
+				snprintf(b, sizeof b, "\t\t%s c%d[U];", ctype_of(vals[k].first.type), ci[k]);
+			line(b);
+		}
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) {");
+		line("\t\t\tconst u32 i = i0 + u * blockDim.x + threadIdx.x;");
+		line("\t\t\tlive |= (u32)(i < rtotal) << u;");
+		snprintf(b, sizeof b, "\t\t\tconst VH_GLOBAL u64* t = rt + (i < rtotal ? radix_tuple_index(rpre, rbase, R, i) : 0ull) * %du;", W);
+		line(b);
+		line("\t\t\toff[u] = (u32)t[0];");
+		for (size_t k = 0; k < vals.size(); k++) {
+			if (vals[k].first.kind == ROLE_PAY)
+				snprintf(b, sizeof b, "\t\t\tpay%d[u] = (u32)t[%zu];", vals[k].first.slot, k + 1);
+			else
+				snprintf(b, sizeof b, "\t\t\tc%d[u] = (%s)(i64)t[%zu];", ci[k], ctype_of(vals[k].first.type), k + 1);
+			line(b);
+		}
+		line("\t\t}");
+		// lookups in the LDS sub-table (local slot = global slot & smask)
+		line("\t\tu32 payl[U];");
+		line("\t\tu64 h[U], s[U];");
+		line("\t\tu32 ovf_first = 0;");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + sm + ";");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) s[u] = tab[h[u]];");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) {");
+		line("\t\t\tpayl[u] = 0;");
+		line("\t\t\tif (!VH_LIVE(u)) continue;");
+		line("\t\t\tu64 n = 0;");
+		line("\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n <= " + sm + ") {");
+		line("\t\t\t\tn++;");
+		line("\t\t\t\th[u] = (h[u] + 1) & " + sm + ";");
+		line("\t\t\t\ts[u] = tab[h[u]];");
+		line("\t\t\t}");
+		snprintf(b, sizeof b, "\t\t\tif ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> %d) & 1u)) {", kSpillBit + last);
+		line(b);
+		line("\t\t\t\tovf_first |= 1u << u;");
+		line("\t\t\t\th[u] = voila_hash((u64)off[u]) & " + mask + ";");
+		line("\t\t\t\ts[u] = " + ovf + "[h[u]];");
+		line("\t\t\t\tfor (n = 0; s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe; n++) {");
+		line("\t\t\t\t\th[u] = (h[u] + 1) & " + mask + ";");
+		line("\t\t\t\t\ts[u] = " + ovf + "[h[u]];");
+		line("\t\t\t\t}");
+		line("\t\t\t\tif (n >= kMaxProbe) gatomic_or(" + err + ", 32u);");
+		line("\t\t\t}");
+		line("\t\t\tif (s[u] == kEmpty || (u32)s[u] != off[u]) VH_KILL(u, 1);");
+		line("\t\t\telse payl[u] = (u32)(s[u] >> 32);");
+		line("\t\t}");
+		line("\t\tauto consume = [&](int u, u32 tp) __attribute__((always_inline)) {");
+		out += consume_body(mode, last, "tp");
+		line("\t\t};");
+		line("#pragma unroll");
+		line("\t\tfor (int u = 0; u < U; u++) if (VH_LIVE(u)) consume(u, payl[u]);");
+		if (J.variant == VH_JOINN) {
+			// further matches: the rest of the sub-table's sequence (it ends
+			// where it began when the sub-table is full), then the overflow
+			// table after a spill
+			snprintf(b, sizeof b, "\t\tif ((dups_mask >> %d) & 1u) {", last);
+			line(b);
+			line("#pragma unroll");
+			line("\t\t\tfor (int u = 0; u < U; u++) {");
+			line("\t\t\t\tif (!VH_LIVE(u)) continue;");
+			line("\t\t\t\tbool ov = (ovf_first >> u) & 1u;");
+			line("\t\t\t\tconst u64 h0 = voila_hash((u64)off[u]) & " + sm + ";");
+			line("\t\t\t\tu64 hh = ov ? (h[u] + 1) & " + mask + " : (h[u] + 1) & " + sm + ";");
+			line("\t\t\t\tfor (u64 n = 0;; n++) {");
+			line("\t\t\t\t\tif (n >= kMaxProbe) { gatomic_or(" + err + ", 32u); break; }");
+			line("\t\t\t\t\tconst u64 sv = ov ? " + ovf + "[hh] : (hh == h0 ? kEmpty : tab[hh]);");
+			line("\t\t\t\t\tif (sv == kEmpty) {");
+			snprintf(b, sizeof b, "\t\t\t\t\t\tif (!ov && ((dups_mask >> %d) & 1u)) {", kSpillBit + last);
+			line(b);
+			line("\t\t\t\t\t\t\tov = true;");
+			line("\t\t\t\t\t\t\thh = voila_hash((u64)off[u]) & " + mask + ";");
+			line("\t\t\t\t\t\t\tcontinue;");
+			line("\t\t\t\t\t\t}");
+			line("\t\t\t\t\t\tbreak;");
+			line("\t\t\t\t\t}");
+			line("\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
+			line("\t\t\t\t\thh = ov ? (hh + 1) & " + mask + " : (hh + 1) & " + sm + ";");
+			line("\t\t\t\t}");
+			line("\t\t\t}");
+			line("\t\t}");
+		}
+		line("\t}");
+		const std::string loop = out;
+
+		std::string k;
+		k += "// generated join kernel of a radix-partitioned probe\n";
+		for (const char* t : {"int8_t", "uint8_t", "int16_t", "uint16_t", "int32_t", "uint32_t", "int64_t", "uint64_t"})
+			k += std::string("typedef __hip_internal::") + t + " " + t + ";\n";
+		k += "#include \"vh_fused.hpp\"\nusing namespace vh;\n";
+		k += "#define VH_LIVE(u) ((live >> (u)) & 1u)\n";
+		k += "#define VH_KILL(u, c) (live &= ~((u32)(c) << (u)))\n";
+		k += "static constexpr int U = " + std::to_string(U) + ";\n";
+		k += "extern \"C\" __global__ __launch_bounds__(" + std::to_string(block) +
+			") void vh_generated_join(const Plan* __restrict__ Pp)\n{\n";
+		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
+		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
+		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
+		k += "\tconst u32 S = (u32)(" + jp + ".smask + 1);\n";
+		k += "\tu64* const tab = (u64*)((char*)lds + P.rlds_b);\n";
+		k += "\tu32* const rpre = (u32*)(tab + S);\n\tu32* const rbase = rpre + P.rregions;\n";
+		k += "\tu32* const rtot = rbase + P.rregions;\n";
+		k += "\tconst int R = P.rregions;\n";
+		k += "\t{\n\t\tconst " + gptr("u64") + " src = (" + gptr("u64") + ")" + jp + ".slots + (u64)blockIdx.x * S;\n";
+		k += "\t\tfor (u32 i = threadIdx.x; i < S; i += blockDim.x) tab[i] = src[i];\n\t}\n";
+		k += "\tconst u32 rtotal = radix_runs(P, blockIdx.x, rpre, rbase, rtot);\n";
+		k += "\tconst " + gptr("u64") + " const rt = (" + gptr("u64") + ")P.rtup1;\n";
+		k += "\tu32 dups_mask = 0;\n";
+		if (J.variant == VH_JOINN)
+			k += "\tif (gload(" + jp + ".dups, 0)) dups_mask |= " + std::to_string(1u << last) + "u;\n";
+		k += "\tif (gload(" + jp + ".spill, 0)) dups_mask |= " + std::to_string(1u << (kSpillBit + last)) + "u;\n";
+		k += decls;
+		k += "\t__syncthreads();\n";
+		k += loop;
+		k += "\t__syncthreads();\n";
+		if (mode == GB_LDS)
+			k += "\tlds_dense_flush(P, L);\n";
+		else if (mode == GB_LDSHASH)
+			k += "\tldshash_flush(P, L);\n";
+		k += "}\n";
 		return k;
 	}
 };
@@ -836,7 +1085,21 @@ std::string jit_source(const Plan& P, bool eager, int unroll, bool pipeline, int
 	Gen g(P, eager, pipeline, unroll);
 	g.block = block;
 	g.batch = batch;
+	g.radix = P.radix && !batch;
 	return g.source(P.mode);
+}
+
+std::string jit_join_source(const Plan& P, int unroll, int block)
+{
+	Gen g(P, true, false, unroll);
+	g.block = block;
+	return g.join_source(P.mode);
+}
+
+int jit_radix_words(const Plan& P)
+{
+	Gen g(P, true, false, 0);
+	return (int)g.radix_values().size() + 1;
 }
 
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, const char* name)
@@ -885,11 +1148,11 @@ int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, con
 }
 
 hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s,
-	hipEvent_t start, hipEvent_t stop)
+	hipEvent_t start, hipEvent_t stop, long lds_bytes)
 {
 	const Plan* arg = dP;
 	void* args[] = {(void*)&arg};
-	const unsigned lds = (unsigned)pipeline_lds_bytes(P);
+	const unsigned lds = lds_bytes >= 0 ? (unsigned)lds_bytes : (unsigned)pipeline_lds_bytes(P);
 	if (start || stop)
 		return hipExtModuleLaunchKernel(fn, (uint32_t)grid * (uint32_t)block, 1, 1, (uint32_t)block, 1, 1, lds, s,
 			args, nullptr, start, stop, 0);

@@ -44,7 +44,14 @@ int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, con
 // leave per-block partials for launch_merge.
 // start / stop (may be null): events the runtime stamps with the kernel's
 // own start and end (hipExtModuleLaunchKernel).
+// lds_bytes: dynamic LDS of the launch (< 0: pipeline_lds_bytes(P)).
 hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int grid, int block, hipStream_t s,
-	hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+	hipEvent_t start = nullptr, hipEvent_t stop = nullptr, long lds_bytes = -1);
+// Radix-partitioned probe (P.radix): jit_source gives the probe kernel that
+// writes the partitioned tuples; this is the join kernel that consumes them
+// (vh_generated_join, one workgroup per partition of the last join).
+std::string jit_join_source(const Plan& P, int unroll, int block);
+// Words (u64) of one radix probe tuple of P.
+int jit_radix_words(const Plan& P);
 
 } // namespace vh

@@ -169,7 +169,31 @@ struct Plan {
 	int gshift;
 	int gparts;
 	u32* error;          // bit 0: hash table full, bit 3: group key outside [min, max]
+	// Radix-partitioned probe of the last join (radix != 0; that join's
+	// table is partitioned, see BuildPlan).  The probe kernel ends each row
+	// that reaches the last join by appending a tuple of rw words to its
+	// workgroup's region of rtup0 (region b = tuples [b * rcap, (b + 1) *
+	// rcap)): word 0 = key - kmin | partition << 32, then the values the
+	// group-by needs (probe columns, earlier joins' payloads), counting the
+	// tuples per partition in LDS.  At its end it orders its region by
+	// partition into rtup1 and records each partition's run
+	// (rcnt / roff[partition * rregions + region]).  The join kernel then
+	// takes one partition per workgroup: its sub-table copied into LDS, its
+	// runs of every region, the lookups and the group-by.
+	u64* rtup0;
+	u64* rtup1;
+	u32* rcnt;
+	u32* roff;
+	i64 rcap;
+	int radix;
+	int rw;
+	int rregions;
+	u32 rlds_a;          // probe kernel: LDS byte offset of its partition counts
+	u32 rlds_b;          // join kernel: LDS byte offset of the sub-table
+	int rpad;
 };
+
+static constexpr int kRadixMaxWords = 16;   // words of a radix probe tuple
 
 // A work item of a batched launch: rows [r0, r1) of one pipeline's build or
 // probe side.

@@ -163,10 +163,11 @@ class Event:
 
 
 def make_config(group_mode=-1, grid=0, table_shift=0, tile_shift=0, join_layout=0, kernel=0, unroll=0, merge=0,
-                part_shift=0, block=0):
+                part_shift=0, block=0, radix=0):
     c = abi.vh_config()
     c.part_shift = part_shift
     c.block = block
+    c.radix = radix
     c.group_mode = group_mode
     c.grid = grid
     c.table_shift = table_shift
