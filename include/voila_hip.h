@@ -61,7 +61,8 @@ enum vh_join_variant { VH_JOIN01 = 0, VH_JOINN };
 #define VH_MAX_COLS     8
 #define VH_MAX_PREDS    8
 #define VH_MAX_JOINS    4
-#define VH_MAX_PAYLOAD  2
+#define VH_MAX_PAYLOAD  4
+#define VH_MAX_JOIN_KEYS 3
 #define VH_MAX_KEYS     4
 #define VH_MAX_AGGRS    8
 #define VH_MAX_EXPR     32
@@ -110,14 +111,21 @@ typedef struct vh_table_desc {
  * (relalg.hpp:135-165, relalg_translator.cpp:812-1115).  The build side is
  * materialised (write), indexed (bucket_build) and probed with
  * bucket_lookup / check / gather / bucket_next.  The payload columns become
- * new virtual columns of the probe stream, in join order. */
+ * new virtual columns of the probe stream, in join order.  A join on several
+ * key columns (the translator's key lists, hashed with rehash) names the
+ * first in build_key / probe_key and the others in build_keys / probe_keys;
+ * the keys' value ranges together must span at most 32 bits, and so must the
+ * payload columns' ranges. */
 typedef struct vh_join_desc {
 	vh_table_desc build;
-	int32_t build_key;                  /* build column index (single key)   */
-	int32_t probe_key;                  /* virtual column of the probe stream */
-	int32_t variant;                    /* enum vh_join_variant              */
+	int32_t build_key;                  /* build column of the first key      */
+	int32_t probe_key;                  /* virtual column of the first key    */
+	int32_t variant;                    /* enum vh_join_variant               */
 	int32_t npayload;
-	int32_t payload[VH_MAX_PAYLOAD];    /* build column indices              */
+	int32_t payload[VH_MAX_PAYLOAD];    /* build column indices               */
+	int32_t nkeys;                      /* key columns; 0 and 1 both mean one */
+	int32_t build_keys[VH_MAX_JOIN_KEYS - 1];   /* further build key columns  */
+	int32_t probe_keys[VH_MAX_JOIN_KEYS - 1];   /* further probe key columns  */
 } vh_join_desc;
 
 typedef struct vh_aggr_desc {
@@ -469,7 +477,7 @@ int vh_mlp_train_step(vh_mlp* m, const float* x, const uint8_t* mask, const int3
 /* Relational front end ("Flavor::Hip").  Plans are the reference's          */
 /* relalg operators (relalg.hpp:12-181) in S-expression form:               */
 /*   (hashaggr hash|global (keys k...) (aggrs (sum E) (count) ...) PLAN)    */
-/*   (hashjoin join01|joinn LEFT (keys k) (payl c...) RIGHT (keys k) (payl c...)) */
+/*   (hashjoin join01|joinn LEFT (keys k...) (payl c...) RIGHT (keys k...) (payl c...)) */
 /*   (select PRED PLAN)  (project (assign name E)... PLAN)  (scan table c...) */
 /* E := column | integer | (+|-|* E E);  PRED := (<|<=|>|>=|=|!= col int)    */
 /* | (and PRED PRED).  Columns are named as scanned ("table.col" or "col"). */

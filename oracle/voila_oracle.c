@@ -976,6 +976,10 @@ static void stage_groupby(vo_thread* th, const sel_t* sel, sel_t num)
 	vo_groupby_batch(th, &th->gt, sel, num, keyv, aggv, is_count);
 }
 
+static int join_nkeys(const vh_join_desc* jd) { return jd->nkeys > 1 ? jd->nkeys : 1; }
+static int join_build_key(const vh_join_desc* jd, int k) { return k ? jd->build_keys[k - 1] : jd->build_key; }
+static int join_probe_key(const vh_join_desc* jd, int k) { return k ? jd->probe_keys[k - 1] : jd->probe_key; }
+
 /* Probe stage of join j (relalg_translator.cpp:933-1111, §3.3):
  * bucket_lookup, check, gather, emit, [Join01 stop], bucket_next. */
 static void stage_join(vo_thread* th, int j, const sel_t* sel, sel_t num)
@@ -991,7 +995,10 @@ static void stage_join(vo_thread* th, int j, const sel_t* sel, sel_t num)
 	const vh_join_desc* jd = &d->joins[j];
 	vo_htab* t = &q->jt[j];
 	vo_level* L = &th->lvl[j];
-	const int64_t* key = th->v[jd->probe_key];
+	const int nk = join_nkeys(jd);
+	const int64_t* keyv[VH_MAX_JOIN_KEYS];
+	for (int k = 0; k < nk; k++)
+		keyv[k] = th->v[join_probe_key(jd, k)];
 	int first_pay = q->d->probe.ncols;
 	for (int jj = 0; jj < j; jj++)
 		first_pay += d->joins[jj].npayload;
@@ -999,7 +1006,10 @@ static void stage_join(vo_thread* th, int j, const sel_t* sel, sel_t num)
 	/* bucket = bucket_lookup(ht, hash(key)); active = selfalse(eq(0, bucket)) */
 	for (sel_t k = 0; k < num; k++) {
 		sel_t i = sel[k];
-		L->hash[i] = vo_hash(key[i]);
+		uint64_t h = vo_hash(keyv[0][i]);
+		for (int c = 1; c < nk; c++)
+			h = vo_rehash(h, keyv[c][i]);
+		L->hash[i] = h;
 		L->bucket[i] = t->heads[L->hash[i] & t->mask];
 		L->match[i] = L->bucket[i] == 0;
 	}
@@ -1011,7 +1021,11 @@ static void stage_join(vo_thread* th, int j, const sel_t* sel, sel_t num)
 		/* match = check(ht.key, bucket, key) | active; hit = seltrue(match) */
 		for (sel_t k = 0; k < active_num; k++) {
 			sel_t i = active[k];
-			L->match[i] = ROW_KEY(t, (const char*)(uintptr_t)L->bucket[i], 0) == key[i];
+			const char* row = (const char*)(uintptr_t)L->bucket[i];
+			int eq = 1;
+			for (int c = 0; c < nk; c++)
+				eq = eq && ROW_KEY(t, row, c) == keyv[c][i];
+			L->match[i] = (uint8_t)eq;
 		}
 		sel_t hit_num = vo_seltrue(active, active_num, L->hit, L->match);
 		if (hit_num) {
@@ -1020,7 +1034,7 @@ static void stage_join(vo_thread* th, int j, const sel_t* sel, sel_t num)
 				int64_t* dst = th->v[first_pay + p];
 				for (sel_t k = 0; k < hit_num; k++) {
 					sel_t i = L->hit[k];
-					dst[i] = ((const int64_t*)(uintptr_t)L->bucket[i])[1 + p];
+					dst[i] = ((const int64_t*)(uintptr_t)L->bucket[i])[nk + p];
 				}
 			}
 			stage_join(th, j + 1, L->hit, hit_num);
@@ -1075,11 +1089,17 @@ static void run_build(vo_thread* th, int j)
 			for (sel_t k = 0; k < num; k++) {
 				sel_t i = th->sel_a[k];
 				char* row = w + (size_t)k * t->width;
-				int64_t key = load_col(&tb->cols[jd->build_key], base + i);
-				((int64_t*)row)[0] = key;
+				/* keys (hash, then rehash per further column), then payloads */
+				const int nk = join_nkeys(jd);
+				uint64_t h = 0;
+				for (int k = 0; k < nk; k++) {
+					int64_t key = load_col(&tb->cols[join_build_key(jd, k)], base + i);
+					ROW_KEY(t, row, k) = key;
+					h = k ? vo_rehash(h, key) : vo_hash(key);
+				}
 				for (int p = 0; p < jd->npayload; p++)
-					((int64_t*)row)[1 + p] = load_col(&tb->cols[jd->payload[p]], base + i);
-				ROW_HASH(t, row) = vo_hash(key);
+					((int64_t*)row)[nk + p] = load_col(&tb->cols[jd->payload[p]], base + i);
+				ROW_HASH(t, row) = h;
 			}
 			b->num += (size_t)num;
 		}
@@ -1250,7 +1270,7 @@ int vo_run_pipeline(const vh_pipeline_desc* d, int nthreads, int vector_size,
 	for (int j = 0; j < d->njoins; j++) {
 		if (d->joins[j].npayload > VH_MAX_PAYLOAD)
 			return VH_ERR_INVALID;
-		htab_init(&q.jt[j], 1, d->joins[j].npayload, 8, nthreads, 0, q.vsize);
+		htab_init(&q.jt[j], join_nkeys(&d->joins[j]), d->joins[j].npayload, 8, nthreads, 0, q.vsize);
 	}
 	pthread_barrier_init(&q.barrier, NULL, (unsigned)nthreads);
 	q.th = (vo_thread*)xalloc(sizeof(vo_thread) * (size_t)nthreads);
@@ -1357,13 +1377,23 @@ int vo_run_pipeline(const vh_pipeline_desc* d, int nthreads, int vector_size,
 /* Closed-form checker                                                       */
 /* ======================================================================== */
 
-typedef struct bf_entry { int64_t key; int64_t row; } bf_entry;
+typedef struct bf_entry { int64_t key[VH_MAX_JOIN_KEYS]; int64_t row; } bf_entry;
+
+static int bf_key_cmp(const int64_t* x, const int64_t* y)
+{
+	for (int k = 0; k < VH_MAX_JOIN_KEYS; k++)
+		if (x[k] != y[k])
+			return x[k] < y[k] ? -1 : 1;
+	return 0;
+}
 
 static int bf_entry_cmp(const void* a, const void* b)
 {
 	const bf_entry* x = (const bf_entry*)a;
 	const bf_entry* y = (const bf_entry*)b;
-	if (x->key != y->key) return x->key < y->key ? -1 : 1;
+	const int c = bf_key_cmp(x->key, y->key);
+	if (c)
+		return c;
 	return x->row < y->row ? -1 : x->row > y->row;
 }
 
@@ -1413,14 +1443,16 @@ static void bf_join(bf_ctx* c, int j, int vbase)
 		return;
 	}
 	const vh_join_desc* jd = &d->joins[j];
-	int64_t key = c->vals[jd->probe_key];
+	int64_t key[VH_MAX_JOIN_KEYS] = {0};
+	for (int k = 0; k < join_nkeys(jd); k++)
+		key[k] = c->vals[join_probe_key(jd, k)];
 	/* lower bound */
 	int64_t lo = 0, hi = c->nsorted[j];
 	while (lo < hi) {
 		int64_t mid = (lo + hi) / 2;
-		if (c->sorted[j][mid].key < key) lo = mid + 1; else hi = mid;
+		if (bf_key_cmp(c->sorted[j][mid].key, key) < 0) lo = mid + 1; else hi = mid;
 	}
-	for (int64_t m = lo; m < c->nsorted[j] && c->sorted[j][m].key == key; m++) {
+	for (int64_t m = lo; m < c->nsorted[j] && bf_key_cmp(c->sorted[j][m].key, key) == 0; m++) {
 		int64_t row = c->sorted[j][m].row;
 		for (int p = 0; p < jd->npayload; p++)
 			c->vals[vbase + p] = load_col(&jd->build.cols[jd->payload[p]], row);
@@ -1463,7 +1495,9 @@ int vo_bruteforce_pipeline(const vh_pipeline_desc* d, char* out, size_t cap, siz
 		for (int64_t r = 0; r < jd->build.rows; r++) {
 			if (!probe_pass(&jd->build, r))
 				continue;
-			c.sorted[j][n].key = load_col(&jd->build.cols[jd->build_key], r);
+			memset(c.sorted[j][n].key, 0, sizeof(c.sorted[j][n].key));
+			for (int k = 0; k < join_nkeys(jd); k++)
+				c.sorted[j][n].key[k] = load_col(&jd->build.cols[join_build_key(jd, k)], r);
 			c.sorted[j][n].row = r;
 			n++;
 		}

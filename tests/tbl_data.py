@@ -147,3 +147,82 @@ def green_revenue(li, part_keys, green, li_bounds, addrs=None, green_addr=None):
     p.join(pt, "p_partkey", "l_partkey", build_preds=[("p_green", "==", 1)])
     p.groupby(["l_returnflag"], [("sum", col("l_extendedprice") * (100 - col("l_discount"))), ("count",)])
     return p
+
+
+SEGMENTS = ["AUTOMOBILE", "BUILDING", "FURNITURE", "MACHINERY", "HOUSEHOLD"]
+Q3_DATE = tbl.julian_day(1995, 3, 15)
+
+
+def write_customer(path, rows, seed=1):
+    """customer.tbl (eight fields): c_mktsegment one of TPC-H's five."""
+    rng = np.random.default_rng(seed)
+    lines = []
+    for k in range(1, rows + 1):
+        fields = [str(k), f"Customer#{k:09d}", "an address", str(int(rng.integers(0, 25))), "11-111-111-1111",
+                  "100.00", SEGMENTS[int(rng.integers(0, 5))], "a comment"]
+        lines.append("|".join(fields) + "|\n")
+    with open(path, "w") as f:
+        f.writelines(lines)
+
+
+def write_orders(path, rows, customers, seed=1):
+    """orders.tbl (nine fields) with orderkeys 1..rows, the keys
+    write_lineitem's l_orderkey (row // 4 + 1) refers to."""
+    rng = np.random.default_rng(seed)
+    lines = []
+    for k in range(1, rows + 1):
+        d = EPOCH + datetime.timedelta(days=int(rng.integers(0, SPAN_DAYS)))
+        fields = [str(k), str(int(rng.integers(1, customers + 1))), "O", "1000.00", d.isoformat(), "1-URGENT",
+                  "Clerk#000000001", "0", "a comment"]
+        lines.append("|".join(fields) + "|\n")
+    with open(path, "w") as f:
+        f.writelines(lines)
+
+
+def q3_from_text(lineitem_path, orders_path, customer_path, segment="BUILDING"):
+    """TPC-H Q3 from the text with Python integers: rows
+    l_orderkey|o_orderdate|o_shippriority|revenue (revenue scaled by 10^4)."""
+    cust = {int(f[0]) for f in (line.split("|") for line in open(customer_path)) if f[6] == segment}
+    orders = {}
+    for line in open(orders_path):
+        f = line.split("|")
+        y, m, d = (int(x) for x in f[4].split("-"))
+        od = tbl.julian_day(y, m, d)
+        if od < Q3_DATE and int(f[1]) in cust:
+            orders[int(f[0])] = (od, int(f[7]))
+    groups = {}
+    for line in open(lineitem_path):
+        f = line.split("|")
+        y, m, d = (int(x) for x in f[10].split("-"))
+        if tbl.julian_day(y, m, d) <= Q3_DATE or int(f[0]) not in orders:
+            continue
+        p, dc = (int(round(float(x) * 100)) for x in (f[5], f[6]))
+        key = (int(f[0]), *orders[int(f[0])])
+        groups[key] = groups.get(key, 0) + p * (100 - dc)
+    return sorted(f"{k[0]}|{k[1]}|{k[2]}|{v}" for k, v in groups.items())
+
+
+def q3(li, li_b, orders, o_b, customer, c_b, addrs=None):
+    """TPC-H Q3 as one pipeline: lineitem (l_shipdate > date) joins orders
+    (o_orderdate < date; payloads o_orderdate, o_shippriority, o_custkey),
+    then customer on o_custkey (a u8 c_building predicate column, the
+    varchar equality's result for c_mktsegment = 'BUILDING'); group by
+    (l_orderkey, o_orderdate, o_shippriority): sum(l_extendedprice * (100 -
+    l_discount)).  The reference plan joins customer into orders first
+    (bench_tpch_rel.cpp:235-323); with unique keys on both build sides the
+    rows are the same."""
+    a = addrs or {}
+    lt = table_from_arrays("lineitem", li, a.get("lineitem"), li_b)
+    ot = table_from_arrays("orders", orders, a.get("orders"), o_b)
+    ct = table_from_arrays("customer", customer, a.get("customer"), c_b)
+    from vldb_voila_amd import abi
+    p = Pipeline(lt)
+    p.select("l_shipdate", ">", Q3_DATE)
+    p.join(ot, "o_orderkey", "l_orderkey", payload=["o_orderdate", "o_shippriority", "o_custkey"],
+           variant=abi.VH_JOIN01, build_preds=[("o_orderdate", "<", Q3_DATE)],
+           names=["o_orderdate", "o_shippriority", "o_custkey"])
+    p.join(ct, "c_custkey", "o_custkey", payload=[], variant=abi.VH_JOIN01, build_preds=[("c_building", "==", 1)],
+           names=[])
+    p.groupby(["l_orderkey", "o_orderdate", "o_shippriority"],
+              [("sum", col("l_extendedprice") * (100 - col("l_discount")))])
+    return p

@@ -14,7 +14,8 @@ VH_JOIN01, VH_JOINN = range(2)
 VH_MAX_COLS = 8
 VH_MAX_PREDS = 8
 VH_MAX_JOINS = 4
-VH_MAX_PAYLOAD = 2
+VH_MAX_PAYLOAD = 4
+VH_MAX_JOIN_KEYS = 3
 VH_MAX_KEYS = 4
 VH_MAX_AGGRS = 8
 VH_MAX_EXPR = 32
@@ -64,7 +65,9 @@ class vh_table_desc(C.Structure):
 class vh_join_desc(C.Structure):
     _fields_ = [("build", vh_table_desc), ("build_key", C.c_int32), ("probe_key", C.c_int32),
                 ("variant", C.c_int32), ("npayload", C.c_int32),
-                ("payload", C.c_int32 * VH_MAX_PAYLOAD)]
+                ("payload", C.c_int32 * VH_MAX_PAYLOAD), ("nkeys", C.c_int32),
+                ("build_keys", C.c_int32 * (VH_MAX_JOIN_KEYS - 1)),
+                ("probe_keys", C.c_int32 * (VH_MAX_JOIN_KEYS - 1))]
 
 
 class vh_aggr_desc(C.Structure):

@@ -139,6 +139,12 @@ int type_pipeline(const vh_pipeline_desc* d, Typing& ty)
 		const vh_join_desc& jd = d->joins[j];
 		if (jd.probe_key < 0 || jd.probe_key >= (int)ty.vcols.size())
 			return fail(VH_ERR_INVALID, "join %d: probe_key %d out of range", j, jd.probe_key);
+		if (jd.nkeys < 0 || jd.nkeys > VH_MAX_JOIN_KEYS)
+			return fail(VH_ERR_INVALID, "join %d: nkeys %d", j, jd.nkeys);
+		for (int k = 1; k < jd.nkeys; k++)
+			if (jd.probe_keys[k - 1] < 0 || jd.probe_keys[k - 1] >= (int)ty.vcols.size() ||
+				jd.build_keys[k - 1] < 0 || jd.build_keys[k - 1] >= jd.build.ncols)
+				return fail(VH_ERR_INVALID, "join %d: key column %d out of range", j, k);
 		if (jd.npayload < 0 || jd.npayload > VH_MAX_PAYLOAD)
 			return fail(VH_ERR_INVALID, "join %d: npayload %d", j, jd.npayload);
 		for (int p = 0; p < jd.npayload; p++) {
@@ -253,6 +259,7 @@ static int role_of_vcol(const vh_pipeline_desc* d, int v, Role& r)
 		if (v < base + d->joins[j].npayload) {
 			r.kind = ROLE_PAY;
 			r.slot = j;
+			r.item = v - base;
 			r.type = d->joins[j].build.cols[d->joins[j].payload[v - base]].type;
 			return VH_OK;
 		}
@@ -355,13 +362,35 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		memset(&B, 0, sizeof(B));
 		if (jd.build_key < 0 || jd.build_key >= jd.build.ncols)
 			return fail(VH_ERR_INVALID, "join %d: build_key %d", j, jd.build_key);
-		if (jd.npayload > 1)
-			return fail(VH_ERR_UNSUPPORTED, "join %d: more than one payload column", j);
 		if (jd.variant != VH_JOIN01 && jd.variant != VH_JOINN)
 			return fail(VH_ERR_INVALID, "join %d: variant %d", j, jd.variant);
 		const vh_column& kc = jd.build.cols[jd.build_key];
-		if ((double)kc.max - (double)kc.min >= 4294967295.0)
+		const int nk = std::max(jd.nkeys, 1);
+		// a multi-column key: each build column's offset from its minimum in
+		// a bit field wide enough for its range; the fields together form
+		// the entry's key (at most 32 bits).  A probe value outside the
+		// build column's range matches no row.
+		int kbits = 0;
+		u64 kpacked_max = 0;
+		if (nk > 1) {
+			for (int c = 0; c < nk; c++) {
+				const int bc = c ? jd.build_keys[c - 1] : jd.build_key;
+				const vh_column& col = jd.build.cols[bc];
+				const double lo = (double)col.min, hi = (double)col.max;
+				const double range = hi - lo;
+				int bits = 0;
+				while (bits < 40 && (double)(1ull << bits) <= range)
+					bits++;
+				if (kbits + bits > 32)
+					return fail(VH_ERR_UNSUPPORTED, "join %d: key columns span more than 32 bits", j);
+				kpacked_max |= (u64)range << kbits;
+				kbits += bits;
+			}
+			if (kpacked_max >= 0xFFFFFFFFull)
+				return fail(VH_ERR_UNSUPPORTED, "join %d: key columns span more than 32 bits", j);
+		} else if ((double)kc.max - (double)kc.min >= 4294967295.0) {
 			return fail(VH_ERR_UNSUPPORTED, "join %d: key range exceeds 32 bits", j);
+		}
 		B.rows = jd.build.rows;
 		B.npreds = jd.build.npreds;
 		if (B.npreds < 0 || B.npreds > VH_MAX_PREDS)
@@ -380,16 +409,59 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		}
 		B.key = kc.data;
 		B.key_type = kc.type;
-		B.kmin = kc.min;
-		B.kmax = kc.max;
+		B.kmin = nk > 1 ? 0 : kc.min;
+		B.kmax = nk > 1 ? (i64)kpacked_max : kc.max;
+		B.nkeys = nk;
+		if (nk > 1) {
+			int shift = 0;
+			for (int c = 0; c < nk; c++) {
+				const vh_column& col = jd.build.cols[c ? jd.build_keys[c - 1] : jd.build_key];
+				B.keyc[c] = col.data;
+				B.keyc_type[c] = col.type;
+				B.kminc[c] = col.min;
+				B.krangec[c] = (u64)(col.max - col.min);
+				B.kshiftc[c] = shift;
+				int bits = 0;
+				while ((double)(1ull << bits) <= (double)col.max - (double)col.min)
+					bits++;
+				shift += bits;
+			}
+		}
+		// payload columns: bit fields of one 32-bit word (a single column
+		// keeps its whole range below 2^32 - 1)
 		B.npay = jd.npayload;
-		if (jd.npayload) {
+		u64 pmax = 0;
+		if (jd.npayload == 1) {
 			const vh_column& pc = jd.build.cols[jd.payload[0]];
 			if ((double)pc.max - (double)pc.min >= 4294967295.0)
 				return fail(VH_ERR_UNSUPPORTED, "join %d: payload range exceeds 32 bits", j);
 			B.pay = pc.data;
 			B.pay_type = pc.type;
 			B.pmin = pc.min;
+			B.payc[0] = pc.data;
+			B.payc_type[0] = pc.type;
+			B.pminc[0] = pc.min;
+			pmax = (u64)(pc.max - pc.min);
+		} else if (jd.npayload > 1) {
+			int shift = 0;
+			for (int k = 0; k < jd.npayload; k++) {
+				const vh_column& pc = jd.build.cols[jd.payload[k]];
+				const double range = (double)pc.max - (double)pc.min;
+				int bits = 0;
+				while (bits < 40 && (double)(1ull << bits) <= range)
+					bits++;
+				if (shift + bits > 32)
+					return fail(VH_ERR_UNSUPPORTED, "join %d: payload columns span more than 32 bits", j);
+				B.payc[k] = pc.data;
+				B.payc_type[k] = pc.type;
+				B.pminc[k] = pc.min;
+				B.pshiftc[k] = shift;
+				pmax |= (u64)range << shift;
+				shift += bits;
+			}
+			if (pmax >= 0xFFFFFFFFull)
+				return fail(VH_ERR_UNSUPPORTED, "join %d: payload columns span more than 32 bits", j);
+			B.pmin = B.pminc[0];
 		}
 		uint64_t cap = calc_num_buckets((uint64_t)std::max<int64_t>(jd.build.rows, 0));
 		if (cfg && cfg->table_shift > 0)
@@ -403,9 +475,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 
 		// direct layout: an entry per key of [kmin, kmax], payload - pmin in
 		// the fewest bytes that keep all-ones free as "no row"
-		const double krange = (double)kc.max - (double)kc.min + 1;
-		const double prange = jd.npayload
-			? (double)jd.build.cols[jd.payload[0]].max - (double)jd.build.cols[jd.payload[0]].min : 0;
+		const double krange = (double)B.kmax - (double)B.kmin + 1;
+		const double prange = (double)pmax;
 		const bool dense = krange <= 4.0 * (double)std::max<int64_t>(jd.build.rows, 0) + 65536.0;
 		const int pcfg = cfg ? cfg->part_shift : 0;
 		if (pcfg < -1 || pcfg > 30)
@@ -457,11 +528,33 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		J.hashed = B.hashed;
 		if ((rc = role_of_vcol(d, jd.probe_key, J.key)))
 			return rc;
-		J.kmin = kc.min;
-		J.kmax = kc.max;
+		J.kmin = B.kmin;
+		J.kmax = B.kmax;
 		J.pmin = B.pmin;
 		J.dir_w = B.dir_w;
 		J.npay = jd.npayload;
+		J.nkeys = nk;
+		if (nk > 1) {
+			for (int c = 0; c < nk; c++) {
+				if ((rc = role_of_vcol(d, c ? jd.probe_keys[c - 1] : jd.probe_key, J.keyc[c])))
+					return rc;
+				J.kminc[c] = B.kminc[c];
+				J.krangec[c] = B.krangec[c];
+				J.kshiftc[c] = B.kshiftc[c];
+			}
+		}
+		for (int k = 0; k < VH_MAX_PAYLOAD; k++) {
+			J.pminc[k] = B.pminc[k];
+			J.pshiftc[k] = B.pshiftc[k];
+			J.pmaskc[k] = 0xFFFFFFFFu;
+		}
+		if (jd.npayload > 1) {
+			for (int k = 0; k < jd.npayload; k++) {
+				const int next = k + 1 < jd.npayload ? B.pshiftc[k + 1] : 32;
+				const int bits = next - B.pshiftc[k];
+				J.pmaskc[k] = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+			}
+		}
 		J.variant = jd.variant;
 		J.last = j == d->njoins - 1;
 		J.filtered = B.npreds > 0;
@@ -755,6 +848,32 @@ void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* 
 	}
 }
 
+// From the generated probe kernel(s) to the generic one, which computes the
+// same result: it reads no LDS join tables, runs 256-thread workgroups over
+// the grid the lowering chose, and has no radix-partitioned probe.
+static int switch_to_generic(Query* q)
+{
+	Lowered& L = q->L;
+	Plan& P = L.plan;
+	q->jit_fn = nullptr;
+	q->jit_src.clear();
+	q->jit_join_fn = nullptr;
+	q->jit_join_src.clear();
+	P.radix = 0;
+	for (int j = 0; j < P.njoins; j++) {
+		P.joins[j].lds = LDS_NONE;
+		P.joins[j].lds_off = 0;
+		P.joins[j].bits = nullptr;
+		L.builds[j].bits = nullptr;
+	}
+	L.block = 256;
+	L.grid = q->generic_grid;
+	HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
+	if (P.njoins)
+		HIP_TRY(hipMemcpy(q->dbuilds, L.builds.data(), P.njoins * sizeof(BuildPlan), hipMemcpyHostToDevice));
+	return VH_OK;
+}
+
 int query_create(Query* q, bool single)
 {
 	Lowered& L = q->L;
@@ -890,7 +1009,7 @@ int query_create(Query* q, bool single)
 	HIP_TRY(hipMemcpy(q->dclear, segs.data(), segs.size() * sizeof(FillSeg), hipMemcpyHostToDevice));
 
 	// join tables the generated probe kernel keeps in LDS, its workgroup size
-	const int generic_grid = L.grid;
+	q->generic_grid = L.grid;
 	if (single && q->cfg.kernel != VH_KERNEL_GENERIC) {
 		choose_probe_shape(L, &q->desc, &q->cfg);
 		for (int j = 0; j < P.njoins; j++) {
@@ -974,28 +1093,8 @@ int query_create(Query* q, bool single)
 		}
 		if (rc && kernel != VH_KERNEL_AUTO)
 			return rc;
-		if (rc) {
-			// AUTO: the generic kernel computes the same result.  It reads no
-			// LDS join tables and runs 256-thread workgroups, so the probe
-			// shape goes back to what the lowering chose for it.
-			q->jit_fn = nullptr;
-			q->jit_src.clear();
-			q->jit_join_fn = nullptr;
-			q->jit_join_src.clear();
-			P.radix = 0;
-			for (int j = 0; j < P.njoins; j++) {
-				P.joins[j].lds = LDS_NONE;
-				P.joins[j].lds_off = 0;
-				P.joins[j].bits = nullptr;
-				L.builds[j].bits = nullptr;
-			}
-			L.block = 256;
-			L.grid = generic_grid;
-			HIP_TRY(hipMemcpy(q->dplan, &P, sizeof(Plan), hipMemcpyHostToDevice));
-			if (P.njoins)
-				HIP_TRY(hipMemcpy(q->dbuilds, L.builds.data(), P.njoins * sizeof(BuildPlan),
-					hipMemcpyHostToDevice));
-		}
+		if (rc)   // AUTO: the generic kernel computes the same result
+			return switch_to_generic(q);
 	}
 	return VH_OK;
 }
@@ -1091,12 +1190,21 @@ int query_result(Query* q, hipStream_t s, std::string& text, int64_t& nrows)
 		return fail(VH_ERR_INVALID, "a group key lies outside its column's [min, max]");
 	if (flags[0] & 32u)
 		return fail(VH_ERR_CAPACITY, "a hash table probe sequence exceeded its bound (damaged or full table)");
-	if (flags[0] & 16u)
-		return fail(VH_ERR_UNSUPPORTED, "a JoinN before the last join matched duplicate build keys");
-	for (int j = 0; j < P.njoins - 1; j++)
-		if (d.joins[j].variant == VH_JOINN && flags[2 + 2 * j])
-			return fail(VH_ERR_UNSUPPORTED,
-				"join %d: JoinN with duplicate build keys before the last join", j);
+	if (flags[0] & 16u) {
+		// a JoinN before the last join matched a repeated build key: the
+		// generated kernels take first matches only there, the generic
+		// kernel fans out at any join.  Run again with it (and keep it).
+		if (!q->jit_fn || q->fanout_rerun)
+			return fail(VH_ERR_UNSUPPORTED, "a JoinN before the last join matched a repeated build key: batched "
+				"generated kernels take first matches there (give the pipeline the generic kernel)");
+		int rc = switch_to_generic(q);
+		if (rc)
+			return rc;
+		q->fanout_rerun = true;
+		if ((rc = query_run(q, s, -1)))
+			return rc;
+		return query_result(q, s, text, nrows);
+	}
 	const u64 groups = L.groups;
 	const int nsums = std::max(P.nsums, 1);
 	std::vector<u64> lo(groups * nsums), cnt(groups), hk;

@@ -73,6 +73,8 @@ struct Query {
 	std::string jit_join_src;
 	size_t radix_lds_a = 0, radix_lds_b = 0;
 	int radix_parts = 0;
+	int generic_grid = 0;              // probe grid of the generic kernel
+	bool fanout_rerun = false;         // switched to the generic kernel for a JoinN fan-out
 	~Query();
 	int alloc(void** p, size_t bytes);
 };

@@ -102,6 +102,47 @@ __device__ __forceinline__ bool preds_pass_row(const Pred* preds, int npreds, i6
 	return ok;
 }
 
+// Key offset and packed payload word of build row `row`.  ok: the row passed
+// the build predicates; cleared, with error bit 4, when a key column lies
+// outside its declared range.  A multi-column key packs each column's offset
+// at its bit shift, as the probe does (probe_key_off).
+__device__ __forceinline__ void build_row(const BuildPlan& B, i64 row, bool& ok, u32& off, u32& pay)
+{
+	if (B.nkeys <= 1) {
+		const i64 key = ok ? load1(B.key, B.key_type, row) : B.kmin;
+		if (ok && (key < B.kmin || key > B.kmax)) {
+			gatomic_or(B.error, 4u);
+			ok = false;
+		}
+		off = (u32)(key - B.kmin);
+	} else {
+		u32 o = 0;
+#pragma unroll
+		for (int c = 0; c < VH_MAX_JOIN_KEYS; c++) {
+			if (c < B.nkeys) {
+				const i64 v = ok ? load1(B.keyc[c], B.keyc_type[c], row) : B.kminc[c];
+				const u64 d = (u64)(v - B.kminc[c]);
+				if (ok && d > B.krangec[c]) {
+					gatomic_or(B.error, 4u);
+					ok = false;
+				}
+				o |= (u32)d << B.kshiftc[c];
+			}
+		}
+		off = ok ? o : 0u;
+	}
+	if (B.npay <= 1) {
+		pay = (ok && B.npay) ? (u32)(load1(B.pay, B.pay_type, row) - B.pmin) : 0u;
+	} else {
+		u32 p = 0;
+#pragma unroll
+		for (int k = 0; k < VH_MAX_PAYLOAD; k++)
+			if (k < B.npay && ok)
+				p |= (u32)(load1(B.payc[k], B.payc_type[k], row) - B.pminc[k]) << B.pshiftc[k];
+		pay = p;
+	}
+}
+
 // --------------------------------------------------------------------------
 // chunk state
 // --------------------------------------------------------------------------
@@ -138,9 +179,13 @@ __device__ __forceinline__ void role_u(const Plan& P, const Role& r, const Chunk
 		load_col_u(r.ptr, r.type, C, out);
 		break;
 	case ROLE_PAY: {
-		const i64 pmin = P.joins[r.slot].pmin;
+		// payload `item` of the join, packed in the entry's payload word
+		const JoinT& J = P.joins[r.slot];
+		const i64 pmin = J.pminc[r.item];
+		const int sh = J.pshiftc[r.item];
+		const u32 m = J.pmaskc[r.item];
 		switch (r.slot) {
-#define VH_PAYU(J) _Pragma("unroll") for (int u = 0; u < kU; u++) out[u] = (i64)C.pay[J][u] + pmin;
+#define VH_PAYU(J) _Pragma("unroll") for (int u = 0; u < kU; u++) out[u] = (i64)((C.pay[J][u] >> sh) & m) + pmin;
 		case 0: VH_PAYU(0); break;
 		case 1: VH_PAYU(1); break;
 		case 2: VH_PAYU(2); break;
@@ -160,6 +205,43 @@ __device__ __forceinline__ void role_u(const Plan& P, const Role& r, const Chunk
 			out[u] = 0;
 		break;
 	}
+}
+
+// Key offsets of the chunk's rows for join J (key - kmin, or a multi-column
+// key's packed offsets); rows whose key lies outside its range die.
+__device__ __forceinline__ void probe_key_off(const Plan& P, const JoinT& J, Chunk& C, u32 off[kU])
+{
+	if (J.nkeys <= 1) {
+		i64 key[kU];
+		role_u(P, J.key, C, key);
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			if (C.alive[u] && (key[u] < J.kmin || key[u] > J.kmax))
+				C.alive[u] = false;
+			off[u] = (u32)(key[u] - J.kmin);
+		}
+		return;
+	}
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		off[u] = 0;
+#pragma unroll 1
+	for (int c = 0; c < VH_MAX_JOIN_KEYS; c++) {
+		if (c < J.nkeys) {
+			i64 v[kU];
+			role_u(P, J.keyc[c], C, v);
+#pragma unroll
+			for (int u = 0; u < kU; u++) {
+				const u64 d = (u64)(v[u] - J.kminc[c]);
+				if (d > J.krangec[c])
+					C.alive[u] = false;
+				off[u] |= (u32)d << J.kshiftc[c];
+			}
+		}
+	}
+#pragma unroll
+	for (int u = 0; u < kU; u++)
+		off[u] = C.alive[u] ? off[u] : 0u;
 }
 
 __device__ __forceinline__ void eval_sum_u(const Plan& P, const SumArg& s, const Chunk& C, u64 prod[kU])
@@ -643,58 +725,49 @@ __device__ __forceinline__ void consume_u(const Plan& P, const Lds& L, const Chu
 // one chunk: select -> joins -> group-by
 // --------------------------------------------------------------------------
 
-template <int MODE>
-__device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk& C, u32 dups_mask)
+// Join J0 and everything after it for the chunk's live rows.  The first
+// slot of every live row is in flight before any is resolved.  A JoinN whose
+// build saw a repeated key fans out: the rows' further matches are taken in
+// lockstep (the k-th match of every row in the k-th round), each round
+// running the rest of the pipeline, until no row has another.  Further
+// matches follow the first in its probe sequence (hash layout; then the
+// overflow table after a spill) or sit in the overflow slots from
+// voila_hash(key) on (direct layout).
+template <int MODE, int J0>
+__device__ __forceinline__ void joins_from(const Plan& P, const Lds& L, Chunk& C, u32 dups_mask)
 {
-	// select: conjunctive predicates, one column at a time
-#pragma unroll 1
-	for (int p = 0; p < VH_MAX_PREDS; p++) {
-		if (p < P.npreds) {
-			const Pred& Q = P.preds[p];
-			i64 x[kU];
-			load_col_u(Q.r.ptr, Q.r.type, C, x);
+	if constexpr (J0 >= VH_MAX_JOINS) {
+		consume_u<MODE>(P, L, C);
+	} else {
+		if (J0 >= P.njoins) {
+			consume_u<MODE>(P, L, C);
+			return;
+		}
+		const JoinT& J = P.joins[J0];
+		u64 h[kU];
+		u32 off[kU];
+		u32 ovf_first = 0;   // rows whose match came from the overflow table
+		probe_key_off(P, J, C, off);
+#pragma unroll
+		for (int u = 0; u < kU; u++)
+			h[u] = voila_hash((u64)off[u]) & J.mask;
+		if (J.dir_w) {
+			// direct layout: one load per row
+			u32 v[kU];
+			const u32 empty = dir_empty(J.dir_w);
 #pragma unroll
 			for (int u = 0; u < kU; u++)
-				C.alive[u] = C.alive[u] && compare(Q.cmp, Q.r.type, x[u], Q.value);
-		}
-	}
-	// joins: first slot of every live row in flight, then resolve
-	u64 h_last[kU];
-	u32 off_last[kU];
-	u32 ovf_first = 0;   // rows whose last-join match came from the overflow table
-#pragma unroll
-	for (int j = 0; j < VH_MAX_JOINS; j++) {
-		if (j < P.njoins) {
-			const JoinT& J = P.joins[j];
-			i64 key[kU];
-			role_u(P, J.key, C, key);
-			u64 h[kU];
-			u32 off[kU];
-			u64 s[kU];
+				v[u] = C.alive[u] ? dir_load(J.dir, J.dir_w, off[u]) : empty;
 #pragma unroll
 			for (int u = 0; u < kU; u++) {
-				if (C.alive[u] && (key[u] < J.kmin || key[u] > J.kmax))
+				if (v[u] == empty)
 					C.alive[u] = false;
-				off[u] = (u32)(key[u] - J.kmin);
-				h[u] = voila_hash((u64)off[u]) & J.mask;
+				else
+					C.pay[J0][u] = v[u];
+				h[u] = (h[u] - 1) & J.mask;   // the overflow walk starts at h + 1
 			}
-			if (J.dir_w) {
-				// direct layout: one load per row; repeated keys (if any) are
-				// found by the JoinN tail in the overflow slots from h
-				u32 v[kU];
-				const u32 empty = dir_empty(J.dir_w);
-#pragma unroll
-				for (int u = 0; u < kU; u++)
-					v[u] = C.alive[u] ? dir_load(J.dir, J.dir_w, off[u]) : empty;
-#pragma unroll
-				for (int u = 0; u < kU; u++) {
-					if (v[u] == empty)
-						C.alive[u] = false;
-					else
-						C.pay[j][u] = v[u];
-					h[u] = (h[u] - 1) & J.mask;
-				}
-			} else {
+		} else {
+			u64 s[kU];
 #pragma unroll
 			for (int u = 0; u < kU; u++)
 				s[u] = C.alive[u] ? gload(J.slots, (i64)h[u]) : kEmpty;
@@ -713,11 +786,10 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				}
 				if (n >= kMaxProbe)
 					gatomic_or(P.error, 32u);
-				if ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> (kSpillBit + j)) & 1u)) {
+				if ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> (kSpillBit + J0)) & 1u)) {
 					// not in its sub-table: rows of a full sub-table are in
 					// the overflow table
-					if (J.last)
-						ovf_first |= 1u << u;
+					ovf_first |= 1u << u;
 					h[u] = voila_hash((u64)off[u]) & J.mask;
 					s[u] = gload(J.ovf, (i64)h[u]);
 					for (n = 0; s[u] != kEmpty && (u32)s[u] != off[u] && n < kMaxProbe; n++) {
@@ -730,78 +802,83 @@ __device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk
 				if (s[u] == kEmpty || (u32)s[u] != off[u])
 					C.alive[u] = false;
 				else
-					C.pay[j][u] = (u32)(s[u] >> 32);
-			}
-			}
-			if (!J.last && J.variant == VH_JOINN && (dups_mask >> j) & 1u) {
-				// a first-match join may not fan out: the host reruns elsewhere
-#pragma unroll
-				for (int u = 0; u < kU; u++)
-					if (C.alive[u])
-						gatomic_or(P.error, 16u);
-			}
-			if (J.last) {
-#pragma unroll
-				for (int u = 0; u < kU; u++) {
-					h_last[u] = h[u];
-					off_last[u] = off[u];
-				}
+					C.pay[J0][u] = (u32)(s[u] >> 32);
 			}
 		}
-	}
-	consume_u<MODE>(P, L, C);
-
-	// JoinN: further matches of the last join follow the first one in the
-	// probe sequence (hash layout) or sit in the overflow slots from
-	// voila_hash(key) on (direct layout); each is aggregated on its own.
-	// Without a repeated build key there are none.
-	if (P.njoins > 0) {
-		const int last = P.njoins - 1;
-		const JoinT& J = P.joins[last];
-		if (J.variant == VH_JOINN && ((dups_mask >> last) & 1u)) {
+		if (!(J.variant == VH_JOINN && ((dups_mask >> J0) & 1u))) {
+			joins_from<MODE, J0 + 1>(P, L, C, dups_mask);
+			return;
+		}
+		// fan-out: the rows' state after this join, and where each one's
+		// sequence continues
+		const Chunk keep = C;
+		bool walking[kU], ov[kU];
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			walking[u] = keep.alive[u];
+			ov[u] = (ovf_first >> u) & 1u;
+			h[u] = ov[u] ? (h[u] + 1) & J.mask : hnext(h[u], J.smask);
+		}
+		for (int round = 0;; round++) {
+			joins_from<MODE, J0 + 1>(P, L, C, dups_mask);
+			C = keep;
+			bool any = false;
 #pragma unroll
 			for (int u = 0; u < kU; u++) {
-				if (!C.alive[u])
+				C.alive[u] = false;
+				if (!walking[u])
 					continue;
-				// the rest of the match's sequence: its sub-table (or the
-				// whole table), then the overflow table if the build spilled
-				bool ov = (ovf_first >> u) & 1u;
-				const u64 h0 = voila_hash((u64)off_last[u]) & J.mask;   // where the sequence began
-				u64 h = ov ? (h_last[u] + 1) & J.mask : hnext(h_last[u], J.smask);
+				const u64 h0 = voila_hash((u64)off[u]) & J.mask;   // where the sequence began
 				for (u64 n = 0;; n++) {
 					if (n >= kMaxProbe) {
 						gatomic_or(P.error, 32u);
+						walking[u] = false;
 						break;
 					}
 					// a full sub-table has no empty slot: its sequence ends
 					// where it began
-					const u64 s = (!ov && J.hashed && h == h0) ? kEmpty : gload(ov ? J.ovf : J.slots, (i64)h);
-					if (s == kEmpty) {
-						if (!ov && ((dups_mask >> (kSpillBit + last)) & 1u)) {
-							ov = true;
-							h = voila_hash((u64)off_last[u]) & J.mask;
+					const u64 sv = (!ov[u] && J.hashed && h[u] == h0) ? kEmpty
+						: gload(ov[u] ? J.ovf : J.slots, (i64)h[u]);
+					if (sv == kEmpty) {
+						if (!ov[u] && ((dups_mask >> (kSpillBit + J0)) & 1u)) {
+							ov[u] = true;
+							h[u] = h0;
 							continue;
 						}
+						walking[u] = false;
 						break;
 					}
-					if ((u32)s == off_last[u]) {
-						Chunk one = C;
-#pragma unroll
-						for (int w = 0; w < kU; w++)
-							one.alive[w] = w == u;
-						switch (last) {
-						case 0: one.pay[0][u] = (u32)(s >> 32); break;
-						case 1: one.pay[1][u] = (u32)(s >> 32); break;
-						case 2: one.pay[2][u] = (u32)(s >> 32); break;
-						default: one.pay[3][u] = (u32)(s >> 32); break;
-						}
-						consume_u<MODE>(P, L, one);
+					h[u] = ov[u] ? (h[u] + 1) & J.mask : hnext(h[u], J.smask);
+					if ((u32)sv == off[u]) {
+						C.alive[u] = true;
+						C.pay[J0][u] = (u32)(sv >> 32);
+						any = true;
+						break;
 					}
-					h = ov ? (h + 1) & J.mask : hnext(h, J.smask);
 				}
 			}
+			if (!any)
+				break;
 		}
 	}
+}
+
+template <int MODE>
+__device__ __forceinline__ void process_chunk(const Plan& P, const Lds& L, Chunk& C, u32 dups_mask)
+{
+	// select: conjunctive predicates, one column at a time
+#pragma unroll 1
+	for (int p = 0; p < VH_MAX_PREDS; p++) {
+		if (p < P.npreds) {
+			const Pred& Q = P.preds[p];
+			i64 x[kU];
+			load_col_u(Q.r.ptr, Q.r.type, C, x);
+#pragma unroll
+			for (int u = 0; u < kU; u++)
+				C.alive[u] = C.alive[u] && compare(Q.cmp, Q.r.type, x[u], Q.value);
+		}
+	}
+	joins_from<MODE, 0>(P, L, C, dups_mask);
 }
 
 // Rows [r0, r1) in chunks of kU * blockDim.
@@ -1141,13 +1218,8 @@ __device__ __forceinline__ void build_chunk(const BuildPlan& B, i64 base, i64 en
 	u32 off[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
-		const i64 key = ok[u] ? load1(B.key, B.key_type, row[u]) : B.kmin;
-		const u32 pay = (ok[u] && B.npay) ? (u32)(load1(B.pay, B.pay_type, row[u]) - B.pmin) : 0u;
-		if (ok[u] && (key < B.kmin || key > B.kmax)) {
-			gatomic_or(B.error, 4u);
-			ok[u] = false;
-		}
-		off[u] = (u32)(key - B.kmin);
+		u32 pay;
+		build_row(B, row[u], ok[u], off[u], pay);
 		val[u] = (u64)off[u] | ((u64)pay << 32);
 		h[u] = voila_hash((u64)off[u]) & B.mask;
 	}

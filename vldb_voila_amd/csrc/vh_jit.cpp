@@ -180,12 +180,26 @@ struct Gen {
 			snprintf(b, sizeof b, "((i64)c%d[u])", col_index(r, path, false));
 			return b;
 		case ROLE_PAY: {
-			snprintf(b, sizeof b, "P.joins[%d].pmin", r.slot);
-			const std::string pmin = uniform("i64", b);
+			// payload `item` of the join: the whole entry word for a join
+			// with one payload column, else its bit field
+			std::string word;
 			if (r.slot == tail_join)
-				return "((i64)" + std::string(tail_pay) + " + " + pmin + ")";
-			snprintf(b, sizeof b, "((i64)pay%d[u] + %s)", r.slot, pmin.c_str());
-			return b;
+				word = tail_pay;
+			else {
+				snprintf(b, sizeof b, "pay%d[u]", r.slot);
+				word = b;
+			}
+			if (P.joins[r.slot].npay <= 1) {
+				snprintf(b, sizeof b, "P.joins[%d].pmin", r.slot);
+				return "((i64)" + word + " + " + uniform("i64", b) + ")";
+			}
+			snprintf(b, sizeof b, "P.joins[%d].pminc[%d]", r.slot, r.item);
+			const std::string pmin = uniform("i64", b);
+			snprintf(b, sizeof b, "P.joins[%d].pshiftc[%d]", r.slot, r.item);
+			const std::string sh = uniform("int", b);
+			snprintf(b, sizeof b, "P.joins[%d].pmaskc[%d]", r.slot, r.item);
+			const std::string m = uniform("u32", b);
+			return "((i64)((" + word + " >> " + sh + ") & " + m + ") + " + pmin + ")";
 		}
 		case ROLE_CONST:
 			return "(" + uniform("i64", path + ".c") + ")";
@@ -443,9 +457,30 @@ struct Gen {
 			line("\t\t\tu32 off[U];");
 			line("#pragma unroll");
 			line("\t\t\tfor (int u = 0; u < U; u++) {");
-			line("\t\t\t\tconst i64 key = " + role(J.key, jp + ".key") + ";");
-			line("\t\t\t\tVH_KILL(u, key < " + kmin + " || key > " + kmax + ");");
-			line("\t\t\t\toff[u] = VH_LIVE(u) ? (u32)(key - " + kmin + ") : 0u;");
+			if (J.nkeys <= 1) {
+				line("\t\t\t\tconst i64 key = " + role(J.key, jp + ".key") + ";");
+				line("\t\t\t\tVH_KILL(u, key < " + kmin + " || key > " + kmax + ");");
+				line("\t\t\t\toff[u] = VH_LIVE(u) ? (u32)(key - " + kmin + ") : 0u;");
+			} else {
+				// multi-column key: each column's offset at its bit shift
+				line("\t\t\t\tu32 o = 0;");
+				for (int c = 0; c < J.nkeys; c++) {
+					snprintf(b, sizeof b, "%s.keyc[%d]", jp.c_str(), c);
+					const std::string kp = b;
+					snprintf(b, sizeof b, "%s.kminc[%d]", jp.c_str(), c);
+					const std::string cmin = uniform("i64", b);
+					snprintf(b, sizeof b, "%s.krangec[%d]", jp.c_str(), c);
+					const std::string crange = uniform("u64", b);
+					snprintf(b, sizeof b, "%s.kshiftc[%d]", jp.c_str(), c);
+					const std::string csh = uniform("int", b);
+					line("\t\t\t\t{");
+					line("\t\t\t\t\tconst u64 d = (u64)(" + role(J.keyc[c], kp) + " - " + cmin + ");");
+					line("\t\t\t\t\tVH_KILL(u, d > " + crange + ");");
+					line("\t\t\t\t\to |= (u32)d << " + csh + ";");
+					line("\t\t\t\t}");
+				}
+				line("\t\t\t\toff[u] = VH_LIVE(u) ? o : 0u;");
+			}
 			line("\t\t\t}");
 			if (!scalar && j == last && late_stage == P.njoins + 1)
 				emit_late();   // aggregation columns, before this join's lookups (and the next chunk's loads)
@@ -662,8 +697,15 @@ struct Gen {
 			note_role(P.preds[p].r, b, true, 0);
 		}
 		for (int j = 0; j < P.njoins; j++) {
-			snprintf(b, sizeof b, "P.joins[%d].key", j);
-			note_role(P.joins[j].key, b, false, j + 1);
+			if (P.joins[j].nkeys <= 1) {
+				snprintf(b, sizeof b, "P.joins[%d].key", j);
+				note_role(P.joins[j].key, b, false, j + 1);
+			} else {
+				for (int c = 0; c < P.joins[j].nkeys; c++) {
+					snprintf(b, sizeof b, "P.joins[%d].keyc[%d]", j, c);
+					note_role(P.joins[j].keyc[c], b, false, j + 1);
+				}
+			}
 		}
 		for (int k = 0; k < P.nkeys; k++) {
 			snprintf(b, sizeof b, "P.keys[%d].r", k);

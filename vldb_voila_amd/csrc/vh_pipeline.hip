@@ -187,15 +187,18 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 	const i64 r0 = (i64)chunk * kPartChunk;
 	u32 off[kU], pay[kU], rank[kU];
 	bool ok[kU];
-	// every column of the kU rows is requested before any is tested (rows
-	// past the end read row 0), so the chunk costs one memory round trip
+	// one key and at most one payload column: every column of the kU rows is
+	// requested before any is tested (rows past the end read row 0), so the
+	// chunk costs one memory round trip; packed keys and payloads are read
+	// by build_row after the predicates
+	const bool packed = B.nkeys > 1 || B.npay > 1;
 	i64 key[kU], pv[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
 		const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
 		const i64 rr = r < B.rows ? r : 0;
-		key[u] = load1(B.key, B.key_type, rr);
-		pv[u] = B.npay ? load1(B.pay, B.pay_type, rr) : B.pmin;
+		key[u] = packed ? B.kmin : load1(B.key, B.key_type, rr);
+		pv[u] = (B.npay && !packed) ? load1(B.pay, B.pay_type, rr) : B.pmin;
 		ok[u] = r < B.rows;
 	}
 #pragma unroll
@@ -215,6 +218,10 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 	}
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
+		if (packed) {
+			build_row(B, r0 + (i64)u * blockDim.x + threadIdx.x, ok[u], off[u], pay[u]);
+			continue;
+		}
 		if (ok[u] && (key[u] < B.kmin || key[u] > B.kmax)) {
 			gatomic_or(B.error, 4u);
 			ok[u] = false;

@@ -23,8 +23,8 @@ struct Role {
 	i64 c;               // ROLE_CONST: value
 	int kind;
 	int type;            // ROLE_COL: enum vh_type of the column
-	int slot;            // ROLE_PAY: join index (one payload per join)
-	int pad;
+	int slot;            // ROLE_PAY: join index
+	int item;            // ROLE_PAY: which of the join's payload columns
 };
 
 // A linear term sa*a + sb*b + c with sa, sb in {-1, 0, 1}.  A sum argument is
@@ -74,11 +74,25 @@ struct JoinT {
 	const void* dir;
 	int dir_w;
 	int hashed;          // partitioned hash table (smask < mask, ovf and spill set)
-	Role key;            // probe-side key
-	i64 kmin;
+	Role key;            // probe-side key (nkeys == 1)
+	i64 kmin;            // key range; 0 .. the largest packed key when nkeys > 1
 	i64 kmax;
-	i64 pmin;
+	i64 pmin;            // payload 0's minimum (npay == 1)
 	int npay;
+	// multi-column keys (nkeys > 1): the entry's key offset is the sum of
+	// (column value - kminc[c]) << kshiftc[c]; a value outside
+	// [kminc[c], kminc[c] + krangec[c]] matches no row
+	int nkeys;
+	Role keyc[VH_MAX_JOIN_KEYS];
+	i64 kminc[VH_MAX_JOIN_KEYS];
+	u64 krangec[VH_MAX_JOIN_KEYS];
+	int kshiftc[VH_MAX_JOIN_KEYS];
+	int kpad;
+	// payload columns packed in the entry's payload word: payload k is
+	// ((word >> pshiftc[k]) & pmaskc[k]) + pminc[k]
+	i64 pminc[VH_MAX_PAYLOAD];
+	int pshiftc[VH_MAX_PAYLOAD];
+	u32 pmaskc[VH_MAX_PAYLOAD];
 	int variant;         // enum vh_join_variant
 	int last;            // last join of the pipeline: JoinN may match many
 	int filtered;        // the build side has predicates (probe keys often miss)
@@ -212,12 +226,24 @@ struct BuildPlan {
 	int key_type;
 	Pred preds[VH_MAX_PREDS];
 	const void* key;
-	const void* pay;
+	const void* pay;     // payload 0 (npay == 1)
 	int pay_type;
 	int npay;
-	i64 kmin;
+	i64 kmin;            // key range (nkeys == 1); packed keys start at 0
 	i64 kmax;
 	i64 pmin;
+	// multi-column keys and payloads, packed as JoinT describes
+	int nkeys;
+	int pad2;
+	const void* keyc[VH_MAX_JOIN_KEYS];
+	int keyc_type[VH_MAX_JOIN_KEYS];
+	int kshiftc[VH_MAX_JOIN_KEYS];
+	i64 kminc[VH_MAX_JOIN_KEYS];
+	u64 krangec[VH_MAX_JOIN_KEYS];
+	const void* payc[VH_MAX_PAYLOAD];
+	int payc_type[VH_MAX_PAYLOAD];
+	int pshiftc[VH_MAX_PAYLOAD];
+	i64 pminc[VH_MAX_PAYLOAD];
 	u64* slots;
 	u64 mask;
 	void* dir;           // direct layout (see JoinT), dir_w bytes per key

@@ -300,18 +300,28 @@ struct Lowering {
 			const SxP& lk = n->list[3];
 			const SxP& rk = n->list[6];
 			const SxP& rp = n->list[7];
-			if (lk->list.size() != 2 || rk->list.size() != 2)
-				return bad("joins on exactly one key are lowered");
-			auto lit = vcol.find(lk->list[1]->atom);
-			if (lit == vcol.end())
-				return bad("probe key " + lk->list[1]->atom + " is not available");
-			J.probe_key = lit->second;
-			J.build_key = -1;
-			for (size_t i = 0; i < bnames.size(); i++)
-				if (bnames[i] == rk->list[1]->atom)
-					J.build_key = (int)i;
-			if (J.build_key < 0)
-				return bad("build key " + rk->list[1]->atom + " is not scanned");
+			if (lk->list.size() != rk->list.size() || lk->list.size() < 2 ||
+				lk->list.size() > 1 + VH_MAX_JOIN_KEYS)
+				return bad("a join needs 1 to 3 key columns on each side, pairwise");
+			J.nkeys = (int)lk->list.size() - 1;
+			for (int k = 0; k < J.nkeys; k++) {
+				auto lit = vcol.find(lk->list[k + 1]->atom);
+				if (lit == vcol.end())
+					return bad("probe key " + lk->list[k + 1]->atom + " is not available");
+				int bk = -1;
+				for (size_t i = 0; i < bnames.size(); i++)
+					if (bnames[i] == rk->list[k + 1]->atom)
+						bk = (int)i;
+				if (bk < 0)
+					return bad("build key " + rk->list[k + 1]->atom + " is not scanned");
+				if (k == 0) {
+					J.probe_key = lit->second;
+					J.build_key = bk;
+				} else {
+					J.probe_keys[k - 1] = lit->second;
+					J.build_keys[k - 1] = bk;
+				}
+			}
 			J.variant = var == "join01" ? VH_JOIN01 : VH_JOINN;
 			int nv = (int)d.probe.ncols;
 			for (int j = 0; j < d.njoins; j++)

@@ -98,9 +98,14 @@ class Pipeline:
 
     def join(self, build: Table, build_key, probe_key, payload=(), variant=abi.VH_JOINN,
              build_preds=(), names=None):
+        """build_key / probe_key: a column name, or a list of them for a
+        multi-column key (matched pairwise)."""
         names = list(names) if names is not None else [f"{build.name}.{p}" for p in payload]
-        self.joins.append(dict(build=build, build_key=build.index(build_key),
-                               probe_key=self.vindex(probe_key),
+        bks = [build_key] if isinstance(build_key, str) else list(build_key)
+        pks = [probe_key] if isinstance(probe_key, str) else list(probe_key)
+        assert len(bks) == len(pks) and 1 <= len(bks) <= abi.VH_MAX_JOIN_KEYS
+        self.joins.append(dict(build=build, build_keys=[build.index(k) for k in bks],
+                               probe_keys=[self.vindex(k) for k in pks],
                                payload=[build.index(p) for p in payload], variant=variant,
                                preds=[(build.index(c), abi.CMP_NAMES[o], int(v))
                                       for (c, o, v) in build_preds]))
@@ -140,10 +145,15 @@ class Pipeline:
         d.njoins = len(self.joins)
         for j, jd in enumerate(self.joins):
             self._fill_table(d.joins[j].build, jd["build"], jd["preds"])
-            d.joins[j].build_key = jd["build_key"]
-            d.joins[j].probe_key = jd["probe_key"]
+            d.joins[j].build_key = jd["build_keys"][0]
+            d.joins[j].probe_key = jd["probe_keys"][0]
+            d.joins[j].nkeys = len(jd["build_keys"])
+            for k in range(1, len(jd["build_keys"])):
+                d.joins[j].build_keys[k - 1] = jd["build_keys"][k]
+                d.joins[j].probe_keys[k - 1] = jd["probe_keys"][k]
             d.joins[j].variant = jd["variant"]
             d.joins[j].npayload = len(jd["payload"])
+            assert d.joins[j].npayload <= abi.VH_MAX_PAYLOAD
             for p, ci in enumerate(jd["payload"]):
                 d.joins[j].payload[p] = ci
         d.nkeys = len(self.keys)
