@@ -63,3 +63,33 @@ def test_rejected_plans(plan):
     db = relalg.Database().add_table("l", t)
     with pytest.raises(runtime.VoilaHipError):
         db.lower(plan)
+
+
+def test_multi_column_key_join_plan():
+    """A join on two key columns with two payloads (the translator's key and
+    payload lists, relalg_translator.cpp:812-1115) lowers to the same
+    pipeline as the builder's multi-column join."""
+    from vldb_voila_amd.pipeline import Pipeline, col, table_from_arrays
+    rng = np.random.default_rng(12)
+    ps = {"ps_partkey": rng.integers(1, 200, 3000).astype(np.int32),
+          "ps_suppkey": rng.integers(1, 40, 3000).astype(np.int32),
+          "ps_cost": rng.integers(100, 1000, 3000).astype(np.int64),
+          "ps_qty": rng.integers(1, 10, 3000).astype(np.int32)}
+    li = {"l_partkey": rng.integers(1, 210, 20000).astype(np.int32),
+          "l_suppkey": rng.integers(1, 42, 20000).astype(np.int32),
+          "l_qty": rng.integers(1, 50, 20000).astype(np.int64)}
+    db = relalg.Database().add_table("lineitem", li).add_table("partsupp", ps)
+    plan = """(hashaggr hash (keys ps_qty) (aggrs (sum (* l_qty ps_cost)) (count))
+      (hashjoin joinn (scan lineitem l_partkey l_suppkey l_qty) (keys l_partkey l_suppkey)
+        (payl) (scan partsupp ps_partkey ps_suppkey ps_cost ps_qty) (keys ps_partkey ps_suppkey)
+        (payl ps_cost ps_qty)))"""
+    d = db.lower(plan)
+    assert d.joins[0].nkeys == 2 and d.joins[0].npayload == 2
+    p = Pipeline(table_from_arrays("lineitem", li))
+    p.join(table_from_arrays("partsupp", ps), ["ps_partkey", "ps_suppkey"], ["l_partkey", "l_suppkey"],
+           payload=["ps_cost", "ps_qty"], names=["ps_cost", "ps_qty"])
+    p.groupby(["ps_qty"], [("sum", col("l_qty") * col("ps_cost")), ("count",)])
+    want = oracle.sorted_rows(oracle.bruteforce(p.desc()))
+    assert len(want) == 9
+    assert oracle.sorted_rows(oracle.bruteforce(d)) == want
+    assert oracle.sorted_rows(oracle.run_pipeline(d, threads=2)[0]) == want
