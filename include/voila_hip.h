@@ -453,6 +453,20 @@ int vh_variant_config(int32_t variant, vh_config* out);
  * `stream`), or -1 when the variant does not apply to the pipeline. */
 int vh_tune_time(const vh_pipeline_desc* desc, const vh_config* cfgs, int32_t n, int32_t reps,
 	float* ms, void* stream);
+/* Tuning cache: the best variant per pipeline signature (column types,
+ * predicate / join / key / sum counts, build sides and probe rows and group
+ * domain bucketed by powers of two).  The reference keeps its explorer's
+ * timings in SQLite and picks the fastest blend per query with SQL
+ * (compiler.cpp:100-122, print_base_flavors.py:32-40); here the pick is kept
+ * in memory, and callers persist entries with get / put.
+ * vh_tune_best: the cached index (cached = 1), or vh_tune_time over cfgs and
+ * the fastest, then cached (cached = 0). */
+int vh_tune_signature(const vh_pipeline_desc* desc, char* buf, size_t cap, size_t* len);
+int vh_tune_best(const vh_pipeline_desc* desc, const vh_config* cfgs, int32_t n, int32_t reps,
+	int32_t* best, int32_t* cached, void* stream);
+int vh_tune_cache_get(const char* signature, int32_t* variant);   /* -1 = absent */
+int vh_tune_cache_put(const char* signature, int32_t variant);
+int vh_tune_cache_clear(void);
 
 /* ------------------------------------------------------------------------ */
 /* Learned selector: FP32 MLP 6 -> 64 -> 64 -> 48 trained on the GPU.        */

@@ -86,3 +86,61 @@ def test_tune_time_reports_each_variant():
     got = oracle.sorted_rows(runtime.run_pipeline(
         workloads.join_aggregate_pipeline(b, p, runtime.addrs(db), runtime.addrs(dp)), cfg))
     assert got == want
+
+
+def test_every_variant_gives_the_oracle_rows():
+    """All 48 variants (group-by strategy x table layout and partition
+    fan-out x launch shape) compute the same rows."""
+    print("variants: every one against the oracle", flush=True)
+    b, p = datagen.gen_join_aggregate(30_000, 100_000, seed=8)
+    db, dp = runtime.upload(b), runtime.upload(p)
+    want = oracle.sorted_rows(oracle.bruteforce(workloads.join_aggregate_pipeline(b, p).desc()))
+    ran = 0
+    for v, cfg in enumerate(selector.variant_configs()):
+        try:
+            got = oracle.sorted_rows(runtime.run_pipeline(
+                workloads.join_aggregate_pipeline(b, p, runtime.addrs(db), runtime.addrs(dp)), cfg))
+        except runtime.VoilaHipError as e:
+            assert e.code == abi.VH_ERR_UNSUPPORTED, (v, e)
+            continue
+        assert got == want, v
+        ran += 1
+    assert ran >= 36
+
+
+def test_tune_best_caches_per_signature():
+    """vh_tune_best times the variants once per pipeline signature; a second
+    pipeline of the same shape gets the cached pick without timing."""
+    import ctypes as C
+    print("tune cache", flush=True)
+    L = runtime.lib()
+    L.vh_tune_best.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vh_tune_signature.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.vh_tune_cache_get.argtypes = [C.c_char_p, C.c_void_p]
+    runtime.check(L.vh_tune_cache_clear())
+    cfgs = selector.variant_configs()
+    arr = (abi.vh_config * len(cfgs))(*cfgs)
+    descs = []
+    keep = []
+    for seed in (1, 2):
+        b, p = datagen.gen_join_aggregate(20_000, 70_000, seed=seed)
+        db, dp = runtime.upload(b), runtime.upload(p)
+        keep += [b, p, db, dp]
+        descs.append(workloads.join_aggregate_pipeline(b, p, runtime.addrs(db), runtime.addrs(dp)).desc())
+    sigs = []
+    for d in descs:
+        n = C.c_size_t()
+        buf = C.create_string_buffer(512)
+        runtime.check(L.vh_tune_signature(C.byref(d), buf, 512, C.byref(n)))
+        sigs.append(buf.value)
+    assert sigs[0] == sigs[1]
+    best, cached = C.c_int32(), C.c_int32()
+    runtime.check(L.vh_tune_best(C.byref(descs[0]), C.addressof(arr), len(cfgs), 1, C.byref(best), C.byref(cached),
+                                 None))
+    assert cached.value == 0 and 0 <= best.value < 48
+    got = C.c_int32()
+    runtime.check(L.vh_tune_cache_get(sigs[0], C.byref(got)))
+    assert got.value == best.value
+    b2, c2 = C.c_int32(), C.c_int32()
+    runtime.check(L.vh_tune_best(C.byref(descs[1]), C.addressof(arr), len(cfgs), 1, C.byref(b2), C.byref(c2), None))
+    assert c2.value == 1 and b2.value == best.value

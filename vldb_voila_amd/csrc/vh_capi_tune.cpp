@@ -4,7 +4,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/voila_hip.h"
@@ -20,14 +23,21 @@ using namespace vh;
 			return fail(VH_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e));           \
 	} while (0)
 
-// Variant index -> knobs: 4 group-by strategies x 3 join-table layouts x
-// 4 probe grid sizes = 48 (the analog of the explorer's blend space,
-// explorer_helper.cpp:6-17).
+// Variant index -> knobs: 4 group-by strategies x 3 join-table layouts and
+// partition fan-outs x 4 launch shapes = 48, the analog of the explorer's
+// blend space (explorer_helper.cpp:6-17: computation type x concurrent FSMs
+// x prefetch).
 static const int kModes[4] = {VH_GROUP_DENSE, VH_GROUP_LDSHASH, VH_GROUP_PART, VH_GROUP_HASH};
-// join table: {hash layout, hash layout at twice the slots, direct when dense}
-static const int kTableShift[3] = {0, 1, 0};
-static const int kTableLayout[3] = {VH_LAYOUT_HASH, VH_LAYOUT_HASH, VH_LAYOUT_AUTO};
-static const int kGrids[4] = {256, 512, 1024, 2048};
+// join tables: {direct when the keys are dense, built partitioned by key
+// range (else a hash table, partitioned above 4 MiB)}, {hash table built in
+// 2^13-slot partitions, fan-out = slots / 8192}, {one hash table, global
+// CAS build, no partitions}
+static const int kLayout[3] = {VH_LAYOUT_AUTO, VH_LAYOUT_HASH, VH_LAYOUT_HASH};
+static const int kPartShift[3] = {0, 13, -1};
+// launch shapes: {threads per workgroup, rows per lane (tile = threads x
+// rows), workgroups (0 = the lowering's: two 256-thread ones per CU),
+// batched work item 16384 << tile_shift rows}.  Waves per CU: 8, 16, 8, 16.
+static const int kShape[4][4] = {{256, 8, 0, 0}, {256, 2, 1024, -1}, {512, 4, 0, 0}, {1024, 4, 0, 1}};
 
 struct vh_mlp {
 	float* P = nullptr;   // params
@@ -55,9 +65,12 @@ int vh_variant_config(int32_t v, vh_config* out)
 		return fail(VH_ERR_INVALID, "variant %d", v);
 	memset(out, 0, sizeof(*out));
 	out->group_mode = kModes[v / 12];
-	out->table_shift = kTableShift[(v / 4) % 3];
-	out->join_layout = kTableLayout[(v / 4) % 3];
-	out->grid = kGrids[v % 4];
+	out->join_layout = kLayout[(v / 4) % 3];
+	out->part_shift = kPartShift[(v / 4) % 3];
+	out->block = kShape[v % 4][0];
+	out->unroll = kShape[v % 4][1];
+	out->grid = kShape[v % 4][2];
+	out->tile_shift = kShape[v % 4][3];
 	return VH_OK;
 }
 
@@ -113,6 +126,120 @@ int vh_tune_time(const vh_pipeline_desc* desc, const vh_config* cfgs, int32_t n,
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
 	return rc;
+}
+
+// ---- tuning cache ----------------------------------------------------------
+
+// What the best variant of a pipeline depends on: its column types, the
+// numbers of predicates, joins, keys and sums, each join's build side
+// (rows, key range, payload columns, filtered or not) and the probe rows and
+// group domain, sizes bucketed by powers of two.  Pipelines with the same
+// signature share a tuning result.
+static std::string tune_signature(const vh_pipeline_desc* d)
+{
+	auto bucket = [](double x) {
+		int b = 0;
+		while (b < 63 && (double)(1ull << b) < x)
+			b++;
+		return b;
+	};
+	std::string s = "p" + std::to_string(bucket((double)d->probe.rows)) + "c";
+	for (int c = 0; c < d->probe.ncols; c++)
+		s += std::to_string(d->probe.cols[c].type);
+	s += "f" + std::to_string(d->probe.npreds);
+	for (int j = 0; j < d->njoins; j++) {
+		const vh_join_desc& J = d->joins[j];
+		const vh_column& k = J.build.cols[J.build_key];
+		s += "|j" + std::to_string(bucket((double)J.build.rows)) + "r" +
+			std::to_string(bucket((double)k.max - (double)k.min + 1)) + "k" + std::to_string(std::max(J.nkeys, 1)) +
+			"p" + std::to_string(J.npayload) + "v" + std::to_string(J.variant) + "f" + std::to_string(J.build.npreds);
+	}
+	double domain = 1;
+	Typing ty;
+	if (type_pipeline(d, ty) == VH_OK)
+		for (int k = 0; k < d->nkeys; k++)
+			domain *= ty.vcols[d->keys[k]].dmax - ty.vcols[d->keys[k]].dmin + 1;
+	s += "|g" + std::to_string(d->nkeys) + "d" + std::to_string(bucket(domain)) + "a" + std::to_string(d->naggrs);
+	return s;
+}
+
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_tune_cache;
+
+int vh_tune_signature(const vh_pipeline_desc* desc, char* buf, size_t cap, size_t* len)
+{
+	if (!desc || !len)
+		return fail(VH_ERR_INVALID, "vh_tune_signature arguments");
+	const std::string s = tune_signature(desc);
+	*len = s.size();
+	if (buf) {
+		if (cap <= s.size())
+			return fail(VH_ERR_CAPACITY, "signature needs %zu bytes", s.size() + 1);
+		memcpy(buf, s.data(), s.size() + 1);
+	}
+	return VH_OK;
+}
+
+int vh_tune_cache_put(const char* signature, int32_t variant)
+{
+	if (!signature || variant < 0)
+		return fail(VH_ERR_INVALID, "vh_tune_cache_put arguments");
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	g_tune_cache[signature] = variant;
+	return VH_OK;
+}
+
+int vh_tune_cache_get(const char* signature, int32_t* variant)
+{
+	if (!signature || !variant)
+		return fail(VH_ERR_INVALID, "vh_tune_cache_get arguments");
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	auto it = g_tune_cache.find(signature);
+	*variant = it == g_tune_cache.end() ? -1 : it->second;
+	return VH_OK;
+}
+
+int vh_tune_cache_clear(void)
+{
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	g_tune_cache.clear();
+	return VH_OK;
+}
+
+int vh_tune_best(const vh_pipeline_desc* desc, const vh_config* cfgs, int32_t n, int32_t reps, int32_t* best,
+	int32_t* cached, void* stream)
+{
+	if (!desc || !cfgs || !best || n <= 0 || reps <= 0)
+		return fail(VH_ERR_INVALID, "vh_tune_best arguments");
+	const std::string sig = tune_signature(desc);
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		auto it = g_tune_cache.find(sig);
+		if (it != g_tune_cache.end() && it->second < n) {
+			*best = it->second;
+			if (cached)
+				*cached = 1;
+			return VH_OK;
+		}
+	}
+	std::vector<float> ms((size_t)n);
+	int rc = vh_tune_time(desc, cfgs, n, reps, ms.data(), stream);
+	if (rc)
+		return rc;
+	int b = -1;
+	for (int i = 0; i < n; i++)
+		if (ms[i] >= 0 && (b < 0 || ms[i] < ms[b]))
+			b = i;
+	if (b < 0)
+		return fail(VH_ERR_UNSUPPORTED, "no variant applies to the pipeline");
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		g_tune_cache[sig] = b;
+	}
+	*best = b;
+	if (cached)
+		*cached = 0;
+	return VH_OK;
 }
 
 int vh_mlp_new(vh_mlp** out, int32_t batch_capacity, const float* init_params)

@@ -679,6 +679,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 	// radix-partitioned probe: the last join's table is built partitioned
 	// and the group-by keeps its state per workgroup or in global memory
 	// (the join kernel's workgroups are partitions, not row slabs)
+	if (cfg && cfg->block != 0 && cfg->block != 256 && cfg->block != 512 && cfg->block != 1024)
+		return fail(VH_ERR_INVALID, "block %d: 256, 512 or 1024", cfg->block);
 	const int rcfg = cfg ? cfg->radix : 0;
 	if (rcfg < -1 || rcfg > 1)
 		return fail(VH_ERR_INVALID, "radix %d", rcfg);
@@ -834,6 +836,7 @@ static void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d, size_t budg
 void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* cfg)
 {
 	Plan& P = L.plan;
+	const int lowered_grid = L.grid;   // the per-workgroup partials are sized for it
 	const bool fixed = P.mode == GB_REG || P.mode == GB_PART;
 	const size_t budget = fixed ? 64 * 1024 : std::min<size_t>((size_t)device_lds_per_block(), 160 * 1024);
 	assign_lds_tables(L, d, budget);
@@ -845,6 +848,20 @@ void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* 
 		L.block = 1024;
 		if (!(cfg && cfg->grid > 0))
 			L.grid = std::max(1, std::min(L.grid, device_cus()));
+	}
+	// an explicit workgroup size (waves per CU = block / 64 x workgroups per
+	// CU); the automatic grid then keeps the same threads per CU, at least
+	// one workgroup per CU
+	const int want = cfg ? cfg->block : 0;
+	if (want > 0 && !fixed && want != L.block) {
+		const int per_cu_before = L.block;
+		L.block = want;
+		if (!(cfg && cfg->grid > 0)) {
+			const int64_t threads = (int64_t)L.grid * per_cu_before;
+			L.grid = (int)std::max<int64_t>(1, std::min<int64_t>(threads / want, 4LL * device_cus()));
+			L.grid = std::max(L.grid, std::min(device_cus(), (int)((P.rows + 1023) / 1024)));
+			L.grid = std::min(L.grid, lowered_grid);
+		}
 	}
 }
 

@@ -6,13 +6,15 @@
    (probe / build rows and group counts log-uniform, selectivity uniform,
    key skew theta in {0, 0.5, 1, 1.5}; sizes shrunk to probe rows in
    [1e4, 5e5] and build rows in [1e3, 1e5] so 4,500 x 48 timings fit a few
-   minutes) and times all 48 kernel variants plus the built-in default of
-   each on the GPU.  The reference's explorer does the same over CPU blends
+   minutes), measures each one's features from its data (selector.features),
+   and times all 48 kernel variants plus the built-in default of each on the
+   GPU, and the wall time of that exhaustive search.  The reference's explorer does the same over CPU blends
    and stores the timings in SQLite (explorer.cpp:33-86, compiler.cpp:124-255).
 2. Trains the selector MLP on the training pipelines with the HIP kernels.
-3. Evaluates on the held-out pipelines: the selector's choice (exhaustive
-   fallback below 0.5 confidence) against the exhaustive best, the built-in
-   default heuristic, and the best fixed blend: the one variant (applicable
+3. Evaluates on the held-out pipelines (evaluate): the selector's choice
+   (exhaustive fallback below 0.5 confidence, the search's measured time
+   charged) against the exhaustive search on every pipeline, the built-in
+   default heuristic, the untrained network, and the best fixed blend: the one variant (applicable
    to every pipeline) with the lowest total over the training pipelines,
    used for all of them -- VOILA's default blend, one flavor for every
    pipeline (QueryConfig::default_blend, runtime_framework.hpp:78), as the
@@ -43,6 +45,7 @@ def explore(count, seed, reps=2, log_every=100, out=None, log=None):
     cfgs = selector.variant_configs() + [runtime.make_config()]   # 48 variants + the default
     X = np.zeros((count, selector.N_FEATURES), dtype=np.float32)
     T = np.zeros((count, len(cfgs)), dtype=np.float32)
+    S = np.zeros(count, dtype=np.float32)   # wall seconds of the exhaustive search
     stream = runtime.Stream()
     t0 = time.time()
     for i, prm in enumerate(params):
@@ -51,8 +54,10 @@ def explore(count, seed, reps=2, log_every=100, out=None, log=None):
         b, p = datagen.gen_batch_pipeline(prm, seed=seed, index=i)
         db, dp = runtime.upload(b), runtime.upload(p)
         d = workloads.batch_pipeline(prm, b, p, runtime.addrs(db), runtime.addrs(dp)).desc()
-        X[i] = selector.features(prm)
+        X[i] = selector.features(d, p, "fk")
+        s0 = time.perf_counter()
         T[i] = selector.time_variants(d, reps=reps, stream=stream, configs=cfgs)
+        S[i] = time.perf_counter() - s0
         if log is not None:
             # one row per variant in the reference's run log: the pipeline is
             # the query, the variant its blend (the default last, with mode
@@ -69,8 +74,8 @@ def explore(count, seed, reps=2, log_every=100, out=None, log=None):
         if (i + 1) % log_every == 0:
             print(f"explore seed={seed}: {i + 1}/{count} pipelines, {time.time() - t0:.0f}s", flush=True)
             if out:
-                np.savez(out, X=X[:i + 1], T=T[:i + 1])
-    return X, T, params
+                np.savez(out, X=X[:i + 1], T=T[:i + 1], S=S[:i + 1])
+    return X, T, S, params
 
 
 def fixed_blend(T_train, T_eval):
@@ -82,41 +87,76 @@ def fixed_blend(T_train, T_eval):
     return int(np.argmin(np.where(ok, tot, np.inf)))
 
 
-def evaluate(net, mu, sd, X, T, params, fixed=None):
-    """Selector decision per held-out pipeline, priced with the measured
-    timings: chosen variant (or the exhaustive best when it falls back).
-    fixed: the fixed blend's variant index (see fixed_blend)."""
+def policy(net, mu, sd, X, T):
+    """A network's decision per pipeline: its most probable applicable
+    variant, and whether that probability is below the confidence
+    threshold (then the exhaustive search runs instead)."""
     var = T[:, :selector.N_VARIANTS]
-    default = T[:, selector.N_VARIANTS]
     mask = (var > 0).astype(np.uint8)
-    best = np.where(var > 0, var, np.inf).min(1)
     p = net.predict((X - mu) / sd, mask)
     pick = p.argmax(1)
     conf = p[np.arange(len(p)), pick]
     fallback = (conf < selector.CONFIDENCE) | (mask[np.arange(len(p)), pick] == 0)
-    chosen = np.where(fallback, best, var[np.arange(len(p)), pick])
+    return pick, fallback
+
+
+def evaluate(net, mu, sd, X, T, S, params, fixed=None, runs=100, baseline_net=None):
+    """Price each policy on the held-out pipelines with the measured timings.
+    Every pipeline runs `runs` times (a query executed repeatedly); a policy
+    pays its variant's time per run plus, when it searches, the measured wall
+    time of that pipeline's exhaustive 48-variant search (S, seconds):
+      selector    the network's choice, or the search and its best when the
+                  choice's probability is below 0.5;
+      exhaustive  the search on every pipeline, then its best;
+      fixed blend one variant for every pipeline (see fixed_blend);
+      default     the built-in heuristic (the last timing column).
+    baseline_net: another network (e.g. its random initialisation) priced
+    the same way."""
+    var = T[:, :selector.N_VARIANTS]
+    default = T[:, selector.N_VARIANTS]
+    best = np.where(var > 0, var, np.inf).min(1)
+    search_ms = np.asarray(S, dtype=np.float64) * 1e3
+    idx = np.arange(len(X))
+    pick, fallback = policy(net, mu, sd, X, T)
+    picked = np.where(var[idx, pick] > 0, var[idx, pick], best)
+    chosen = np.where(fallback, best, picked)
     theta = np.array([q["theta"] for q in params])
     skew = theta >= 1.0
-    extra = {}
+    R = float(runs)
+
+    def total(per_run, searched):
+        return float((R * per_run + np.where(searched, search_ms, 0.0)).sum())
+    rep = {
+        "heldout": int(len(X)),
+        "runs_per_pipeline": runs,
+        "fallback_fraction": float(fallback.mean()),
+        "top1_accuracy": float(np.mean(pick == np.where(var > 0, var, np.inf).argmin(1))),
+        "within_10pct_of_best": float(np.mean(chosen <= 1.1 * best)),
+        "run_ms": {"selector": float(chosen.sum()), "selector_no_fallback": float(picked.sum()),
+                   "exhaustive_best": float(best.sum()), "default": float(default.sum())},
+        "search_ms_mean": float(search_ms.mean()),
+        "policy_total_ms": {"selector": total(chosen, fallback), "exhaustive": total(best, np.ones(len(X), bool)),
+                            "default": total(default, np.zeros(len(X), bool))},
+        "selector_run_over_best": float(chosen.sum() / best.sum()),
+        "default_run_over_selector": float(default.sum() / chosen.sum()),
+        "default_run_over_selector_skewed": float(default[skew].sum() / chosen[skew].sum()) if skew.any() else None,
+    }
+    rep["selector_policy_over_exhaustive"] = rep["policy_total_ms"]["selector"] / rep["policy_total_ms"]["exhaustive"]
     if fixed is not None:
         f = var[:, fixed]
-        extra = {"fixed_blend": {"variant": fixed, "total_ms": float(f.sum()),
-                                 "over_selector_total": float(f.sum() / chosen.sum()),
-                                 "over_selector_skewed": float(f[skew].sum() / chosen[skew].sum()) if skew.any() else None,
-                                 "over_best_total": float(f.sum() / best.sum())}}
-    return dict(extra, **{
-        "heldout": int(len(X)),
-        "fallback_fraction": float(fallback.mean()),
-        "chosen_over_best_mean": float(np.mean(chosen / best)),
-        "within_10pct_of_best": float(np.mean(chosen <= 1.1 * best)),
-        "total_ms": {"selector": float(chosen.sum()), "exhaustive_best": float(best.sum()),
-                     "default": float(default.sum())},
-        "selector_vs_best_total": float(chosen.sum() / best.sum()),
-        "default_over_selector_total": float(default.sum() / chosen.sum()),
-        "default_over_selector_skewed": float(default[skew].sum() / chosen[skew].sum()) if skew.any() else None,
-        "default_over_best_skewed": float(default[skew].sum() / best[skew].sum()) if skew.any() else None,
-        "top1_accuracy": float(np.mean(pick == np.where(var > 0, var, np.inf).argmin(1))),
-    })
+        rep["fixed_blend"] = {"variant": int(fixed), "run_ms": float(f.sum()),
+                              "run_over_selector": float(f.sum() / chosen.sum()),
+                              "run_over_selector_skewed": float(f[skew].sum() / chosen[skew].sum())
+                              if skew.any() else None,
+                              "run_over_best": float(f.sum() / best.sum()),
+                              "policy_total_ms": total(f, np.zeros(len(X), bool))}
+    if baseline_net is not None:
+        bp, bf = policy(baseline_net, mu, sd, X, T)
+        bpicked = np.where(var[idx, bp] > 0, var[idx, bp], best)
+        bchosen = np.where(bf, best, bpicked)
+        rep["baseline_net"] = {"run_ms_no_fallback": float(bpicked.sum()), "run_ms": float(bchosen.sum()),
+                               "fallback_fraction": float(bf.mean()), "policy_total_ms": total(bchosen, bf)}
+    return rep
 
 
 def main(argv=None):
@@ -134,26 +174,28 @@ def main(argv=None):
     runtime.init(0)
     if a.timings:
         z = np.load(a.timings)
-        Xtr, Ttr, Xho, Tho = z["Xtr"], z["Ttr"], z["Xho"], z["Tho"]
+        Xtr, Ttr, Str, Xho, Tho, Sho = z["Xtr"], z["Ttr"], z["Str"], z["Xho"], z["Tho"], z["Sho"]
         pho = datagen.batch_params(len(Xho), seed=4242, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
                                    group_range=GROUP_RANGE)
         ptr = datagen.batch_params(len(Xtr), seed=42, probe_range=PROBE_RANGE, build_range=BUILD_RANGE,
                                    group_range=GROUP_RANGE)
     else:
         log = runlog.open_log(a.runlog) if a.runlog else None
-        Xtr, Ttr, ptr = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"), log=log)
-        Xho, Tho, pho = explore(a.heldout, seed=4242, reps=a.reps, out=os.path.join(a.out, "partial_heldout.npz"),
-                                log=log)
-        np.savez(os.path.join(a.out, "timings.npz"), Xtr=Xtr, Ttr=Ttr, Xho=Xho, Tho=Tho)
+        Xtr, Ttr, Str, ptr = explore(a.train, seed=42, reps=a.reps, out=os.path.join(a.out, "partial_train.npz"),
+                                     log=log)
+        Xho, Tho, Sho, pho = explore(a.heldout, seed=4242, reps=a.reps,
+                                     out=os.path.join(a.out, "partial_heldout.npz"), log=log)
+        np.savez(os.path.join(a.out, "timings.npz"), Xtr=Xtr, Ttr=Ttr, Str=Str, Xho=Xho, Tho=Tho, Sho=Sho)
     t0 = time.time()
     net, mu, sd, losses = selector.train(Xtr, Ttr[:, :selector.N_VARIANTS], epochs=a.epochs)
     train_s = time.time() - t0
     print(f"trained {a.epochs} epochs in {train_s:.1f}s: loss {losses[0]:.3f} -> {losses[-1]:.3f}", flush=True)
     np.savez(os.path.join(a.out, "model.npz"), params=net.params(), mu=mu, sd=sd, losses=np.array(losses))
-    rep = evaluate(net, mu, sd, Xho, Tho, pho, fixed=fixed_blend(Ttr, Tho))
+    untrained = selector.MLP(selector._he_init(np.random.default_rng(0)))
+    rep = evaluate(net, mu, sd, Xho, Tho, Sho, pho, fixed=fixed_blend(Ttr, Tho), baseline_net=untrained)
     rep["train"] = {"pipelines": int(len(Xtr)), "epochs": a.epochs, "seconds": train_s,
                     "loss_first": losses[0], "loss_last": losses[-1]}
-    rep["train_fit"] = evaluate(net, mu, sd, Xtr, Ttr, ptr)
+    rep["train_fit"] = evaluate(net, mu, sd, Xtr, Ttr, Str, ptr)
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(rep, f, indent=1)
     print(json.dumps(rep), flush=True)

@@ -60,12 +60,61 @@ def time_variants(desc, reps=3, stream=None, configs=None) -> np.ndarray:
     return ms
 
 
-def features(params: dict, key_width=4) -> np.ndarray:
-    """log10 probe rows, log10 build rows, selectivity, log10 groups, theta,
-    key width (bytes)."""
-    return np.array([math.log10(max(params["probe_rows"], 1)), math.log10(max(params["build_rows"], 1)),
-                     params["selectivity"], math.log10(max(params["groups"], 1)), params["theta"],
-                     key_width], dtype=np.float32)
+SAMPLE = 8192
+_TYPE_BYTES = {abi.VH_U8: 1, abi.VH_I8: 1, abi.VH_U16: 2, abi.VH_I16: 2, abi.VH_U32: 4, abi.VH_I32: 4,
+               abi.VH_U64: 8, abi.VH_I64: 8}
+_CMP = {abi.VH_LT: np.less, abi.VH_LE: np.less_equal, abi.VH_GT: np.greater, abi.VH_GE: np.greater_equal,
+        abi.VH_EQ: np.equal, abi.VH_NE: np.not_equal}
+
+
+def zipf_theta(keys: np.ndarray) -> float:
+    """Skew estimate of a key sample: minus the slope of log(count) against
+    log(rank) over its most frequent keys (those seen at least twice, at
+    most 64 ranks); 0 when no key repeats."""
+    _, counts = np.unique(keys, return_counts=True)
+    c = np.sort(counts)[::-1]
+    c = c[c >= 2][:64]
+    if len(c) < 3:
+        return 0.0
+    x = np.log(np.arange(1, len(c) + 1, dtype=np.float64))
+    y = np.log(c.astype(np.float64))
+    slope = np.polyfit(x, y, 1)[0]
+    return float(min(max(-slope, 0.0), 3.0))
+
+
+def features(desc, probe: dict, key_col: str) -> np.ndarray:
+    """The selector's six features, measured from the pipeline and its host
+    columns: log10 probe rows, log10 build rows of the (last) join, the
+    selectivity of the probe predicates on an evenly strided sample of
+    SAMPLE rows, log10 of the group-key domain (the keys' metadata, as the
+    lowering sizes the group-by), the skew of the probe join key on the
+    same sample (zipf_theta), and the probe key's width in bytes."""
+    n = int(desc.probe.rows)
+    idx = np.linspace(0, max(n - 1, 0), num=min(SAMPLE, max(n, 1))).astype(np.int64) if n else np.zeros(0, np.int64)
+    names = list(probe.keys())
+    keep = np.ones(len(idx), dtype=bool)
+    for k in range(desc.probe.npreds):
+        pr = desc.probe.preds[k]
+        keep &= _CMP[pr.cmp](probe[names[pr.col]][idx], pr.value)
+    sel = float(keep.mean()) if len(idx) else 0.0
+    build_rows = int(desc.joins[desc.njoins - 1].build.rows) if desc.njoins else 0
+    domain = 1.0
+    for k in range(desc.nkeys):
+        v = desc.keys[k]
+        c = desc.probe.cols[v] if v < desc.probe.ncols else None
+        if c is None:   # a join payload: its build column's range
+            base = desc.probe.ncols
+            for j in range(desc.njoins):
+                J = desc.joins[j]
+                if v < base + J.npayload:
+                    c = J.build.cols[J.payload[v - base]]
+                    break
+                base += J.npayload
+        domain *= float(c.max) - float(c.min) + 1
+    theta = zipf_theta(probe[key_col][idx]) if len(idx) else 0.0
+    width = _TYPE_BYTES.get(desc.probe.cols[names.index(key_col)].type, 8)
+    return np.array([math.log10(max(n, 1)), math.log10(max(build_rows, 1)), sel, math.log10(max(domain, 1)),
+                     theta, width], dtype=np.float32)
 
 
 class MLP:
