@@ -512,6 +512,27 @@ int vh_plan_query_new(vh_query** q, vh_db* db, const char* plan, const vh_config
 /* The VOILA statement sequences the plan becomes and how they are fused. */
 int vh_plan_explain(vh_db* db, const char* plan, char* buf, size_t cap, size_t* len);
 
+/* VOILA programs (Program / Pipeline / Lolepop / Statement / Expression /
+ * DataStructure, voila.hpp:142-290) in text form:
+ *   (program DATA... PIPELINE...)
+ *   DATA  (basetable NAME SOURCE (col NAME SOURCE value|key|hash)...)
+ *         (table NAME hashtable|table (FLAG...) (col ...)...), FLAG one of
+ *         threadlocal readafterwrite flushtomaster
+ *   PIPELINE (pipeline (lolepop NAME STMT...)...)
+ *   STMT  (assign VAR EXPR PRED) (effect EXPR) (emit EXPR PRED) (loop PRED STMT...)
+ *         (wrap PRED STMT...) (blend PRED STMT...) (done) (meta KIND [VAR])
+ *   EXPR  (f NAME (ARG...) PRED) (ref NAME) (const TEXT) lolearg lolepred _ (none)
+ * The lowering recognises the statement sequences of the relalg translator
+ * (scan, select, project, HashJoin materialize / build / probe, HashAggr
+ * with flush and re-aggregation; relalg_translator.cpp:202-1115) and maps
+ * the query's probe pipeline and its joins' build pipelines onto one
+ * pipeline descriptor (VH_ERR_UNSUPPORTED for other shapes).  Base tables
+ * resolve against the catalog by their source name.  This is the
+ * Flavor::Hip backend's entry point for a typed Program
+ * (Compiler::_generate, compiler.cpp:343-385). */
+int vh_program_lower(vh_db* db, const char* program, vh_pipeline_desc* out);
+int vh_program_query_new(vh_query** q, vh_db* db, const char* program, const vh_config* cfg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvoila_hip.so")
 SOURCES = ["vh_pipeline.hip", "vh_primitives.hip", "vh_varchar.hip", "vh_batch.hip", "vh_mlp.hip", "vh_engine.cpp",
            "vh_capi.cpp", "vh_capi_prims.cpp", "vh_batch_host.cpp", "vh_capi_tune.cpp", "vh_relalg.cpp",
-           "vh_jit.cpp"]
+           "vh_jit.cpp", "vh_voila.cpp"]
 # Headers the generated kernels include, embedded in the library as text
 # (name as #included, path): hipRTC compiles against these copies.
 JIT_HEADERS = [("vh_fused.hpp", os.path.join(CSRC, "vh_fused.hpp")),

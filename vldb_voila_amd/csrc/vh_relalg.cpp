@@ -24,80 +24,12 @@
 
 using namespace vh;
 
-namespace {
+#include "vh_sexpr.hpp"
 
-// ---- S-expressions -----------------------------------------------------------
-
-struct Sx {
-	std::string atom;                 // empty for a list
-	std::vector<std::shared_ptr<Sx>> list;
-	bool is_list() const { return atom.empty(); }
-};
-typedef std::shared_ptr<Sx> SxP;
-
-struct Parser {
-	const char* s;
-	std::string err;
-	void ws()
-	{
-		while (*s && (isspace((unsigned char)*s) || *s == ';')) {
-			if (*s == ';')
-				while (*s && *s != '\n')
-					s++;
-			else
-				s++;
-		}
-	}
-	SxP parse()
-	{
-		ws();
-		if (!*s) {
-			err = "unexpected end of plan";
-			return nullptr;
-		}
-		if (*s == '(') {
-			s++;
-			auto n = std::make_shared<Sx>();
-			for (;;) {
-				ws();
-				if (!*s) {
-					err = "missing ')'";
-					return nullptr;
-				}
-				if (*s == ')') {
-					s++;
-					break;
-				}
-				auto c = parse();
-				if (!c)
-					return nullptr;
-				n->list.push_back(c);
-			}
-			return n;
-		}
-		if (*s == ')') {
-			err = "unexpected ')'";
-			return nullptr;
-		}
-		auto n = std::make_shared<Sx>();
-		while (*s && !isspace((unsigned char)*s) && *s != '(' && *s != ')')
-			n->atom.push_back(*s++);
-		return n;
-	}
-};
-
-// ---- catalog -----------------------------------------------------------------
-
-struct Table {
-	int64_t rows = -1;
-	std::map<std::string, vh_column> cols;
-};
-
-} // namespace
-
-struct vh_db {
-	std::map<std::string, Table> tables;
-};
+using vh::Sx;
+using vh::SxP;
+using vh::Parser;
+using vh::CatalogTable;
 
 namespace {
 
@@ -494,7 +426,7 @@ int vh_db_add_column(vh_db* db, const char* table, const char* column, int32_t t
 {
 	if (!db || !table || !column || type < VH_U8 || type > VH_I64 || rows < 0 || (rows && !data) || min > max)
 		return fail(VH_ERR_INVALID, "vh_db_add_column arguments");
-	Table& t = db->tables[table];
+	CatalogTable& t = db->tables[table];
 	if (t.rows >= 0 && t.rows != rows)
 		return fail(VH_ERR_INVALID, "table %s: column %s has %lld rows, table has %lld", table, column,
 			(long long)rows, (long long)t.rows);

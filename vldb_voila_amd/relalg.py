@@ -26,6 +26,7 @@ def _bind():
     L.vh_plan_lower.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
     L.vh_plan_query_new.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_char_p, C.c_void_p]
     L.vh_plan_explain.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.vh_program_lower.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
     return L
 
 
@@ -56,6 +57,16 @@ class Database:
         d = abi.vh_pipeline_desc()
         check(self.L.vh_plan_lower(self.ptr, plan.encode(), C.byref(d)))
         return d
+
+    def lower_program(self, program: str) -> abi.vh_pipeline_desc:
+        """A VOILA program (the text form of include/voila_hip.h) lowered to
+        the pipeline descriptor its probe pipeline and joins make."""
+        d = abi.vh_pipeline_desc()
+        check(self.L.vh_program_lower(self.ptr, program.encode(), C.byref(d)))
+        return d
+
+    def query_program(self, program: str, config=None) -> Query:
+        return Query(self.lower_program(program), config)
 
     def explain(self, plan: str) -> str:
         n = C.c_size_t()
