@@ -65,6 +65,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every usable host core, at most 16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and so the result check)")
+    ap.add_argument("--tuned-layout-only", action="store_true",
+                    help="with --tuned: take only the group-by strategy and table layout of each pick")
+    ap.add_argument("--tuned", action="store_true",
+                    help="batch: per-pipeline variants from the learned selector (autotuner below 0.5 "
+                         "confidence), timed beside the default configuration")
     return ap.parse_args()
 
 
@@ -417,6 +422,27 @@ def bench_batch(args):
     queries = [runtime.Query(d, runtime.make_config(kernel=1)) for d in descs]
     queries_gen = [runtime.Query(d) for d in descs]
     batch = runtime.Batch(descs)
+    tuned = None
+    if args.tuned:
+        # per-pipeline variants: the selector's pick, or the autotuner's
+        # exhaustive timing when the network is not confident
+        from vldb_voila_amd import selector
+        net, mu, sd = selector.load_model()
+        cfgs, fallbacks = [], 0
+        t_tune = time.perf_counter()
+        for i, d in enumerate(descs):
+            cfg, v, fb = selector.pick_config(net, mu, sd, d, hosts[i][2], "fk", stream=stream, batch=True)
+            if args.tuned_layout_only:
+                # the variant's group-by strategy and table layout; the batch's
+                # own launch shape (work items, rows per lane)
+                cfg.tile_shift, cfg.unroll, cfg.block, cfg.grid = 0, 0, 0, 0
+            cfgs.append(cfg)
+            fallbacks += fb
+        t_tune = time.perf_counter() - t_tune
+        tuned_batch = runtime.Batch(descs, cfgs)
+        tuned_batch.run_graph(stream)
+        stream.synchronize()
+        tuned = (tuned_batch, fallbacks, t_tune)
     for _ in range(max(args.warmup, 1)):
         batch.run_graph(stream)
         for q in queries:
@@ -426,6 +452,9 @@ def bench_batch(args):
     seq_rows = [sorted(q.result(stream).splitlines()) for q in queries]
     if gpu_rows != seq_rows:
         print(json.dumps({"error": "batched and sequential results differ"}), flush=True)
+        sys.exit(1)
+    if tuned and [sorted(tuned[0].result(i, stream).splitlines()) for i in range(len(descs))] != gpu_rows:
+        print(json.dumps({"error": "the tuned batch's results differ"}), flush=True)
         sys.exit(1)
     for q in queries_gen:
         q.run(stream)
@@ -453,6 +482,18 @@ def bench_batch(args):
     batched_ms = e0.elapsed_ms(e1) / K
     seq_ms = e1.elapsed_ms(e2) / K
     seq_gen_ms = e2.elapsed_ms(e3) / K
+    tuned_info = None
+    if tuned:
+        t0e, t1e = runtime.Event(), runtime.Event()
+        t0e.record(stream)
+        for _ in range(K):
+            tuned[0].run_graph(stream)
+        t1e.record(stream)
+        stream.synchronize()
+        tuned_ms = t0e.elapsed_ms(t1e) / K
+        tuned_info = {"ms": tuned_ms, "speedup_vs_default": batched_ms / tuned_ms,
+                      "autotuner_fallbacks": tuned[1], "tuning_seconds": tuned[2],
+                      "selector": "vldb_voila_amd/data/selector_model.npz"}
     binfo = batch.info()
     # every probe row's columns (fk 4 + a 4 + g 4 + v 8 + w 8 B) and every
     # build row's key and payload (8 B)
@@ -466,7 +507,8 @@ def bench_batch(args):
                    "input_tuples": tuples, "batch": binfo, "sequential_ms": seq_ms,
                    "speedup_vs_sequential": seq_ms / batched_ms,
                    "sequential_generated_ms": seq_gen_ms,
-                   "speedup_vs_sequential_generated": seq_gen_ms / batched_ms},
+                   "speedup_vs_sequential_generated": seq_gen_ms / batched_ms,
+                   "tuned": tuned_info},
         "roofline": {"bound": "hbm", "kernel": "whole batch (clear + builds + probes)",
                      "achieved": alg / (batched_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -35,6 +36,7 @@ def _bind():
     L.vh_mlp_predict.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     L.vh_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
                                     C.c_void_p, C.POINTER(C.c_float), C.c_void_p]
+    L.vh_tune_best.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     return L
 
 
@@ -232,3 +234,53 @@ def choose(net, mu, sd, feats, mask, fallback):
     if p[v] >= CONFIDENCE and mask[v]:
         return v, False
     return int(fallback()), True
+
+
+MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector_model.npz")
+
+
+def load_model(path=MODEL):
+    """(network, mu, sd) of a trained selector saved by the explorer
+    (model.npz: params, mu, sd)."""
+    z = np.load(path)
+    return MLP(z["params"]), z["mu"].astype(np.float32), z["sd"].astype(np.float32)
+
+
+def tune_best(desc, reps=2, stream=None, configs=None):
+    """The autotuner through its cache (vh_tune_best): (variant index, cached)."""
+    init()
+    L = _bind()
+    cfgs = configs or variant_configs()
+    arr = (abi.vh_config * len(cfgs))(*cfgs)
+    best, cached = C.c_int32(), C.c_int32()
+    check(L.vh_tune_best(C.byref(desc), C.addressof(arr), len(cfgs), reps, C.byref(best), C.byref(cached),
+                         stream.ptr if stream else None))
+    return best.value, bool(cached.value)
+
+
+def applicable(desc, configs=None, batch=False) -> np.ndarray:
+    """Which variants lower for the pipeline (host-side lowering only, as a
+    single query or as a batch's pipeline): the selector's mask."""
+    L = lib()
+    fn = L.vh_generate_batch_source if batch else L.vh_generate_source
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    cfgs = configs or variant_configs()
+    n = C.c_size_t()
+    return np.array([fn(C.byref(desc), C.byref(c), None, 0, C.byref(n)) == 0 for c in cfgs], dtype=np.uint8)
+
+
+def pick_config(net, mu, sd, desc, probe: dict, key_col: str, configs=None, reps=2, stream=None, batch=False):
+    """The selector's variant for a pipeline among those that apply, or the
+    autotuner's (exhaustive timing, cached per signature) when the network
+    is below 0.5 confidence.  Returns (config, variant, used_fallback)."""
+    cfgs = configs or variant_configs()
+    x = features(desc, probe, key_col)
+    mask = applicable(desc, cfgs, batch=batch)
+    p = net.predict(((x - mu) / sd)[None, :], mask[None, :])[0]
+    v = int(p.argmax())
+    if p[v] >= CONFIDENCE and mask[v]:
+        return cfgs[v], v, False
+    v, _ = tune_best(desc, reps=reps, stream=stream, configs=cfgs)
+    if batch and not mask[v]:
+        v = int(np.argmax(np.where(mask > 0, p, -1.0)))   # the fastest alone does not apply in a batch
+    return cfgs[v], v, True
