@@ -14,47 +14,10 @@
 #include "vh_batch.hpp"
 #include "vh_device.hpp"
 #include "vh_fused.hpp"
+#include "vh_kernels.hpp"
 #include "vh_plan.hpp"
 
 namespace vh {
-
-template <int MODE>
-__device__ __forceinline__ void probe_item_m(const Plan& P, i64 r0, i64 r1, u64* lds, u32 dups_mask)
-{
-	Lds L = lds_layout<MODE>(P, lds);
-	if (MODE == GB_PART) {
-		// an item is one region: its rows' tuple space and table row
-		L.rbase = r0;
-		L.rcap = (u32)(r1 - r0);
-		L.region = (u32)(r0 / P.gregion_rows);
-	}
-	lds_init<MODE>(P, L);
-	__syncthreads();
-	process_range<MODE>(P, L, r0, r1, (i64)kU * blockDim.x, dups_mask);
-	__syncthreads();
-	if (MODE == GB_LDS)
-		lds_dense_flush(P, L);
-	else if (MODE == GB_LDSHASH)
-		ldshash_flush(P, L);
-	else if (MODE == GB_PART)
-		part_finish(P, L);
-	__syncthreads();
-}
-
-// A workgroup's next item of this launch, or false when its queue is empty.
-__device__ __forceinline__ bool next_item(const BatchItem* __restrict__ items, u32 first, u32 count,
-	u32* __restrict__ counter, u32* slot, BatchItem& I)
-{
-	if (threadIdx.x == 0)
-		*slot = gatomic_add(counter, 1u);
-	__syncthreads();
-	const u32 k = *slot;
-	__syncthreads();
-	if (k >= count)
-		return false;
-	I = items[first + k];
-	return true;
-}
 
 // Items [first, first + count) of the queue; *counter is this launch's
 // cursor.  One kernel per kind of item (and per group-by strategy), so each
@@ -64,24 +27,10 @@ __global__ __launch_bounds__(256) void batch_build_kernel(const BuildPlan* __res
 {
 	__shared__ u32 slot;
 	BatchItem I;
-	while (next_item(items, first, count, counter, &slot, I)) {
+	while (next_batch_item(items, first, count, counter, &slot, I)) {
 		const BuildPlan& B = builds[I.build];
 		for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
 			build_chunk(B, base, I.r1);
-	}
-}
-
-template <int MODE>
-__global__ __launch_bounds__(256) void batch_probe_kernel(const Plan* __restrict__ plans,
-	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
-{
-	extern __shared__ u64 lds[];
-	__shared__ u32 slot;
-	BatchItem I;
-	while (next_item(items, first, count, counter, &slot, I)) {
-		const Plan& P = plans[I.plan];
-		const u32 dups_mask = join_flags(P);
-		probe_item_m<MODE>(P, I.r0, I.r1, lds, dups_mask);
 	}
 }
 
@@ -93,7 +42,7 @@ __global__ __launch_bounds__(256) void batch_agg_kernel(const Plan* __restrict__
 	extern __shared__ u64 lds[];
 	__shared__ u32 slot;
 	BatchItem I;
-	while (next_item(items, first, count, counter, &slot, I)) {
+	while (next_batch_item(items, first, count, counter, &slot, I)) {
 		const Plan& P = plans[I.plan];
 		const u64 slice_words = (((u64)1 << P.gshift) * ((u64)P.nsums * 12 + 4) + 7) / 8;
 		const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
@@ -130,8 +79,7 @@ hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds
 {
 	if (!count)
 		return hipSuccess;
-#define VH_PROBE(M) hipLaunchKernelGGL((batch_probe_kernel<M>), dim3(grid), dim3(256), lds_bytes, s, d_plans, \
-	d_items, first, count, d_counter)
+#define VH_PROBE(M) return launch_batch_probe_mode<M>(d_plans, d_items, first, count, d_counter, grid, lds_bytes, s)
 	switch (kind) {
 	case kBatchBuild:
 		hipLaunchKernelGGL(batch_build_kernel, dim3(grid), dim3(256), 0, s, d_builds, d_items, first, count,
@@ -141,11 +89,11 @@ hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds
 		hipLaunchKernelGGL(batch_agg_kernel, dim3(grid), dim3(256), lds_bytes, s, d_plans, d_items, first, count,
 			d_counter);
 		break;
-	case GB_LDS: VH_PROBE(GB_LDS); break;
-	case GB_LDSHASH: VH_PROBE(GB_LDSHASH); break;
-	case GB_GLOBAL: VH_PROBE(GB_GLOBAL); break;
-	case GB_PART: VH_PROBE(GB_PART); break;
-	default: VH_PROBE(GB_HASH); break;
+	case GB_LDS: VH_PROBE(GB_LDS);
+	case GB_LDSHASH: VH_PROBE(GB_LDSHASH);
+	case GB_GLOBAL: VH_PROBE(GB_GLOBAL);
+	case GB_PART: VH_PROBE(GB_PART);
+	default: VH_PROBE(GB_HASH);
 	}
 #undef VH_PROBE
 	return hipGetLastError();

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "vh_device.hpp"
 #include "vh_plan.hpp"
 
 namespace vh {
@@ -24,5 +25,21 @@ static constexpr int kBatchAgg = -2;
 // Items [first, first + count) of one kind with the launch's own queue counter.
 hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds, const BatchItem* d_items,
 	u32 first, u32 count, u32* d_counter, int grid, size_t lds_bytes, hipStream_t s);
+
+// A workgroup's next item of a persistent launch, or false when its queue is
+// empty (*counter is the launch's cursor, *slot a word of the workgroup's LDS).
+__device__ __forceinline__ bool next_batch_item(const BatchItem* __restrict__ items, u32 first, u32 count,
+	u32* __restrict__ counter, u32* slot, BatchItem& I)
+{
+	if (threadIdx.x == 0)
+		*slot = gatomic_add(counter, 1u);
+	__syncthreads();
+	const u32 k = *slot;
+	__syncthreads();
+	if (k >= count)
+		return false;
+	I = items[first + k];
+	return true;
+}
 
 } // namespace vh

@@ -28,6 +28,32 @@ static constexpr int kMaxSliceBytes = 32768;     // LDS slice of a partition
 // launch_pipeline runs the generic fused kernel; GB_REG / GB_LDS then need
 // launch_merge.
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s);
+// The generic kernels of one group-by strategy (vh_modes.hip, one translation
+// unit per strategy): the pipeline kernel with `lds` bytes of dynamic LDS,
+// and the batched probe kernel over work items (every strategy but GB_REG).
+template <int MODE>
+hipError_t launch_pipeline_mode(const Plan* dP, int grid, size_t lds, hipStream_t s);
+template <int MODE>
+hipError_t launch_batch_probe_mode(const Plan* d_plans, const BatchItem* d_items, u32 first, u32 count,
+	u32* d_counter, int grid, size_t lds_bytes, hipStream_t s);
+#define VH_DECLARE_MODE(M) \
+	template <> hipError_t launch_pipeline_mode<M>(const Plan*, int, size_t, hipStream_t);
+VH_DECLARE_MODE(GB_REG)
+VH_DECLARE_MODE(GB_LDS)
+VH_DECLARE_MODE(GB_GLOBAL)
+VH_DECLARE_MODE(GB_HASH)
+VH_DECLARE_MODE(GB_LDSHASH)
+VH_DECLARE_MODE(GB_PART)
+#undef VH_DECLARE_MODE
+#define VH_DECLARE_MODE(M) \
+	template <> hipError_t launch_batch_probe_mode<M>(const Plan*, const BatchItem*, u32, u32, u32*, int, size_t, \
+		hipStream_t);
+VH_DECLARE_MODE(GB_LDS)
+VH_DECLARE_MODE(GB_GLOBAL)
+VH_DECLARE_MODE(GB_HASH)
+VH_DECLARE_MODE(GB_LDSHASH)
+VH_DECLARE_MODE(GB_PART)
+#undef VH_DECLARE_MODE
 // GB_REG / GB_LDS: merge the per-block partials of a `grid`-block launch
 hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s);
 // Dynamic LDS of the probe kernel: the group-by state, then the join tables

@@ -83,33 +83,6 @@ __global__ __launch_bounds__(kBlock) void merge_kernel(const Plan* __restrict__ 
 
 static constexpr int kPartBlockAgg = 1024;   // workgroup size of part_agg_kernel
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void pipeline_kernel(const Plan* __restrict__ Pp)
-{
-	const Plan& P = *Pp;
-	extern __shared__ u64 lds[];
-
-	Lds L = lds_layout<MODE>(P, lds);
-	lds_init<MODE>(P, L);
-	const u32 dups_mask = join_flags(P);
-	__syncthreads();
-
-	const i64 tile = (i64)kU * blockDim.x;
-	if (MODE == GB_PART) {
-		// a contiguous slab of rows per workgroup: its region of tuple space
-		const i64 r0 = (i64)blockIdx.x * P.gregion_rows;
-		const i64 r1 = r0 + P.gregion_rows < P.rows ? r0 + P.gregion_rows : P.rows;
-		L.rbase = r0;
-		L.rcap = r1 > r0 ? (u32)(r1 - r0) : 0u;
-		L.region = blockIdx.x;
-		process_range<MODE>(P, L, r0, r1, tile, dups_mask);
-	} else {
-		process_range<MODE>(P, L, (i64)blockIdx.x * tile, P.rows, (i64)gridDim.x * tile, dups_mask);
-	}
-
-	block_epilogue<MODE>(P, L);
-}
-
 __global__ __launch_bounds__(kPartBlockAgg) void part_agg_kernel(const Plan* __restrict__ Pp)
 {
 	extern __shared__ u64 slice[];
@@ -643,17 +616,16 @@ hipError_t launch_merge(const Plan& P, const Plan* dP, int grid, hipStream_t s)
 hipError_t launch_pipeline(const Plan& P, const Plan* dP, int grid, hipStream_t s)
 {
 	const size_t lds = pipeline_lds_bytes(P);
-#define VH_LAUNCH(M) hipLaunchKernelGGL((pipeline_kernel<M>), dim3(grid), dim3(kBlock), lds, s, dP)
+#define VH_LAUNCH(M) return launch_pipeline_mode<M>(dP, grid, lds, s)
 	switch (P.mode) {
-	case GB_REG: VH_LAUNCH(GB_REG); break;
-	case GB_LDS: VH_LAUNCH(GB_LDS); break;
-	case GB_GLOBAL: VH_LAUNCH(GB_GLOBAL); break;
-	case GB_LDSHASH: VH_LAUNCH(GB_LDSHASH); break;
-	case GB_PART: VH_LAUNCH(GB_PART); break;
-	default: VH_LAUNCH(GB_HASH); break;
+	case GB_REG: VH_LAUNCH(GB_REG);
+	case GB_LDS: VH_LAUNCH(GB_LDS);
+	case GB_GLOBAL: VH_LAUNCH(GB_GLOBAL);
+	case GB_LDSHASH: VH_LAUNCH(GB_LDSHASH);
+	case GB_PART: VH_LAUNCH(GB_PART);
+	default: VH_LAUNCH(GB_HASH);
 	}
 #undef VH_LAUNCH
-	return hipGetLastError();
 }
 
 } // namespace vh
