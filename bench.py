@@ -44,8 +44,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of each workload's probe kernel, from rocprofv3
 # FETCH_SIZE + WRITE_SIZE runs scaled by a copy of known size; each file
 # gives the commands and the counter values
-TRAFFIC_FILES = {"join": "traffic_join_1.5Mx6M.md", "filter": "traffic_filter_groupby_6M.md",
-                 "star": "traffic_star_60M.md", "contains": "traffic_contains_10M.md"}
+TRAFFIC_FILES = {"join": "traffic_join_1.5Mx6M.md", "join_sparse": "traffic_join_sparse_1.5Mx6M.md",
+                 "filter": "traffic_filter_groupby_6M.md", "star": "traffic_star_60M.md",
+                 "contains": "traffic_contains_10M.md"}
 METRIC = "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend"
 
 
@@ -264,7 +265,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.workload, args.zipf),
+            # the records are of the default configuration
+            "traffic": None if (args.cfg or args.part_shift or args.join_layout) else
+            measured_traffic(args.workload, args.zipf),
             "avg_kernel_ms": probe_avg_ms,
             "min_kernel_ms": probe_ms[0],
             "median_kernel_ms": probe_ms[len(probe_ms) // 2],
