@@ -105,16 +105,17 @@ def split_radix(src):
 
 def test_radix_probe_sources():
     """A last join over a partitioned hash table with radix = 1: a probe
-    kernel that writes partitioned tuples and a join kernel over LDS
-    sub-tables, for every group-by strategy it takes, with the carried values
-    (a probe column, an earlier join's payload)."""
+    kernel that stages each chunk's tuples in LDS and stores them sorted by
+    partition, and a join kernel over the partitions' LDS copies, for every
+    group-by strategy it takes, with the carried values (a probe column, an
+    earlier join's payload)."""
     import numpy as np
     b, p = datagen.gen_join_sparse(600_000, 6000, seed=5)
     d = workloads.join_aggregate_pipeline(b, p).desc()
     assert "vh_generated_join" not in source(d)
     probe, join = split_radix(source(d, radix=1))
-    assert "radix_slot<2>" in probe and "radix_finish<2>" in probe
-    assert "vh_generated_join" in join and "tab[h[u]]" in join
+    assert "radix_flush_begin" in probe and "sv0[pos]" in probe and "sv1[pos]" not in probe
+    assert "vh_generated_join" in join and "tab[h[u]]" in join and "rv0[x]" in join
     rng = np.random.default_rng(2)
     dim = {"k": np.arange(1, 101, dtype=np.int32), "n": rng.integers(0, 7, 100).astype(np.int16)}
     fact = {"a": rng.integers(1, 101, 500).astype(np.int32), "fk": p["fk"][:500],
@@ -131,7 +132,7 @@ def test_radix_probe_sources():
                 rc, log, size = compile_for_gfx950(part)
                 assert rc == 0 and size > 0, (mode, log[:2000])
     probe, join = split_radix(source(d2, radix=1))
-    assert "radix_slot<3>" in probe and "pay0[u]" in join
+    assert "sv1[pos]" in probe and "pay0[u] = rv0[x]" in join
 
 
 def test_source_reflects_the_plan():

@@ -226,9 +226,10 @@ int vh_generate_source(const vh_pipeline_desc* desc, const vh_config* cfg, char*
 	const int kernel = cfg ? cfg->kernel : VH_KERNEL_AUTO;
 	if (kernel != VH_KERNEL_GENERIC)
 		choose_probe_shape(L, desc, cfg);
-	std::string src = jit_source(L.plan, jit_eager(kernel), jit_unroll(cfg), jit_pipeline(kernel), L.block);
+	std::string src = jit_source(L.plan, jit_eager(kernel), L.plan.radix ? L.radix_unroll : jit_unroll(cfg),
+		jit_pipeline(kernel), L.block);
 	if (L.plan.radix)
-		src += jit_join_source(L.plan, jit_unroll(cfg), 256);
+		src += jit_join_source(L.plan, jit_unroll(cfg), L.radix_block);
 	*len = src.size();
 	if (buf) {
 		if (cap <= src.size())

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -833,6 +834,72 @@ static void assign_lds_tables(Lowered& L, const vh_pipeline_desc* d, size_t budg
 // dependent lookups), and the tables may use the whole LDS of a workgroup
 // (160 KB on gfx950).  Lane-private and partitioned group-bys keep 256
 // threads: their LDS and row slabs are laid out for that.
+// Radix-partitioned probe: workgroups of 512 threads (or the configured
+// size) take chunks of U * block rows whose tuples fit 64 KiB of LDS
+// staging, two workgroups per CU (U = 8 for tuples of 12 bytes, a key
+// offset and an int64).  A probe partition is the largest aligned block of
+// the table, of at least one of its build's sub-tables and at most 2^14
+// slots (128 KiB), whose LDS copy fits beside the group-by state and the
+// run tables; the join kernel runs one 1024-thread workgroup per partition.
+// Where no such layout exists the one-pass probe runs instead (P.radix = 0).
+static void choose_radix_shape(Lowered& L, const vh_config* cfg)
+{
+	Plan& P = L.plan;
+	const size_t lds_max = 160 * 1024;   // gfx950: LDS per workgroup
+	const size_t stage_max = 64 * 1024;
+	auto r16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+	const JoinT& J = P.joins[P.njoins - 1];
+	std::vector<int> widths;
+	const int tb = jit_radix_values(P, widths);
+	const int block = (cfg && cfg->block > 0) ? cfg->block : 512;
+	int U = jit_unroll(cfg);
+	if (!U) {
+		// a tuple's bytes in LDS: key offset, values and a 2-byte sort position
+		const size_t per_row = (size_t)tb + 2;
+		U = per_row * 8 * block <= stage_max ? 8 : (per_row * 4 * block <= stage_max ? 4 : 2);
+	}
+	const u64 CR = (u64)U * (u64)block;
+	const u64 RC = ((u64)std::max<int64_t>(P.rows, 0) + CR - 1) / CR + 1;
+	const int tshift = 63 - __builtin_clzll(J.mask + 1);
+	const int bshift = 63 - __builtin_clzll(J.smask + 1);
+	const size_t gb = r16(groupby_lds_bytes(P));
+	int rps = -1;
+	for (int sh = std::min(tshift, 14); sh >= bshift; sh--)
+		if (gb + ((size_t)8 << sh) + (2 * RC + 2) * 4 <= lds_max) {
+			rps = sh;
+			break;
+		}
+	bool ok = rps >= 0 && widths.size() <= (size_t)kRadixMaxVals && RC <= (u64)kRadixMaxChunks &&
+		RC * CR < ((u64)1 << 32);
+	if (ok) {
+		P.rparts = (int)((J.mask + 1) >> rps);
+		P.rlds_a = (u32)r16(pipeline_lds_bytes(P));
+		const size_t stage = jit_radix_stage_bytes(P, CR);
+		ok = P.rparts <= 4096 && P.rlds_a + stage <= lds_max;
+		if (ok) {
+			P.rpshift = rps;
+			P.rchunk = (i64)CR;
+			P.rchunks = (int)RC;
+			P.rw = (int)widths.size();
+			P.rlds_b = (u32)gb;
+			L.radix_lds_a = P.rlds_a + stage;
+			L.radix_lds_b = gb + ((size_t)8 << rps) + (2 * RC + 2) * 4;
+			L.radix_unroll = U;
+			L.radix_block = 1024;
+			L.radix_grid = P.rparts;
+			L.block = block;
+			if (!(cfg && cfg->grid > 0)) {
+				// workgroups a CU holds: its LDS, and 32 waves
+				const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(lds_max / L.radix_lds_a,
+					(size_t)(2048 / block)));
+				L.grid = (int)std::max<u64>(1, std::min<u64>(RC, (u64)per_cu * (u64)device_cus()));
+			}
+		}
+	}
+	if (!ok)
+		P.radix = 0;
+}
+
 void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* cfg)
 {
 	Plan& P = L.plan;
@@ -863,6 +930,8 @@ void choose_probe_shape(Lowered& L, const vh_pipeline_desc* d, const vh_config* 
 			L.grid = std::min(L.grid, lowered_grid);
 		}
 	}
+	if (P.radix)
+		choose_radix_shape(L, cfg);
 }
 
 // From the generated probe kernel(s) to the generic one, which computes the
@@ -1040,39 +1109,20 @@ int query_create(Query* q, bool single)
 		}
 	}
 
-	// radix-partitioned probe: generated kernels only; its tuple space, run
-	// tables and both launches' LDS
+	// radix-partitioned probe: generated kernels only (its shape is set by
+	// choose_probe_shape); its tuple arrays and run table
 	if (P.radix && !(single && q->cfg.kernel != VH_KERNEL_GENERIC))
 		P.radix = 0;
 	if (P.radix) {
-		const JoinT& J = P.joins[P.njoins - 1];
-		const u64 S = J.smask + 1;
-		const u64 NP = (J.mask + 1) / S;
-		const u64 R = (u64)L.grid;
-		const size_t lds_max = (size_t)std::min(device_lds_per_block(), 160 * 1024);
-		P.rw = jit_radix_words(P);
-		P.rregions = (int)R;
-		P.rlds_a = (u32)((pipeline_lds_bytes(P) + 15) & ~(size_t)15);
-		P.rlds_b = (u32)((groupby_lds_bytes(P) + 15) & ~(size_t)15);
-		q->radix_lds_a = P.rlds_a + (2 * NP + 2) * sizeof(u32);
-		q->radix_lds_b = P.rlds_b + S * sizeof(u64) + (2 * R + 2) * sizeof(u32);
-		// a workgroup's rows: at most one tile per grid stride of the
-		// generated loops plus the last partial chunk (tile bound: 16 rows
-		// per lane)
-		const u64 tile = 16 * (u64)L.block;
-		const u64 rows = (u64)std::max<int64_t>(P.rows, 1);
-		P.rcap = (i64)(((rows + R * tile - 1) / (R * tile) + 2) * tile);
-		if (P.rw > kRadixMaxWords || q->radix_lds_a > lds_max || q->radix_lds_b > lds_max || R > 4096 ||
-			NP > (u64)kMaxBuildParts || R * (u64)P.rcap >= ((u64)1 << 32)) {
-			P.radix = 0;
-		} else {
-			const size_t tup = (size_t)(R * (u64)P.rcap * (u64)P.rw * sizeof(u64));
-			if ((rc = q->alloc((void**)&P.rtup0, tup)) || (rc = q->alloc((void**)&P.rtup1, tup)) ||
-				(rc = q->alloc((void**)&P.rcnt, NP * R * sizeof(u32))) ||
-				(rc = q->alloc((void**)&P.roff, NP * R * sizeof(u32))))
+		std::vector<int> widths;
+		jit_radix_values(P, widths);
+		const u64 tuples = (u64)P.rchunks * (u64)P.rchunk;
+		if ((rc = q->alloc((void**)&P.rkey, tuples * sizeof(u32))) ||
+			(rc = q->alloc((void**)&P.rend, (u64)P.rchunks * (u64)P.rparts * sizeof(u32))))
+			return rc;
+		for (size_t k = 0; k < widths.size(); k++)
+			if ((rc = q->alloc(&P.rval[k], tuples * (u64)widths[k])))
 				return rc;
-			q->radix_parts = (int)NP;
-		}
 	}
 
 	// device copies of the plans
@@ -1091,11 +1141,12 @@ int query_create(Query* q, bool single)
 	if (kernel < VH_KERNEL_AUTO || kernel > VH_KERNEL_JIT_PIPE)
 		return fail(VH_ERR_INVALID, "kernel %d", kernel);
 	if (single && kernel != VH_KERNEL_GENERIC) {
-		q->jit_src = jit_source(P, jit_eager(kernel), jit_unroll(&q->cfg), jit_pipeline(kernel), L.block);
+		q->jit_src = jit_source(P, jit_eager(kernel), P.radix ? L.radix_unroll : jit_unroll(&q->cfg),
+			jit_pipeline(kernel), L.block);
 		std::string log;
 		rc = jit_compile(q->jit_src, &q->jit_fn, log);
 		if (!rc && P.radix) {
-			q->jit_join_src = jit_join_source(P, jit_unroll(&q->cfg), 256);
+			q->jit_join_src = jit_join_source(P, jit_unroll(&q->cfg), L.radix_block);
 			rc = jit_compile(q->jit_join_src, &q->jit_join_fn, log, "vh_generated_join");
 			if (rc && kernel == VH_KERNEL_AUTO) {
 				// the one-pass probe computes the same result
@@ -1149,9 +1200,9 @@ int query_run(Query* q, hipStream_t s, int phase)
 			// radix-partitioned probe: partitioned tuples, then one join
 			// workgroup per partition (the stamps span both launches)
 			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s, q->probe_start, nullptr,
-				(long)q->radix_lds_a));
-			HIP_TRY(launch_generated(q->jit_join_fn, P, q->dplan, q->radix_parts, 256, s, nullptr, q->probe_stop,
-				(long)q->radix_lds_b));
+				(long)L.radix_lds_a));
+			HIP_TRY(launch_generated(q->jit_join_fn, P, q->dplan, L.radix_grid, L.radix_block, s, nullptr,
+				q->probe_stop, (long)L.radix_lds_b));
 		} else if (q->jit_fn)
 			HIP_TRY(launch_generated(q->jit_fn, P, q->dplan, L.grid, L.block, s, q->probe_start, q->probe_stop));
 		else

@@ -38,6 +38,13 @@ struct Lowered {
 	bool wide = false;
 	int grid = 0;
 	int block = 256;                     // generated probe kernel: threads per workgroup
+	// radix-partitioned probe (Plan::radix): rows per lane of its probe
+	// kernel, both launches' LDS bytes, the join kernel's workgroup size and
+	// grid (one workgroup per partition)
+	int radix_unroll = 0;
+	size_t radix_lds_a = 0, radix_lds_b = 0;
+	int radix_block = 1024;
+	int radix_grid = 1;
 };
 
 struct Query {
@@ -68,11 +75,9 @@ struct Query {
 	hipEvent_t probe_stop = nullptr;
 	std::string jit_src;
 	// radix-partitioned probe (Plan::radix): the join kernel after the probe
-	// kernel, its source, and both launches' LDS bytes and the join grid
+	// kernel and its source (one workgroup per partition)
 	hipFunction_t jit_join_fn = nullptr;
 	std::string jit_join_src;
-	size_t radix_lds_a = 0, radix_lds_b = 0;
-	int radix_parts = 0;
 	int generic_grid = 0;              // probe grid of the generic kernel
 	bool fanout_rerun = false;         // switched to the generic kernel for a JoinN fan-out
 	~Query();

@@ -1091,85 +1091,68 @@ __device__ __forceinline__ void lds_dense_flush(const Plan& P, const Lds& L)
 // radix-partitioned probe (Plan::radix)
 // --------------------------------------------------------------------------
 
-// Partition of the last join's table that key offset `off` belongs to.
-__device__ __forceinline__ u32 radix_part(const JoinT& J, u32 off)
+// Probe partition of key offset `off` in join J: the aligned block of
+// 2^rpshift slots its hash falls in.
+__device__ __forceinline__ u32 radix_part(const Plan& P, const JoinT& J, u32 off)
 {
-	return (u32)((voila_hash((u64)off) & J.mask) >> (31 - __builtin_clz((u32)(J.smask + 1))));
+	return (u32)((voila_hash((u64)off) & J.mask) >> P.rpshift);
 }
 
-// Append one tuple (W words, word 0 = off | partition << 32) to this
-// workgroup's region of rtup0; rc = per-partition counts, rmeta[0] = the
-// region's tuple count (LDS).  One LDS atomic per wave for the position.
-template <int W>
-__device__ __forceinline__ VH_GLOBAL u64* radix_slot(const Plan& P, u32* rc, u32* rmeta, u32 q)
-{
-	const u32 pos = wave_append(rmeta);
-	if ((i64)pos >= P.rcap) {
-		gatomic_or(P.error, 64u);   // region bound exceeded (never expected)
-		return nullptr;
-	}
-	atomicAdd(&rc[q], 1u);
-	return (VH_GLOBAL u64*)P.rtup0 + ((u64)blockIdx.x * (u64)P.rcap + pos) * W;
-}
-
-// End of a radix probe launch: record this region's run of each partition
-// ([partition][region]) and order its tuples by partition (rtup0 -> rtup1,
-// same region).  rc / rcur: NP words of LDS each, rmeta: 2 words.
-template <int W>
-__device__ __forceinline__ void radix_finish(const Plan& P, u32* rc, u32* rcur, u32* rmeta, int NP)
+// First half of the flush of chunk c: the tuples staged in LDS have been
+// counted per partition in rc (rparts words); the partitions' running ends
+// go to rend's row c and their starts to rcur (rparts words), the cursors
+// the tuples are then placed with.  rtot: one word of LDS.
+__device__ __forceinline__ void radix_flush_begin(const Plan& P, u32 c, const u32* rc, u32* rcur, u32* rtot)
 {
 	__syncthreads();
-	const u32 n = (i64)rmeta[0] < P.rcap ? rmeta[0] : (u32)P.rcap;
-	const u64 R = (u64)P.rregions;
-	const u32 b = blockIdx.x;
-	for (int q = threadIdx.x; q < NP; q += blockDim.x) {
-		P.rcnt[(u64)q * R + b] = rc[q];
-		rcur[q] = rc[q];
-	}
-	__syncthreads();
-	block_exclusive_scan(rcur, NP, &rmeta[1]);
+	const int NP = P.rparts;
 	for (int q = threadIdx.x; q < NP; q += blockDim.x)
-		P.roff[(u64)q * R + b] = rcur[q];
+		rcur[q] = rc[q];
 	__syncthreads();
-	const VH_GLOBAL u64* src = (const VH_GLOBAL u64*)P.rtup0 + (u64)b * (u64)P.rcap * W;
-	VH_GLOBAL u64* dst = (VH_GLOBAL u64*)P.rtup1 + (u64)b * (u64)P.rcap * W;
-	// every word of a thread's V tuples is loaded before any is stored
-	constexpr int V = 4;
-	for (u32 i0 = 0; i0 < n; i0 += V * blockDim.x) {
-		u64 x[V][W];
-#pragma unroll
-		for (int v = 0; v < V; v++) {
-			const u32 i = i0 + v * blockDim.x + threadIdx.x;
-#pragma unroll
-			for (int k = 0; k < W; k++)
-				x[v][k] = i < n ? src[(u64)i * W + k] : ~0ull;
-		}
-#pragma unroll
-		for (int v = 0; v < V; v++) {
-			if (x[v][0] == ~0ull)
-				continue;
-			const u32 pos = atomicAdd(&rcur[(u32)(x[v][0] >> 32)], 1u);
-#pragma unroll
-			for (int k = 0; k < W; k++)
-				dst[(u64)pos * W + k] = x[v][k];
-		}
-	}
+	block_exclusive_scan(rcur, NP, rtot);
+	VH_GLOBAL u32* end = (VH_GLOBAL u32*)P.rend + (u64)c * (u64)NP;
+	for (int q = threadIdx.x; q < NP; q += blockDim.x)
+		end[q] = rcur[q] + rc[q];
+	__syncthreads();
 }
 
-// Join launch, partition p: pre / base (R words of LDS each) locate its runs
-// (tuple i of the partition is rtup1[base[r] + i] for the last region r with
-// pre[r] <= i); returns the partition's tuple count.
+// Second half: the counts and the staged tuple count start again at zero.
+__device__ __forceinline__ void radix_flush_end(const Plan& P, u32* rc, u32* rn)
+{
+	__syncthreads();
+	for (int q = threadIdx.x; q < P.rparts; q += blockDim.x)
+		rc[q] = 0;
+	if (threadIdx.x == 0)
+		*rn = 0;
+	__syncthreads();
+}
+
+// Empty runs for the chunk ids [from, rchunks) no iteration of the launch
+// used (one workgroup writes them).
+__device__ __forceinline__ void radix_pad(const Plan& P, u32 from)
+{
+	VH_GLOBAL u32* end = (VH_GLOBAL u32*)P.rend;
+	for (u64 i = (u64)from * (u64)P.rparts + threadIdx.x; i < (u64)P.rchunks * (u64)P.rparts; i += blockDim.x)
+		end[i] = 0;
+}
+
+// Join launch, partition p: pre / base (rchunks words of LDS each) locate
+// its run of every chunk (tuple i of the partition is at base[c] + i for the
+// last chunk c with pre[c] <= i); returns the partition's tuple count.
 __device__ __forceinline__ u32 radix_runs(const Plan& P, u32 p, u32* pre, u32* base, u32* total)
 {
-	const int R = P.rregions;
-	for (int r = threadIdx.x; r < R; r += blockDim.x) {
-		pre[r] = P.rcnt[(u64)p * R + r];
-		base[r] = P.roff[(u64)p * R + r];
+	const int RC = P.rchunks;
+	const u64 NP = (u64)P.rparts;
+	for (int c = threadIdx.x; c < RC; c += blockDim.x) {
+		const VH_GLOBAL u32* e = (const VH_GLOBAL u32*)P.rend + (u64)c * NP;
+		const u32 b = p ? e[p - 1] : 0u;
+		pre[c] = e[p] - b;
+		base[c] = b;
 	}
 	__syncthreads();
-	const u32 t = block_exclusive_scan(pre, R, total);
-	for (int r = threadIdx.x; r < R; r += blockDim.x)
-		base[r] = (u32)((u64)r * (u64)P.rcap) + base[r] - pre[r];
+	const u32 t = block_exclusive_scan(pre, RC, total);
+	for (int c = threadIdx.x; c < RC; c += blockDim.x)
+		base[c] = (u32)((u64)c * (u64)P.rchunk) + base[c] - pre[c];
 	__syncthreads();
 	return t;
 }

@@ -151,6 +151,84 @@ struct Gen {
 		return v;
 	}
 
+	// C type and width of a radix tuple value: a probe column's own type,
+	// or an earlier join's packed payload word
+	static std::string radix_vtype(const Role& r)
+	{
+		return r.kind == ROLE_PAY ? "u32" : ctype_of(r.type);
+	}
+	static int radix_vwidth(const Role& r)
+	{
+		return r.kind == ROLE_PAY ? 4 : host_width(r.type);
+	}
+
+	// The probe kernel's LDS staging of a chunk's tuples, from P.rlds_a:
+	// counts and cursors per partition, two words (staged count, scan
+	// total), then the key offsets and each value as arrays of rchunk
+	// entries, every area rounded up to 16 bytes
+	void radix_stage_decls(std::string& k) const
+	{
+		char b[512];
+		k += "\tconst int NP = P.rparts;\n\tconst u32 CRr = (u32)P.rchunk;\n";
+		k += "\tu32* const rc = (u32*)((char*)lds + P.rlds_a);\n";
+		k += "\tu32* const rcur = rc + NP;\n\tu32* const rmeta = rcur + NP;\n";
+		k += "\tu64 ro_ = ((u64)P.rlds_a + (2 * (u64)NP + 4) * 4 + 15) & ~15ull;\n";
+		k += "\tu32* const skey = (u32*)((char*)lds + ro_);\n\tro_ += ((u64)CRr * 4 + 15) & ~15ull;\n";
+		const auto vals = radix_values();
+		for (size_t i = 0; i < vals.size(); i++) {
+			const std::string t = radix_vtype(vals[i].first);
+			snprintf(b, sizeof b, "\t%s* const sv%zu = (%s*)((char*)lds + ro_);\n\tro_ += ((u64)CRr * %d + 15) & ~15ull;\n",
+				t.c_str(), i, t.c_str(), radix_vwidth(vals[i].first));
+			k += b;
+		}
+		k += "\tuint16_t* const sperm = (uint16_t*)((char*)lds + ro_);\n\tro_ += ((u64)CRr * 2 + 15) & ~15ull;\n";
+		k += "\t(void)ro_;\n";
+		k += "\tfor (int i = threadIdx.x; i < NP; i += blockDim.x) rc[i] = 0;\n";
+		k += "\tif (threadIdx.x == 0) rmeta[0] = 0;\n";
+	}
+
+	// Bytes of that staging for chunks of `chunk` rows
+	size_t radix_stage_bytes(u64 chunk) const
+	{
+		auto r16 = [](u64 x) { return (x + 15) & ~15ull; };
+		u64 n = r16((2 * (u64)P.rparts + 4) * 4) + r16(chunk * 4);
+		for (const auto& v : radix_values())
+			n += r16(chunk * (u64)radix_vwidth(v.first));
+		n += r16(chunk * 2);   // the sort's permutation
+		return (size_t)n;
+	}
+
+	// The flush of chunk `c` (an expression): the staged tuples go to their
+	// partitions' runs of the chunk, sorted by partition
+	void radix_flush(const std::string& c)
+	{
+		char b[512];
+		const int last = P.njoins - 1;
+		snprintf(b, sizeof b, "P.joins[%d].mask", last);
+		const std::string mask = uniform("u64", b);
+		const std::string rps = uniform("int", "P.rpshift");
+		line("\tradix_flush_begin(P, (u32)(" + c + "), rc, rcur, &rmeta[1]);");
+		// the staged tuples' sorted positions (an LDS permutation), then
+		// coalesced stores of the sorted chunk
+		line("\t{");
+		line("\t\tconst u32 n_ = rmeta[0];");
+		line("\t\tfor (u32 i = threadIdx.x; i < n_; i += blockDim.x)");
+		line("\t\t\tsperm[atomicAdd(&rcur[(u32)((voila_hash((u64)skey[i]) & " + mask + ") >> " + rps + ")], 1u)] = (uint16_t)i;");
+		line("\t\t__syncthreads();");
+		line("\t\tconst u64 cb_ = (u64)(" + c + ") * (u64)CRr;");
+		line("\t\tfor (u32 t = threadIdx.x; t < n_; t += blockDim.x) {");
+		line("\t\t\tconst u32 i = sperm[t];");
+		line("\t\t\t((VH_GLOBAL u32*)P.rkey)[cb_ + t] = skey[i];");
+		const auto vals = radix_values();
+		for (size_t i = 0; i < vals.size(); i++) {
+			snprintf(b, sizeof b, "\t\t\t((VH_GLOBAL %s*)P.rval[%zu])[cb_ + t] = sv%zu[i];", radix_vtype(vals[i].first).c_str(), i, i);
+			line(b);
+		}
+		line("\t\t}");
+		line("\t}");
+		line("\tradix_flush_end(P, rc, &rmeta[0]);");
+	}
+
 	int col_index(const Role& r, const std::string& path, bool pred, int stage = 1 << 20)
 	{
 		for (size_t i = 0; i < cols.size(); i++)
@@ -485,25 +563,25 @@ struct Gen {
 			if (!scalar && j == last && late_stage == P.njoins + 1)
 				emit_late();   // aggregation columns, before this join's lookups (and the next chunk's loads)
 			if (radix && j == last) {
-				// radix-partitioned probe: the row's tuple goes to its
-				// partition; the join kernel does the lookup and the group-by
-				emit_next();
-				const std::string qs = uniform("int", "(31 - __builtin_clz((u32)(" + jp + ".smask + 1)))");
+				// radix-partitioned probe: the row's tuple goes to the
+				// workgroup's LDS staging, counted by partition; the chunk's
+				// flush stores it in its partition's run, and the join kernel
+				// does the lookup and the group-by
+				const std::string rps = uniform("int", "P.rpshift");
 				const auto vals = radix_values();
-				snprintf(b, sizeof b, "%d", (int)vals.size() + 1);
-				const std::string W = b;
 				line("#pragma unroll");
 				line("\t\t\tfor (int u = 0; u < U; u++) {");
 				line("\t\t\t\tif (!VH_LIVE(u)) continue;");
-				line("\t\t\t\tconst u32 q = (u32)((voila_hash((u64)off[u]) & " + mask + ") >> " + qs + ");");
-				line("\t\t\t\tVH_GLOBAL u64* t = radix_slot<" + W + ">(P, rc, rmeta, q);");
-				line("\t\t\t\tif (!t) continue;");
-				line("\t\t\t\tt[0] = (u64)off[u] | ((u64)q << 32);");
+				line("\t\t\t\tconst u32 pos = wave_append(&rmeta[0]);");
+				line("\t\t\t\tif (pos >= CRr) { gatomic_or(" + err + ", 64u); continue; }");
+				line("\t\t\t\tatomicAdd(&rc[(u32)((voila_hash((u64)off[u]) & " + mask + ") >> " + rps + ")], 1u);");
+				line("\t\t\t\tskey[pos] = off[u];");
 				for (size_t k = 0; k < vals.size(); k++) {
 					if (vals[k].first.kind == ROLE_PAY)
-						snprintf(b, sizeof b, "\t\t\t\tt[%zu] = (u64)pay%d[u];", k + 1, vals[k].first.slot);
+						snprintf(b, sizeof b, "\t\t\t\tsv%zu[pos] = pay%d[u];", k, vals[k].first.slot);
 					else
-						snprintf(b, sizeof b, "\t\t\t\tt[%zu] = (u64)%s;", k + 1, role(vals[k].first, vals[k].second).c_str());
+						snprintf(b, sizeof b, "\t\t\t\tsv%zu[pos] = (%s)%s;", k, radix_vtype(vals[k].first).c_str(),
+							role(vals[k].first, vals[k].second).c_str());
 					line(b);
 				}
 				line("\t\t\t}");
@@ -687,6 +765,79 @@ struct Gen {
 		line("\t}");
 	}
 
+	// One wave chunk of the vector loop (64U rows from wave chunk `w`):
+	// its column pairs, the chunk's first row, the rows' values, the body.
+	void vec_chunk(int mode, bool declare_pairs)
+	{
+		char b[512];
+		if (declare_pairs)
+			for (size_t i = 0; i < cols.size(); i++) {
+				if (cols[i].late)
+					continue;
+				snprintf(b, sizeof b, "\t\tvh2_%s q%zu[U / 2];", ctype_of(cols[i].type), i);
+				line(b);
+			}
+		if (!pipe)
+			load_pairs("w");
+		line("\t\tconst u32 cb = (u32)(rb + w * (64 * U));   // first row of the chunk");
+		line("\t\t(void)cb;");
+		for (size_t i = 0; i < cols.size(); i++) {
+			snprintf(b, sizeof b, "\t\t%s c%zu[U];", ctype_of(cols[i].type), i);
+			line(b);
+			if (cols[i].late)
+				continue;
+			line("#pragma unroll");
+			snprintf(b, sizeof b, "\t\tfor (int p = 0; p < U / 2; p++) { c%zu[2 * p] = q%zu[p].x; c%zu[2 * p + 1] = "
+				"q%zu[p].y; }", i, i, i, i);
+			line(b);
+		}
+		snprintf(b, sizeof b, "\t\tu32 live = 0x%xu;", (unsigned)((1ull << U) - 1));
+		line(b);
+		body(mode, false);
+	}
+
+	// Row loops of the radix probe kernel.  Every iteration of a workgroup
+	// takes one chunk of U * blockDim rows and ends with the chunk's flush,
+	// so every wave of the workgroup runs the same iterations (the flush
+	// has barriers).  Vector path: chunk c is wave chunks [c * waves,
+	// (c + 1) * waves); the rows after the last whole wave chunk go through
+	// the scalar loop of workgroup 0 as chunk nmain.  Scalar path: chunk c is
+	// rows [c * CR, (c + 1) * CR).  Workgroup 0 then writes empty runs for
+	// the chunk ids after the last one used.
+	void radix_loops(int mode)
+	{
+		line("\tconst i64 CR = (i64)CRr;   // U * blockDim.x rows per chunk");
+		if (vec) {
+			line("\tconst int lane = threadIdx.x & 63;");
+			line("\tconst i64 rb = 0;");
+			line("\tconst i64 nfull = rows / (64 * U);");
+			line("\tconst i64 wpb = (i64)(blockDim.x >> 6);");
+			line("\tconst i64 wsub = (i64)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));");
+			line("\tconst i64 nmain = (nfull + wpb - 1) / wpb;");
+			line("\tfor (i64 ci = blockIdx.x; ci < nmain; ci += gridDim.x) {");
+			line("\t\tconst i64 w = ci * wpb + wsub;");
+			line("\t\tif (w < nfull) {");
+			vec_chunk(mode, true);
+			line("\t\t}");
+			radix_flush("ci");
+			line("\t}");
+			line("\tif (blockIdx.x == 0) {");
+			scalar_loop(mode, "nfull * (64 * U)", "rows", "tile");
+			radix_flush("nmain");
+			line("\t\tradix_pad(P, (u32)nmain + 1u);");
+			line("\t}");
+		} else {
+			line("\tconst i64 nch = (rows + CR - 1) / CR;");
+			line("\tfor (i64 ci = blockIdx.x; ci < nch; ci += gridDim.x) {");
+			line("\t\tconst i64 ce = ci * CR + CR < rows ? ci * CR + CR : rows;");
+			scalar_loop(mode, "ci * CR", "ce", "tile");
+			radix_flush("ci");
+			line("\t}");
+			line("\tif (blockIdx.x == 0)");
+			line("\t\tradix_pad(P, (u32)nch);");
+		}
+	}
+
 	std::string source(int mode)
 	{
 		char b[1024];
@@ -724,6 +875,8 @@ struct Gen {
 		// its values, an even U and a grid-strided row mapping (not GB_PART's
 		// slab per workgroup); otherwise every row takes the scalar path.
 		vec = U % 2 == 0 && mode != GB_PART;
+		if (radix)
+			pipe = false;   // the last join's lookups are in the join kernel
 		int row_bytes = 0;
 		for (const Col& c : cols) {
 			vec = vec && ((uintptr_t)c.ptr % (uintptr_t)(2 * host_width(c.type))) == 0;
@@ -767,7 +920,9 @@ struct Gen {
 					(double)P.rows * host_width(c.type) < 2147483000.0;   // buffer offsets are 32-bit
 		}
 
-		if (mode == GB_PART) {
+		if (radix) {
+			radix_loops(mode);
+		} else if (mode == GB_PART) {
 			// rows of this workgroup: one contiguous slab (a batch item), the
 			// region its tuples are kept for
 			if (batch) {
@@ -816,23 +971,7 @@ struct Gen {
 			if (pipe)
 				load_pairs("wv");
 			line("\tfor (i64 w = wv; w < nfull; w += nwv) {");
-			if (!pipe)
-				load_pairs("w");
-			line("\t\tconst u32 cb = (u32)(rb + w * (64 * U));   // first row of the chunk");
-			line("\t\t(void)cb;");
-			for (size_t i = 0; i < cols.size(); i++) {
-				snprintf(b, sizeof b, "\t\t%s c%zu[U];", ctype_of(cols[i].type), i);
-				line(b);
-				if (cols[i].late)
-					continue;
-				line("#pragma unroll");
-				snprintf(b, sizeof b, "\t\tfor (int p = 0; p < U / 2; p++) { c%zu[2 * p] = q%zu[p].x; c%zu[2 * p + 1] = "
-					"q%zu[p].y; }", i, i, i, i);
-				line(b);
-			}
-			snprintf(b, sizeof b, "\t\tu32 live = 0x%xu;", (unsigned)((1ull << U) - 1));
-			line(b);
-			body(mode, false);
+			vec_chunk(mode, false);
 			line("\t}");
 			line("\t}");
 			if (batch)
@@ -870,14 +1009,7 @@ struct Gen {
 			k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
 		}
 		if (radix) {
-			// partition counts (NP), cursors (NP), the region's tuple count
-			// and a scan total, after any join tables in LDS
-			const std::string jl = "P.joins[" + std::to_string(P.njoins - 1) + "]";
-			k += "\tconst int NP = (int)((" + jl + ".mask + 1) / (" + jl + ".smask + 1));\n";
-			k += "\tu32* const rc = (u32*)((char*)lds + P.rlds_a);\n";
-			k += "\tu32* const rcur = rc + NP;\n\tu32* const rmeta = rcur + NP;\n";
-			k += "\tfor (int i = threadIdx.x; i < NP; i += blockDim.x) rc[i] = 0;\n";
-			k += "\tif (threadIdx.x == 0) rmeta[0] = 0;\n";
+			radix_stage_decls(k);
 		} else {
 			k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 			k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
@@ -935,7 +1067,7 @@ struct Gen {
 				k += "\tpart_finish(P, L);\n";
 			k += "\t__syncthreads();\n\t}\n}\n";
 		} else if (radix) {
-			k += "\tradix_finish<" + std::to_string(radix_values().size() + 1) + ">(P, rc, rcur, rmeta, NP);\n}\n";
+			k += "}\n";
 		} else {
 			k += "\tblock_epilogue<" + std::to_string(mode) + ">(P, L);\n}\n";
 		}
@@ -943,9 +1075,10 @@ struct Gen {
 	}
 
 	// The join kernel of a radix-partitioned probe: workgroup p copies
-	// partition p's sub-table of the last join into LDS, then takes the
-	// partition's tuples (U per lane, from every region's run), looks each
-	// key up in LDS (rows of a full sub-table: in the overflow table) and
+	// partition p's 2^rpshift slots of the last join's table (whole
+	// sub-tables of its partitioned build) into LDS, then takes the
+	// partition's tuples (U per lane, from every chunk's run), looks each key
+	// up in LDS (rows of a full sub-table: in the overflow table) and
 	// aggregates it as the one-pass kernel would; at its end the LDS
 	// group-by state is added to the global arrays.
 	std::string join_source(int mode)
@@ -958,9 +1091,9 @@ struct Gen {
 		const std::string err = uniform("u32*", "P.error");
 		const std::string mask = uniform("u64", jp + ".mask");
 		const std::string sm = uniform("u64", jp + ".smask");
+		const std::string pm = uniform("u64", "((1ull << P.rpshift) - 1)");
 		const std::string ovf = uniform(gptr("u64"), "(" + gptr("u64") + ")" + jp + ".ovf");
 		const auto vals = radix_values();
-		const int W = (int)vals.size() + 1;
 		if (U <= 0)
 			U = 4;
 		std::vector<int> ci(vals.size(), -1);
@@ -981,23 +1114,23 @@ struct Gen {
 		line("\t\tfor (int u = 0; u < U; u++) {");
 		line("\t\t\tconst u32 i = i0 + u * blockDim.x + threadIdx.x;");
 		line("\t\t\tlive |= (u32)(i < rtotal) << u;");
-		snprintf(b, sizeof b, "\t\t\tconst VH_GLOBAL u64* t = rt + (i < rtotal ? radix_tuple_index(rpre, rbase, R, i) : 0ull) * %du;", W);
-		line(b);
-		line("\t\t\toff[u] = (u32)t[0];");
+		line("\t\t\tconst u64 x = i < rtotal ? radix_tuple_index(rpre, rbase, R, i) : 0ull;");
+		line("\t\t\toff[u] = rk[x];");
 		for (size_t k = 0; k < vals.size(); k++) {
 			if (vals[k].first.kind == ROLE_PAY)
-				snprintf(b, sizeof b, "\t\t\tpay%d[u] = (u32)t[%zu];", vals[k].first.slot, k + 1);
+				snprintf(b, sizeof b, "\t\t\tpay%d[u] = rv%zu[x];", vals[k].first.slot, k);
 			else
-				snprintf(b, sizeof b, "\t\t\tc%d[u] = (%s)(i64)t[%zu];", ci[k], ctype_of(vals[k].first.type), k + 1);
+				snprintf(b, sizeof b, "\t\t\tc%d[u] = rv%zu[x];", ci[k], k);
 			line(b);
 		}
 		line("\t\t}");
-		// lookups in the LDS sub-table (local slot = global slot & smask)
+		// lookups in the LDS copy (local slot = global slot & pm; the
+		// sequence wraps within the build's sub-table of smask + 1 slots)
 		line("\t\tu32 payl[U];");
 		line("\t\tu64 h[U], s[U];");
 		line("\t\tu32 ovf_first = 0;");
 		line("#pragma unroll");
-		line("\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + sm + ";");
+		line("\t\tfor (int u = 0; u < U; u++) h[u] = voila_hash((u64)off[u]) & " + pm + ";");
 		line("#pragma unroll");
 		line("\t\tfor (int u = 0; u < U; u++) s[u] = tab[h[u]];");
 		line("#pragma unroll");
@@ -1007,7 +1140,7 @@ struct Gen {
 		line("\t\t\tu64 n = 0;");
 		line("\t\t\twhile (s[u] != kEmpty && (u32)s[u] != off[u] && n <= " + sm + ") {");
 		line("\t\t\t\tn++;");
-		line("\t\t\t\th[u] = (h[u] + 1) & " + sm + ";");
+		line("\t\t\t\th[u] = hnext(h[u], " + sm + ");");
 		line("\t\t\t\ts[u] = tab[h[u]];");
 		line("\t\t\t}");
 		snprintf(b, sizeof b, "\t\t\tif ((s[u] == kEmpty || (u32)s[u] != off[u]) && ((dups_mask >> %d) & 1u)) {", kSpillBit + last);
@@ -1039,8 +1172,8 @@ struct Gen {
 			line("\t\t\tfor (int u = 0; u < U; u++) {");
 			line("\t\t\t\tif (!VH_LIVE(u)) continue;");
 			line("\t\t\t\tbool ov = (ovf_first >> u) & 1u;");
-			line("\t\t\t\tconst u64 h0 = voila_hash((u64)off[u]) & " + sm + ";");
-			line("\t\t\t\tu64 hh = ov ? (h[u] + 1) & " + mask + " : (h[u] + 1) & " + sm + ";");
+			line("\t\t\t\tconst u64 h0 = voila_hash((u64)off[u]) & " + pm + ";");
+			line("\t\t\t\tu64 hh = ov ? (h[u] + 1) & " + mask + " : hnext(h[u], " + sm + ");");
 			line("\t\t\t\tfor (u64 n = 0;; n++) {");
 			line("\t\t\t\t\tif (n >= kMaxProbe) { gatomic_or(" + err + ", 32u); break; }");
 			line("\t\t\t\t\tconst u64 sv = ov ? " + ovf + "[hh] : (hh == h0 ? kEmpty : tab[hh]);");
@@ -1054,7 +1187,7 @@ struct Gen {
 			line("\t\t\t\t\t\tbreak;");
 			line("\t\t\t\t\t}");
 			line("\t\t\t\t\tif ((u32)sv == off[u]) consume(u, (u32)(sv >> 32));");
-			line("\t\t\t\t\thh = ov ? (hh + 1) & " + mask + " : (hh + 1) & " + sm + ";");
+			line("\t\t\t\t\thh = ov ? (hh + 1) & " + mask + " : hnext(hh, " + sm + ");");
 			line("\t\t\t\t}");
 			line("\t\t\t}");
 			line("\t\t}");
@@ -1067,6 +1200,7 @@ struct Gen {
 		for (const char* t : {"int8_t", "uint8_t", "int16_t", "uint16_t", "int32_t", "uint32_t", "int64_t", "uint64_t"})
 			k += std::string("typedef __hip_internal::") + t + " " + t + ";\n";
 		k += "#include \"vh_fused.hpp\"\nusing namespace vh;\n";
+		k += "typedef u32 vh_u32x4 __attribute__((ext_vector_type(4)));\n";
 		k += "#define VH_LIVE(u) ((live >> (u)) & 1u)\n";
 		k += "#define VH_KILL(u, c) (live &= ~((u32)(c) << (u)))\n";
 		k += "static constexpr int U = " + std::to_string(U) + ";\n";
@@ -1075,15 +1209,28 @@ struct Gen {
 		k += "\tconst Plan& P = *Pp;\n\textern __shared__ u64 lds[];\n";
 		k += "\tLds L = lds_layout<" + std::to_string(mode) + ">(P, lds);\n";
 		k += "\tlds_init<" + std::to_string(mode) + ">(P, L);\n";
-		k += "\tconst u32 S = (u32)(" + jp + ".smask + 1);\n";
+		k += "\tconst u32 S = 1u << P.rpshift;\n";
 		k += "\tu64* const tab = (u64*)((char*)lds + P.rlds_b);\n";
-		k += "\tu32* const rpre = (u32*)(tab + S);\n\tu32* const rbase = rpre + P.rregions;\n";
-		k += "\tu32* const rtot = rbase + P.rregions;\n";
-		k += "\tconst int R = P.rregions;\n";
-		k += "\t{\n\t\tconst " + gptr("u64") + " src = (" + gptr("u64") + ")" + jp + ".slots + (u64)blockIdx.x * S;\n";
-		k += "\t\tfor (u32 i = threadIdx.x; i < S; i += blockDim.x) tab[i] = src[i];\n\t}\n";
+		k += "\tu32* const rpre = (u32*)(tab + S);\n\tu32* const rbase = rpre + P.rchunks;\n";
+		k += "\tu32* const rtot = rbase + P.rchunks;\n";
+		k += "\tconst int R = P.rchunks;\n";
+		// the partition's slots, eight 16-byte loads in flight per thread
+		k += "\t{\n\t\t" + gptr("vh_u32x4") + " src = (" + gptr("vh_u32x4") + ")(" + jp + ".slots + (u64)blockIdx.x * S);\n";
+		k += "\t\tvh_u32x4* dst = (vh_u32x4*)tab;\n";
+		k += "\t\tfor (u32 i0 = 0; i0 < S / 2; i0 += 8 * blockDim.x) {\n";
+		k += "\t\t\tvh_u32x4 v_[8];\n";
+		k += "#pragma unroll\n\t\t\tfor (int j = 0; j < 8; j++) { const u32 i = i0 + j * blockDim.x + threadIdx.x; "
+			"if (i < S / 2) v_[j] = src[i]; }\n";
+		k += "#pragma unroll\n\t\t\tfor (int j = 0; j < 8; j++) { const u32 i = i0 + j * blockDim.x + threadIdx.x; "
+			"if (i < S / 2) dst[i] = v_[j]; }\n";
+		k += "\t\t}\n\t}\n";
 		k += "\tconst u32 rtotal = radix_runs(P, blockIdx.x, rpre, rbase, rtot);\n";
-		k += "\tconst " + gptr("u64") + " const rt = (" + gptr("u64") + ")P.rtup1;\n";
+		k += "\t" + gptr("u32") + " const rk = (" + gptr("u32") + ")P.rkey;\n";
+		for (size_t i = 0; i < vals.size(); i++) {
+			const std::string t = radix_vtype(vals[i].first);
+			k += "\t" + gptr(t) + " const rv" + std::to_string(i) + " = (" + gptr(t) + ")P.rval[" +
+				std::to_string(i) + "];\n";
+		}
 		k += "\tu32 dups_mask = 0;\n";
 		if (J.variant == VH_JOINN)
 			k += "\tif (gload(" + jp + ".dups, 0)) dups_mask |= " + std::to_string(1u << last) + "u;\n";
@@ -1138,10 +1285,22 @@ std::string jit_join_source(const Plan& P, int unroll, int block)
 	return g.join_source(P.mode);
 }
 
-int jit_radix_words(const Plan& P)
+int jit_radix_values(const Plan& P, std::vector<int>& widths)
 {
 	Gen g(P, true, false, 0);
-	return (int)g.radix_values().size() + 1;
+	widths.clear();
+	int bytes = 4;
+	for (const auto& v : g.radix_values()) {
+		widths.push_back(Gen::radix_vwidth(v.first));
+		bytes += widths.back();
+	}
+	return bytes;
+}
+
+size_t jit_radix_stage_bytes(const Plan& P, u64 chunk)
+{
+	Gen g(P, true, false, 0);
+	return g.radix_stage_bytes(chunk);
 }
 
 int jit_compile(const std::string& src, hipFunction_t* fn, std::string& log, const char* name)

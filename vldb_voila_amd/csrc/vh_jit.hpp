@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "vh_plan.hpp"
 
@@ -51,7 +52,11 @@ hipError_t launch_generated(hipFunction_t fn, const Plan& P, const Plan* dP, int
 // writes the partitioned tuples; this is the join kernel that consumes them
 // (vh_generated_join, one workgroup per partition of the last join).
 std::string jit_join_source(const Plan& P, int unroll, int block);
-// Words (u64) of one radix probe tuple of P.
-int jit_radix_words(const Plan& P);
+// The values a radix probe tuple of P carries: their widths in bytes
+// (P.rval[k] holds value k), and the tuple's bytes with its key offset.
+int jit_radix_values(const Plan& P, std::vector<int>& widths);
+// LDS bytes the probe kernel stages one chunk of `chunk` rows' tuples in
+// (after P.rlds_a; P.rparts must be set).
+size_t jit_radix_stage_bytes(const Plan& P, u64 chunk);
 
 } // namespace vh

@@ -184,30 +184,35 @@ struct Plan {
 	int gparts;
 	u32* error;          // bit 0: hash table full, bit 3: group key outside [min, max]
 	// Radix-partitioned probe of the last join (radix != 0; that join's
-	// table is partitioned, see BuildPlan).  The probe kernel ends each row
-	// that reaches the last join by appending a tuple of rw words to its
-	// workgroup's region of rtup0 (region b = tuples [b * rcap, (b + 1) *
-	// rcap)): word 0 = key - kmin | partition << 32, then the values the
-	// group-by needs (probe columns, earlier joins' payloads), counting the
-	// tuples per partition in LDS.  At its end it orders its region by
-	// partition into rtup1 and records each partition's run
-	// (rcnt / roff[partition * rregions + region]).  The join kernel then
-	// takes one partition per workgroup: its sub-table copied into LDS, its
-	// runs of every region, the lookups and the group-by.
-	u64* rtup0;
-	u64* rtup1;
-	u32* rcnt;
-	u32* roff;
-	i64 rcap;
+	// table is a partitioned hash table, see BuildPlan).  The probe kernel
+	// takes the rows in chunks of at most rchunk rows, one per workgroup
+	// iteration, numbered from 0; chunk ids up to rchunks that no iteration
+	// used get empty runs.  Each row that reaches the last join leaves a tuple in the workgroup's
+	// LDS: its key offset (key - kmin) and the rw values the group-by needs
+	// (probe columns, earlier joins' payloads).  At the chunk's end the
+	// workgroup stores the chunk's tuples sorted by probe partition (the
+	// aligned block of 2^rpshift slots of the key's hash, rparts of them)
+	// into rkey / rval[k] at [c * rchunk, ...) and the partitions' running
+	// ends into rend[c * rparts + p] (so partition p's run of chunk c is
+	// [rend[.. p - 1], rend[.. p]) of the chunk).  The join kernel takes one
+	// partition per workgroup: its 2^rpshift slots copied into LDS, its run
+	// of every chunk, the lookups and the group-by.
+	u32* rkey;
+	void* rval[16];      // kRadixMaxVals
+	u32* rend;
+	i64 rchunk;
 	int radix;
-	int rw;
-	int rregions;
-	u32 rlds_a;          // probe kernel: LDS byte offset of its partition counts
+	int rw;              // values per tuple
+	int rchunks;         // chunk ids the run table holds
+	int rparts;          // probe partitions
+	int rpshift;         // log2 slots of a probe partition (>= the build's sub-table)
+	u32 rlds_a;          // probe kernel: LDS byte offset of its tuple staging
 	u32 rlds_b;          // join kernel: LDS byte offset of the sub-table
 	int rpad;
 };
 
-static constexpr int kRadixMaxWords = 16;   // words of a radix probe tuple
+static constexpr int kRadixMaxVals = 16;        // values of a radix probe tuple (Plan::rval)
+static constexpr int kRadixMaxChunks = 4096;    // chunk ids of a radix probe (one scan in LDS)
 
 // A work item of a batched launch: rows [r0, r1) of one pipeline's build or
 // probe side.
