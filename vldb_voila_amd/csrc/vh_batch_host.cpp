@@ -40,6 +40,7 @@ struct BatchLaunch {
 static constexpr int kBatchGenerated = -3;
 static constexpr int kBatchScatter = -4;   // partitioned builds: chunk items
 static constexpr int kBatchSlice = -5;     // partitioned builds: partition items
+static constexpr int kBatchPostFill = -6;  // the fills that wait for the builds' flags
 // probe items' grouping key (BatchItem::build) of generated kernel g
 static constexpr int kGenKey = 1000;
 
@@ -50,6 +51,8 @@ struct vh_batch {
 	BuildPlan* d_builds = nullptr;
 	BatchItem* d_items = nullptr;
 	FillSeg* d_segs = nullptr;
+	FillSeg* d_psegs = nullptr;   // fills after the builds (conditional on their repeated-key flags)
+	int npsegs = 0;
 	u32* d_ctl = nullptr;
 	u32 nitems = 0;
 	std::vector<BatchLaunch> launches;
@@ -133,7 +136,9 @@ static hipError_t enqueue(vh_batch* b, hipStream_t s)
 	hipError_t e = launch_multi_fill(b->d_segs, b->nsegs, s);
 	for (size_t k = 0; k < b->launches.size() && e == hipSuccess; k++) {
 		const BatchLaunch& l = b->launches[k];
-		if (l.kind == kBatchScatter || l.kind == kBatchSlice) {
+		if (l.kind == kBatchPostFill) {
+			e = launch_multi_fill(b->d_psegs, b->npsegs, s);
+		} else if (l.kind == kBatchScatter || l.kind == kBatchSlice) {
 			e = launch_batch_part(l.kind == kBatchSlice, b->d_builds, b->d_items, l.first, l.count, b->d_ctl + k,
 				std::max(1, b->grid / 4), l.lds, s);
 		} else if (l.kind == kBatchGenerated) {
@@ -167,7 +172,7 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	std::vector<BuildPlan> builds;
 	std::vector<BatchItem> build_items, probe_items, agg_items, scatter_items, slice_items;
 	size_t slice_lds = 0;
-	std::vector<FillSeg> segs;
+	std::vector<FillSeg> segs, psegs;
 	std::vector<std::string> gen_src;        // generated batch kernels, by index
 	std::vector<hipFunction_t> gen_fn;
 	std::vector<size_t> gen_lds;
@@ -208,11 +213,22 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 				for (i64 r0 = 0; r0 < B.rows; r0 += T)
 					build_items.push_back({i, 0, first_build + j, 0, r0, std::min<i64>(r0 + T, B.rows)});
 			}
-			// a partitioned hash table's slices store every slot: its
-			// overflow table is what needs refilling
-			segs.push_back({B.hashed ? q->ovf[j] : q->slots[j], q->L.capacity[j], ~0ull});
-			if (q->dir[j] && !B.nparts)      // a partitioned build rewrites all of dir[]
-				segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull});
+			// as a single query clears them: a partitioned hash table's
+			// slices store every slot, so its overflow table is refilled
+			// after a run that spilled into it; a partitioned direct build
+			// rewrites all of dir[], and its overflow slots are refilled
+			// after a run with repeated keys (the scatter launch resets both
+			// flags); a CAS build's slots, dir[] and flag every run
+			if (B.hashed) {
+				segs.push_back({q->ovf[j], q->L.capacity[j], ~0ull, spill_flag(q, j)});
+			} else if (B.nparts) {
+				segs.push_back({q->slots[j], q->L.capacity[j], ~0ull, dups_flag(q, j)});
+			} else {
+				segs.push_back({(u64*)dups_flag(q, j), 1, 0, nullptr});
+				segs.push_back({q->slots[j], q->L.capacity[j], ~0ull, nullptr});
+				if (q->dir[j])
+					segs.push_back({q->dir[j], q->L.dir_words[j], ~0ull, nullptr});
+			}
 		}
 		// the probe kernel: generated for this pipeline's plan (shared with
 		// every pipeline whose source is the same), or generic per strategy
@@ -251,12 +267,28 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		}
 		const u64 groups = q->L.groups;
 		const u64 ns = (u64)std::max(P.nsums, 1);
-		segs.push_back({(u64*)q->flags, (u64)kFlagWords / 2, 0, nullptr});
-		segs.push_back({P.out_lo, groups * ns, 0});
-		segs.push_back({(u64*)P.out_hi, groups * ns, 0});
-		segs.push_back({P.out_cnt, groups, 0});
-		if (P.mode == GB_HASH)
-			segs.push_back({P.hkeys, groups, ~0ull});
+		segs.push_back({(u64*)q->flags, 1, 0, nullptr});   // error bits
+		if (P.mode == GB_PART) {
+			// the partition aggregation stores every group's count (and the
+			// sums of the groups present); the arrays need zeros only when a
+			// JoinN fan-out can add rows into them during the probe, which
+			// takes repeated keys in that JoinN's build of this run (the
+			// segments of two such joins may both clear them)
+			for (int j = 0; j < P.njoins; j++) {
+				if (P.joins[j].variant != VH_JOINN)
+					continue;
+				const u32* f = dups_flag(q, j);
+				psegs.push_back({P.out_lo, groups * ns, 0, f});
+				psegs.push_back({(u64*)P.out_hi, groups * ns, 0, f});
+				psegs.push_back({P.out_cnt, groups, 0, f});
+			}
+		} else {
+			segs.push_back({P.out_lo, groups * ns, 0});
+			segs.push_back({(u64*)P.out_hi, groups * ns, 0});
+			segs.push_back({P.out_cnt, groups, 0});
+			if (P.mode == GB_HASH)
+				segs.push_back({P.hkeys, groups, ~0ull});
+		}
 		plans.push_back(P);
 	}
 	if (rc) {
@@ -273,6 +305,8 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	items.insert(items.end(), scatter_items.begin(), scatter_items.end());
 	b->launches.push_back({kBatchSlice, (u32)items.size(), (u32)slice_items.size(), slice_lds, nullptr});
 	items.insert(items.end(), slice_items.begin(), slice_items.end());
+	if (!psegs.empty())
+		b->launches.push_back({kBatchPostFill, 0, 0, 0, nullptr});
 	const size_t nbuild_items = items.size();
 	for (size_t k = 0; k < probe_items.size();) {
 		size_t e = k;
@@ -303,7 +337,9 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 	}
 	segs.push_back({(u64*)b->d_ctl, ctl_words, 0});
 	b->nsegs = (int)segs.size();
-	if ((rc = b->upload((void**)&b->d_segs, segs.data(), segs.size() * sizeof(FillSeg)))) {
+	b->npsegs = (int)psegs.size();
+	if ((rc = b->upload((void**)&b->d_segs, segs.data(), segs.size() * sizeof(FillSeg))) ||
+		(!psegs.empty() && (rc = b->upload((void**)&b->d_psegs, psegs.data(), psegs.size() * sizeof(FillSeg))))) {
 		delete b;
 		return rc;
 	}

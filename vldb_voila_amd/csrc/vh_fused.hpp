@@ -628,6 +628,33 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 		}
 	}
 	__syncthreads();
+	// Without repeated keys in a JoinN's build no row fanned out into the
+	// global arrays during the probe: every group's count is stored (zero
+	// for the groups absent) and the sums of the groups present, with
+	// nothing read first, so the arrays need no clearing between runs.
+	bool fanout = false;
+	for (int j = 0; j < P.njoins; j++)
+		fanout = fanout || (P.joins[j].variant == VH_JOINN && *(const VH_GLOBAL u32*)P.joins[j].dups);
+	if (!fanout) {
+		for (u64 li = threadIdx.x; li < ng; li += blockDim.x) {
+			const u32 c = cnt[li];
+			const u64 g = g0 + li;
+			P.out_cnt[g] = c;
+			if (!c)
+				continue;
+#pragma unroll
+			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
+				if (s2 < ns) {
+					const SumArg& A = P.sums[s2];
+					const u64 l = lo[(u64)s2 * S + li];
+					const u64 idx = (u64)s2 * P.domain + g;
+					P.out_lo[idx] = l;
+					P.out_hi[idx] = (P.wide && A.wide) ? (i64)hi[(u64)s2 * S + li] : (A.arg_signed ? (i64)l >> 63 : 0);
+				}
+			}
+		}
+		return;
+	}
 	for (u64 li = threadIdx.x; li < ng; li += blockDim.x) {
 		const u32 c = cnt[li];
 		if (!c)
