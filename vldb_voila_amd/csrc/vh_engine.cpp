@@ -612,8 +612,29 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		s.arg_signed = type_signed(at);
 		s.res128 = L.ty.aggrs[a].type == VH_I128;
 		s.wide = s.res128 && (at == VH_U64 || at == VH_I64);
+		const TypeProps& rt = L.ty.nodes[root];
+		s.narrow = s.arg_signed ? (rt.emin >= -2147483648.0 && rt.emax <= 2147483647.0)
+			: (rt.emin >= 0.0 && rt.emax <= 4294967295.0);
 		L.sum_of_aggr[a] = P.nsums;
 		P.nsums++;
+	}
+
+	// partitioned group-by tuple record: the group index (u32) at byte 0,
+	// the 4-byte arguments after it, then the 8-byte ones, 8-aligned
+	{
+		int off = 4;
+		for (int s2 = 0; s2 < P.nsums; s2++)
+			if (P.sums[s2].narrow) {
+				P.sums[s2].toff = off;
+				off += 4;
+			}
+		off = (off + 7) & ~7;
+		for (int s2 = 0; s2 < P.nsums; s2++)
+			if (!P.sums[s2].narrow) {
+				P.sums[s2].toff = off;
+				off += 8;
+			}
+		P.grec = (off + 7) & ~7;
 	}
 
 	// group-by mode
@@ -1023,10 +1044,9 @@ int query_create(Query* q, bool single)
 	}
 	if (P.mode == GB_PART) {
 		const u64 rows = (u64)std::max<int64_t>(P.rows, 1);
-		const u64 W = (u64)P.nsums + 1;
 		const u64 regions = (rows + (u64)P.gregion_rows - 1) / (u64)P.gregion_rows;
-		if ((rc = q->alloc((void**)&P.gtup0, rows * W * sizeof(u64))) ||
-			(rc = q->alloc((void**)&P.gtup1, rows * W * sizeof(u64))) ||
+		if ((rc = q->alloc((void**)&P.gtup0, rows * (u64)P.grec)) ||
+			(rc = q->alloc((void**)&P.gtup1, rows * (u64)P.grec)) ||
 			(rc = q->alloc((void**)&P.gcnt, regions * (u64)P.gparts * sizeof(u32))) ||
 			(rc = q->alloc((void**)&P.goff, regions * (u64)P.gparts * sizeof(u32))))
 			return rc;

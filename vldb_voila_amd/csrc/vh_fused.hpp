@@ -447,29 +447,58 @@ __device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 
 	}
 }
 
+// GB_PART tuple records (Plan::grec bytes): sum s's argument at
+// SumArg::toff, in 4 bytes (SumArg::narrow, sign-extended when the argument
+// is signed and read back) or 8.
+__device__ __forceinline__ void part_put(const Plan& P, VH_GLOBAL char* rec, int s, u64 v)
+{
+	if (P.sums[s].narrow)
+		*(VH_GLOBAL u32*)(rec + P.sums[s].toff) = (u32)v;
+	else
+		*(VH_GLOBAL u64*)(rec + P.sums[s].toff) = v;
+}
+
+// Sum s's argument out of a tuple record held as words in registers (the
+// word is picked by comparisons, not by indexing the array).
+template <int N>
+__device__ __forceinline__ u64 part_field(const Plan& P, const u64 (&wd)[N], int s)
+{
+	const int o = P.sums[s].toff;
+	u64 word = 0;
+#pragma unroll
+	for (int w = 0; w < N; w++)
+		if (w == (o >> 3))
+			word = wd[w];
+	if (P.sums[s].narrow) {
+		const u32 x = (u32)(word >> ((o & 7) * 8));
+		return P.sums[s].arg_signed ? (u64)(i64)(int)x : (u64)x;
+	}
+	return word;
+}
+
 // GB_PART, one tuple per live row: a slot in the region's tuple space (one
-// atomic per wave), the group index, then each sum's argument; the
+// atomic per wave) and the group index there, then each sum's argument; the
 // partition's count in LDS.  Rows beyond the space (JoinN fan-out) add into
-// the global arrays.
-__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, VH_GLOBAL u64*& t)
+// the global arrays (t = null).
+__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, VH_GLOBAL char*& t)
 {
 	t = nullptr;
 	if (!live)
 		return;
 	const u32 pos = wave_append((u32*)L.hi);
 	if (pos < L.rcap) {
-		t = (VH_GLOBAL u64*)P.gtup0 + (u64)(L.rbase + pos) * (u64)(P.nsums + 1);
-		t[0] = g;
+		t = (VH_GLOBAL char*)P.gtup0 + (u64)(L.rbase + pos) * (u64)P.grec;
+		*(VH_GLOBAL u32*)t = (u32)g;
 		atomicAdd(&L.cnt[g >> P.gshift], 1u);
 	} else {
 		gatomic_add(&P.out_cnt[g], 1ull);
 	}
 }
 
-__device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL u64* t, int s, u64 g, u64 v)
+__device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL char* t, int s, u64 g, u64 v)
 {
 	if (t) {
-		t[1 + s] = v;
+		part_put(P, t, s, v);
 	} else {
 		const SumArg& S = P.sums[s];
 		add_global(&P.out_lo[(u64)s * P.domain + g], &P.out_hi[(u64)s * P.domain + g], v, P.wide && S.res128,
@@ -480,7 +509,7 @@ __device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL u64* t
 __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, const Chunk& C, const u64 g[kU],
 	const bool ok[kU])
 {
-	VH_GLOBAL u64* t[kU];
+	VH_GLOBAL char* t[kU];
 	bool live[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
@@ -505,7 +534,8 @@ __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, cons
 }
 
 // End of a GB_PART region: order its tuples by partition (gtup0 -> gtup1,
-// the same rows' positions) and record each partition's run.
+// the same rows' positions; records are moved as grec / 8 words) and record
+// each partition's run.
 __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 {
 	__syncthreads();
@@ -522,34 +552,33 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 	for (int p = threadIdx.x; p < NP; p += blockDim.x)
 		off[p] = L.keys[p];
 	__syncthreads();
-	const u64 W = (u64)P.nsums + 1;
-	const int ns = P.nsums;
+	const int W = P.grec / 8;
+	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
 	// every word of a thread's V tuples is loaded before any is stored, so
 	// the copy costs one memory round trip per V tuples
 	constexpr int V = 4;
 	for (u32 i0 = 0; i0 < n; i0 += V * blockDim.x) {
-		u64 g[V];
-		u64 x[V][VH_MAX_AGGRS];
+		u64 x[V][kMaxW];
+		bool has[V];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			const u32 i = i0 + v * blockDim.x + threadIdx.x;
-			const VH_GLOBAL u64* src = (const VH_GLOBAL u64*)P.gtup0 + (u64)(L.rbase + (i < n ? i : 0u)) * W;
-			g[v] = i < n ? src[0] : ~0ull;
+			has[v] = i < n;
+			const VH_GLOBAL u64* src = (const VH_GLOBAL u64*)P.gtup0 + (u64)(L.rbase + (has[v] ? i : 0u)) * W;
 #pragma unroll
-			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
-				x[v][s2] = (i < n && s2 < ns) ? src[1 + s2] : 0ull;
+			for (int w = 0; w < kMaxW; w++)
+				x[v][w] = (has[v] && w < W) ? src[w] : 0ull;
 		}
 #pragma unroll
 		for (int v = 0; v < V; v++) {
-			if (g[v] == ~0ull)
+			if (!has[v])
 				continue;
-			const u32 pos = atomicAdd(&L.keys[g[v] >> P.gshift], 1u);
+			const u32 pos = atomicAdd(&L.keys[(u32)x[v][0] >> P.gshift], 1u);
 			VH_GLOBAL u64* dst = (VH_GLOBAL u64*)P.gtup1 + (u64)(L.rbase + pos) * W;
-			dst[0] = g[v];
 #pragma unroll
-			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
-				if (s2 < ns)
-					dst[1 + s2] = x[v][s2];
+			for (int w = 0; w < kMaxW; w++)
+				if (w < W)
+					dst[w] = x[v][w];
 		}
 	}
 	__syncthreads();
@@ -586,13 +615,15 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 	for (int r = threadIdx.x; r < R; r += blockDim.x)
 		base[r] = (u32)((i64)r * P.gregion_rows) + base[r] - pre[r];
 	__syncthreads();
-	const u64 W = (u64)ns + 1;
 	// every word of a thread's V tuples is loaded before the LDS adds
 	constexpr int V = 4;
+	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
+	const int W = P.grec / 8;
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
 		u64 g[V];
 		u64 x[V][VH_MAX_AGGRS];
 		bool has[V];
+		u64 wd[V][kMaxW];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			const u32 i = i0 + v * blockDim.x + threadIdx.x;
@@ -606,10 +637,16 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 					b = mid - 1;
 			}
 			const VH_GLOBAL u64* t = (const VH_GLOBAL u64*)P.gtup1 + (u64)(has[v] ? base[a] + i : 0u) * W;
-			g[v] = has[v] ? t[0] : 0ull;
+#pragma unroll
+			for (int w = 0; w < kMaxW; w++)
+				wd[v][w] = (has[v] && w < W) ? t[w] : 0ull;
+		}
+#pragma unroll
+		for (int v = 0; v < V; v++) {
+			g[v] = (u32)wd[v][0];
 #pragma unroll
 			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
-				x[v][s2] = (has[v] && s2 < ns) ? t[1 + s2] : 0ull;
+				x[v][s2] = s2 < ns ? part_field(P, wd[v], s2) : 0ull;
 		}
 #pragma unroll
 		for (int v = 0; v < V; v++) {

@@ -45,6 +45,10 @@ struct SumArg {
 	                     // overflow a 64-bit lane sum
 	int arg_signed;      // argument type signed (sign extends into i128)
 	int res128;          // result typed i128 (global modes accumulate exactly)
+	int narrow;          // the argument's exact interval fits 32 bits (signed
+	                     // if arg_signed): a partitioned group-by's tuples
+	                     // keep it in 4 bytes
+	int toff;            // its byte offset in such a tuple
 };
 
 struct Pred {
@@ -129,10 +133,12 @@ enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH 
 
 // GB_PART: partitioned dense group-by for domains too large for LDS.  The
 // probe rows are cut into regions of gregion_rows rows (a workgroup's slab,
-// or a batched work item).  A region writes one tuple per qualifying row,
-// (g, argument of every sum) as nsums + 1 words, into gtup0 at its rows'
-// own positions, counting them per partition (g >> gshift) in LDS; at its
-// end it reorders them by partition into gtup1 and records, per partition,
+// or a batched work item).  A region writes one tuple per qualifying row
+// into gtup0 at its rows' own positions: a record of grec bytes (a multiple
+// of 8) holding the group index as a u32 at byte 0 and each sum's argument
+// at SumArg::toff (4 bytes when SumArg::narrow, else 8, 8-aligned), counting
+// them per partition (g >> gshift) in LDS; at its end it reorders them by
+// partition into gtup1 and records, per partition,
 // the run's length (gcnt) and offset (goff) in [region][partition] tables.
 // part_agg_kernel then aggregates each partition's runs in an LDS slice and
 // adds the slice into out_*.  A row with more than one match (JoinN) that
@@ -177,6 +183,8 @@ struct Plan {
 	// partitioned mode (GB_PART)
 	u64* gtup0;
 	u64* gtup1;
+	int grec;            // bytes of a tuple record
+	int gpad;
 	u32* gcnt;
 	u32* goff;
 	i64 gregion_rows;
