@@ -25,12 +25,20 @@ namespace vh {
 __global__ __launch_bounds__(256) void batch_build_kernel(const BuildPlan* __restrict__ builds,
 	const BatchItem* __restrict__ items, u32 first, u32 count, u32* __restrict__ counter)
 {
-	__shared__ u32 slot;
-	BatchItem I;
-	while (next_batch_item(items, first, count, counter, &slot, I)) {
+	__shared__ u32 qs[2];
+	u32 cur, nxt;
+	queue_begin(counter, qs, cur, nxt);
+	BatchItem I = items[first + (cur < count ? cur : 0)];
+	while (cur < count) {
+		const u32 req = queue_request(counter);
+		const BatchItem In = items[first + (nxt < count ? nxt : cur)];
 		const BuildPlan& B = builds[I.build];
 		for (i64 base = I.r0; base < I.r1; base += (i64)kU * blockDim.x)
 			build_chunk(B, base, I.r1);
+		const u32 nn = queue_advance(qs, req);
+		cur = nxt;
+		nxt = nn;
+		I = In;
 	}
 }
 

@@ -28,6 +28,8 @@ hipError_t launch_batch(int kind, const Plan* d_plans, const BuildPlan* d_builds
 
 // A workgroup's next item of a persistent launch, or false when its queue is
 // empty (*counter is the launch's cursor, *slot a word of the workgroup's LDS).
+// For launches of large, uneven items; the build launches, whose items are
+// small, hold two at a time (queue_begin in vh_fused.hpp).
 __device__ __forceinline__ bool next_batch_item(const BatchItem* __restrict__ items, u32 first, u32 count,
 	u32* __restrict__ counter, u32* slot, BatchItem& I)
 {

@@ -444,17 +444,18 @@ __global__ __launch_bounds__(kPartBlock) void batch_part_scatter_kernel(const Bu
 	__shared__ u32 hist[kMaxParts];
 	__shared__ u64 staged[kPartChunk];
 	__shared__ u32 total;
-	__shared__ u32 slot;
-	for (;;) {
-		if (threadIdx.x == 0)
-			slot = gatomic_add(counter, 1u);
-		__syncthreads();
-		const u32 k = slot;
-		__syncthreads();
-		if (k >= count)
-			break;
-		const BatchItem I = items[first + k];
+	__shared__ u32 qs[2];
+	u32 cur, nxt;
+	queue_begin(counter, qs, cur, nxt);
+	BatchItem I = items[first + (cur < count ? cur : 0)];
+	while (cur < count) {
+		const u32 req = queue_request(counter);
+		const BatchItem In = items[first + (nxt < count ? nxt : cur)];
 		part_scatter_chunk(builds[I.build], (int)I.r0, hist, staged, &total);
+		const u32 nn = queue_advance(qs, req);
+		cur = nxt;
+		nxt = nn;
+		I = In;
 	}
 }
 
@@ -465,17 +466,18 @@ __global__ __launch_bounds__(kPartBlock) void batch_part_slice_kernel(const Buil
 	__shared__ u32 pre[kMaxChunks];
 	__shared__ u32 base[kMaxChunks];
 	__shared__ u32 total_rows;
-	__shared__ u32 slot;
-	for (;;) {
-		if (threadIdx.x == 0)
-			slot = gatomic_add(counter, 1u);
-		__syncthreads();
-		const u32 k = slot;
-		__syncthreads();
-		if (k >= count)
-			break;
-		const BatchItem I = items[first + k];
+	__shared__ u32 qs[2];
+	u32 cur, nxt;
+	queue_begin(counter, qs, cur, nxt);
+	BatchItem I = items[first + (cur < count ? cur : 0)];
+	while (cur < count) {
+		const u32 req = queue_request(counter);
+		const BatchItem In = items[first + (nxt < count ? nxt : cur)];
 		part_slice(builds[I.build], (int)I.r0, lds_words, pre, base, &total_rows);
+		const u32 nn = queue_advance(qs, req);
+		cur = nxt;
+		nxt = nn;
+		I = In;
 	}
 }
 
