@@ -82,6 +82,9 @@ struct Gen {
 	int late_stage = 0; // vector loop: stage before which late columns are loaded (0: none)
 	int block = 256;    // threads per workgroup
 	bool batch = false; // entry point of a batched launch (vh_generated_batch)
+	bool racc = false;  // a dense LDS group-by of one group: every lane sums in
+	                    // registers and adds into the LDS table once, at the
+	                    // end (64 lanes adding into one LDS word serialise)
 	bool radix = false; // P.radix: the probe kernel ends at the last join with
 	                    // partitioned tuples (vh_generated), the join kernel
 	                    // (vh_generated_join) takes them from there
@@ -308,6 +311,32 @@ struct Gen {
 		return prod.empty() ? "(u64)0" : prod;
 	}
 
+	// Register accumulators of the one-group dense LDS group-by (racc)
+	std::string racc_decls() const
+	{
+		const int ns = P.nsums > 0 ? P.nsums : 1;
+		return "\tu64 racc_lo[" + std::to_string(ns) + "] = {};\n\ti64 racc_hi[" + std::to_string(ns) +
+			"] = {};\n\tu32 racc_cnt = 0;\n\t(void)racc_hi;\n";
+	}
+
+	// ... added into the LDS table by every lane (before the flush's barrier)
+	std::string racc_flush() const
+	{
+		std::string k = "\tatomicAdd(&L.cnt[0], racc_cnt);\n";
+		char b[512];
+		for (int si = 0; si < P.nsums; si++) {
+			const SumArg& S = P.sums[si];
+			if (P.wide && S.wide)
+				snprintf(b, sizeof b, "\t{\n\t\tconst u64 o = atomicAdd(&L.lo[%d], racc_lo[%d]);\n"
+					"\t\tconst i64 h = racc_hi[%d] + (i64)((o + racc_lo[%d]) < o);\n"
+					"\t\tif (h) atomicAdd(&L.hi[%d], (int)h);\n\t}\n", si, si, si, si, si);
+			else
+				snprintf(b, sizeof b, "\tatomicAdd(&L.lo[%d], racc_lo[%d]);\n", si, si);
+			k += b;
+		}
+		return k;
+	}
+
 	// Aggregate one row (u, with the last join's payload tp): group index,
 	// count, sums; add_count / add_sum of vh_fused.hpp with the plan's
 	// flags as literals for the dense modes.  Dense group indexes are below
@@ -346,6 +375,25 @@ struct Gen {
 			for (int si = 0; si < P.nsums; si++) {
 				snprintf(b, sizeof b, "\t\t\tpart_tuple_value(P, t, %d, g, %s);\n", si,
 					sum_expr(si, tail_join, tail_pay).c_str());
+				s += b;
+			}
+			return s;
+		}
+		if (mode == GB_LDS && racc) {
+			// one group: the lane's register accumulators (exact: a carry
+			// count beside each 64-bit sum that needs 128 bits)
+			s += "\t\t\tracc_cnt++;\n";
+			for (int si = 0; si < P.nsums; si++) {
+				const SumArg& S = P.sums[si];
+				snprintf(b, sizeof b, "\t\t\t{\n\t\t\t\tconst u64 v = %s;\n\t\t\t\tconst u64 n = racc_lo[%d] + v;\n",
+					sum_expr(si, tail_join, tail_pay).c_str(), si);
+				s += b;
+				if (P.wide && S.wide) {
+					snprintf(b, sizeof b, "\t\t\t\tracc_hi[%d] += (i64)(n < racc_lo[%d]) - (i64)(%s && (i64)v < 0);\n", si, si,
+						S.arg_signed ? "true" : "false");
+					s += b;
+				}
+				snprintf(b, sizeof b, "\t\t\t\tracc_lo[%d] = n;\n\t\t\t}\n", si);
 				s += b;
 			}
 			return s;
@@ -841,6 +889,7 @@ struct Gen {
 	std::string source(int mode)
 	{
 		char b[1024];
+		racc = mode == GB_LDS && P.domain == 1 && !radix;
 		// register every probe column: predicates first, then join keys,
 		// group keys and sum arguments
 		for (int p = 0; p < P.npreds; p++) {
@@ -1053,8 +1102,12 @@ struct Gen {
 				k += "\tconst __amdgpu_buffer_rsrc_t rs" + std::to_string(i) +
 					" = __builtin_amdgcn_make_buffer_rsrc((void*)" + cols[i].path + ", (short)0, (int)(P.rows * " +
 					std::to_string(host_width(cols[i].type)) + "), 0x00020000);\n";
+		if (racc)
+			k += racc_decls();
 		k += "\t__syncthreads();\n\tconst i64 rows = P.rows;\n\tconst i64 tile = (i64)U * blockDim.x;\n";
 		k += loop;
+		if (racc)
+			k += racc_flush();
 		if (batch) {
 			// the item's LDS group-by state goes to the pipeline's global
 			// arrays before the next item reuses the LDS
@@ -1084,6 +1137,7 @@ struct Gen {
 	std::string join_source(int mode)
 	{
 		char b[1024];
+		racc = mode == GB_LDS && P.domain == 1;
 		const int last = P.njoins - 1;
 		const JoinT& J = P.joins[last];
 		snprintf(b, sizeof b, "P.joins[%d]", last);
@@ -1236,8 +1290,12 @@ struct Gen {
 			k += "\tif (gload(" + jp + ".dups, 0)) dups_mask |= " + std::to_string(1u << last) + "u;\n";
 		k += "\tif (gload(" + jp + ".spill, 0)) dups_mask |= " + std::to_string(1u << (kSpillBit + last)) + "u;\n";
 		k += decls;
+		if (racc)
+			k += racc_decls();
 		k += "\t__syncthreads();\n";
 		k += loop;
+		if (racc)
+			k += racc_flush();
 		k += "\t__syncthreads();\n";
 		if (mode == GB_LDS)
 			k += "\tlds_dense_flush(P, L);\n";
