@@ -1047,8 +1047,7 @@ int query_create(Query* q, bool single)
 		const u64 regions = (rows + (u64)P.gregion_rows - 1) / (u64)P.gregion_rows;
 		if ((rc = q->alloc((void**)&P.gtup0, rows * (u64)P.grec)) ||
 			(rc = q->alloc((void**)&P.gtup1, rows * (u64)P.grec)) ||
-			(rc = q->alloc((void**)&P.gcnt, regions * (u64)P.gparts * sizeof(u32))) ||
-			(rc = q->alloc((void**)&P.goff, regions * (u64)P.gparts * sizeof(u32))))
+			(rc = q->alloc((void**)&P.gend, regions * (u64)P.gparts * sizeof(u32))))
 			return rc;
 	}
 	if (P.mode == GB_HASH) {

@@ -541,16 +541,13 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 	__syncthreads();
 	const u32 n = (u32)L.hi[0] < L.rcap ? (u32)L.hi[0] : L.rcap;
 	const int NP = P.gparts;
-	u32* cnt = P.gcnt + (u64)L.region * NP;
-	u32* off = P.goff + (u64)L.region * NP;
-	for (int p = threadIdx.x; p < NP; p += blockDim.x) {
-		cnt[p] = L.cnt[p];
+	u32* end = P.gend + (u64)L.region * NP;
+	for (int p = threadIdx.x; p < NP; p += blockDim.x)
 		L.keys[p] = L.cnt[p];
-	}
 	__syncthreads();
 	block_exclusive_scan(L.keys, NP, (u32*)&L.hi[1]);
 	for (int p = threadIdx.x; p < NP; p += blockDim.x)
-		off[p] = L.keys[p];
+		end[p] = L.keys[p] + L.cnt[p];
 	__syncthreads();
 	const int W = P.grec / 8;
 	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
@@ -604,9 +601,13 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 		cnt[i] = 0;
 	const int NP = P.gparts;
 	const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
+	// region r's run of partition p: [end[r][p - 1], end[r][p]) of the
+	// region (two neighbouring words, one line read per region)
 	for (int r = threadIdx.x; r < R; r += blockDim.x) {
-		pre[r] = P.gcnt[(u64)r * NP + p];
-		base[r] = P.goff[(u64)r * NP + p];
+		const u32* e = P.gend + (u64)r * NP;
+		const u32 b = p ? e[p - 1] : 0u;
+		pre[r] = e[p] - b;
+		base[r] = b;
 	}
 	__syncthreads();
 	const u32 total = block_exclusive_scan(pre, R, total_slot);

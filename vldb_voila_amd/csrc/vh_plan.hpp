@@ -139,7 +139,8 @@ enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH 
 // at SumArg::toff (4 bytes when SumArg::narrow, else 8, 8-aligned), counting
 // them per partition (g >> gshift) in LDS; at its end it reorders them by
 // partition into gtup1 and records, per partition,
-// the run's length (gcnt) and offset (goff) in [region][partition] tables.
+// the run's end (gend, inclusive prefix of the counts) in a [region][partition]
+// table.
 // part_agg_kernel then aggregates each partition's runs in an LDS slice and
 // adds the slice into out_*.  A row with more than one match (JoinN) that
 // finds its region's tuple space full adds straight into out_* with atomics.
@@ -185,8 +186,7 @@ struct Plan {
 	u64* gtup1;
 	int grec;            // bytes of a tuple record
 	int gpad;
-	u32* gcnt;
-	u32* goff;
+	u32* gend;
 	i64 gregion_rows;
 	int gshift;
 	int gparts;
