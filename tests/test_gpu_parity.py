@@ -508,3 +508,75 @@ def test_more_than_2_31_rows():
     for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_GENERIC):
         got = runtime.run_pipeline(p, runtime.make_config(kernel=kernel)).strip()
         assert got == f"{want_sum}|{want_cnt}", (kernel, got)
+
+
+def test_global_aggregate_register_sums():
+    """One group (no keys): every lane sums in registers and adds into LDS
+    once.  Signed 64-bit values near 2^62 into an int128 sum (the carry
+    count), a narrow signed sum and a count, with and without a selection,
+    as a single query and as a batch's pipeline."""
+    print("global aggregate, register sums", flush=True)
+    rng = np.random.default_rng(17)
+    n = 3_000_000
+    t = {"a": rng.integers(-2**62, 2**62, n).astype(np.int64),
+         "c": rng.integers(-30000, 30000, n).astype(np.int32),
+         "f": rng.integers(0, 10, n).astype(np.int32)}
+
+    def make(addr, select):
+        p = Pipeline(table_from_arrays("t", t, addr))
+        if select:
+            p.select("f", "<", 7)
+        p.groupby([], [("sum", col("a")), ("sum", col("c")), ("count",)])
+        return p
+    devs = [runtime.upload(t)]
+    for select in (False, True):
+        want = oracle.sorted_rows(oracle.bruteforce(make(None, select).desc()))
+        p = make(runtime.addrs(devs[0]), select)
+        src = runtime.Query(p.desc()).kernel()[1]
+        assert "racc_lo" in src
+        got = oracle.sorted_rows(runtime.run_pipeline(p))
+        assert got == want, select
+        batch = runtime.Batch([p.desc(), p.desc()])
+        s = runtime.Stream()
+        for rep in range(2):
+            batch.run_graph(s)
+            s.synchronize()
+            for i in range(2):
+                assert oracle.sorted_rows(batch.result(i, s)) == want, (select, rep, i)
+        batch.free()
+
+
+def test_partitioned_group_by_packed_records():
+    """Partitioned group-by records: a signed argument that fits 32 bits is
+    kept in 4 bytes and sign-extended, a wide one in 8 (into int128 sums),
+    beside the group index; single query and batch, repeated runs (the
+    result arrays are not cleared between runs without a fan-out)."""
+    print("partitioned group-by, packed records", flush=True)
+    rng = np.random.default_rng(23)
+    n = 1_500_000
+    t = {"g": rng.integers(0, 200_000, n).astype(np.int32),
+         "c": rng.integers(-2**31, 2**31 - 1, n).astype(np.int32),
+         "a": rng.integers(-2**62, 2**62, n).astype(np.int64),
+         "f": rng.integers(0, 4, n).astype(np.int32)}
+
+    def make(addr):
+        p = Pipeline(table_from_arrays("t", t, addr))
+        p.select("f", "!=", 2)
+        p.groupby(["g"], [("sum", col("c")), ("sum", col("a")), ("count",)])
+        return p
+    want = oracle.sorted_rows(oracle.bruteforce(make(None).desc()))
+    devs = [runtime.upload(t)]
+    p = make(runtime.addrs(devs[0]))
+    cfg = runtime.make_config(group_mode=abi.VH_GROUP_PART)
+    q = runtime.Query(p.desc(), cfg)
+    assert q.info()["group_mode"] == abi.VH_GROUP_PART
+    for rep in range(2):
+        q.run()
+        assert oracle.sorted_rows(q.result()) == want, rep
+    batch = runtime.Batch([p.desc()], [cfg])
+    s = runtime.Stream()
+    for rep in range(3):
+        batch.run_graph(s)
+        s.synchronize()
+        assert oracle.sorted_rows(batch.result(0, s)) == want, rep
+    batch.free()
