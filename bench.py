@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 # gives the commands and the counter values
 TRAFFIC_FILES = {"join": "traffic_join_1.5Mx6M.md", "join_sparse": "traffic_join_sparse_1.5Mx6M.md",
                  "filter": "traffic_filter_groupby_6M.md", "star": "traffic_star_60M.md",
-                 "contains": "traffic_contains_10M.md"}
+                 "contains": "traffic_contains_10M.md", "batch": "kernels_batch_256.md"}
 METRIC = "input tuples/s + % HBM peak; learned-selector vs default/exhaustive blend"
 
 
@@ -514,7 +514,8 @@ def bench_batch(args):
                    "tuned": tuned_info},
         "roofline": {"bound": "hbm", "kernel": "whole batch (clear + builds + probes)",
                      "achieved": alg / (batched_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": alg / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": alg / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("batch", 0.0),
                      "avg_kernel_ms": batched_ms, "algorithmic_bytes": alg},
         "wall_s": wall,
     }
