@@ -87,16 +87,18 @@ struct vh_batch {
 	}
 };
 
-static int tile_rows_of(const vh_config& c)
+// Rows of a probe work item: 16 K by default, 8 K for a partitioned
+// group-by, whose item stages its first 32 KiB of tuples in LDS and sends
+// the rest through memory (measured on the 256-pipeline batch, staged
+// tuples: 4.39 ms at 8 K rows, 4.63 at 16 K, 5.18 at 4 K; without the
+// stage: 4.88 at 16 K)
+static int tile_rows_of(const vh_config& c, bool part = false)
 {
 	int shift = c.tile_shift;
 	if (shift < -6) shift = -6;
 	if (shift > 6) shift = 6;
-	// 16 K rows by default: a partitioned group-by's tuples of one item are
-	// re-read (sorted by partition) soon after they are written, and smaller
-	// items keep them cached (measured on the 256-pipeline batch: 5.52 ms at
-	// 16 K rows, 5.63 at 32 K, 5.86 at 64 K, 6.14 at 128 K)
-	return shift >= 0 ? (16384 << shift) : (16384 >> (-shift));
+	const int base = part ? 8192 : 16384;
+	return shift >= 0 ? (base << shift) : (base >> (-shift));
 }
 
 // Lowering of one pipeline of a batch: lane-private group-by state cannot
@@ -114,13 +116,20 @@ static int lower_for_batch(int i, const vh_pipeline_desc* d, const vh_config& cf
 	Plan& P = L.plan;
 	if (P.mode == GB_REG)
 		P.mode = GB_LDS;
-	i64 T = tile_rows_of(cfg);
+	i64 T = tile_rows_of(cfg, P.mode == GB_PART);
 	if (P.mode == GB_PART) {
 		// the default item size grows for a pipeline that would otherwise
 		// have more regions than a partition's run table holds
 		while (cfg.tile_shift == 0 && (P.rows + T - 1) / T > kMaxGroupRegions && T < ((i64)1 << 30))
 			T <<= 1;
 		P.gregion_rows = T;
+		P.gstage = part_stage(P);
+		// one aggregation item per partition, each paying a fixed cost (its
+		// runs' table, the slice's clear and store): as few partitions as
+		// the slice size allows, more for a pipeline with many rows, about
+		// 32 K rows each (batch: 4.22 ms at 32 K rows per partition, 4.32 at
+		// 16 K, 4.48 at 64 K, 4.39 with ~256 partitions per pipeline)
+		part_partitions(P, (P.rows + 32767) / 32768);
 		if ((P.rows + T - 1) / T > kMaxGroupRegions)
 			return fail(VH_ERR_UNSUPPORTED, "pipeline %d: more than %d items for a partitioned group-by", i,
 				kMaxGroupRegions);

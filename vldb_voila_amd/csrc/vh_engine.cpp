@@ -121,6 +121,23 @@ static int part_shift(int nsums)
 	return shift;
 }
 
+// GB_PART: the tuples of a region staged in LDS (the rest go through gtup0)
+int part_stage(const Plan& P)
+{
+	return (int)std::min<i64>(kPartStageBytes / P.grec, P.gregion_rows);
+}
+
+// GB_PART slices: the largest the LDS allows (2^part_shift groups), halved
+// (down to 64 groups) while half the slice would still give fewer than
+// `want` partitions (a workgroup or work item each)
+void part_partitions(Plan& P, i64 want)
+{
+	P.gshift = part_shift(P.nsums);
+	while (P.gshift > 6 && (i64)(P.domain >> (P.gshift - 1)) < want)
+		P.gshift--;
+	P.gparts = (int)((P.domain + (1ull << P.gshift) - 1) >> P.gshift);
+}
+
 static bool part_fits(double domain, int nsums)
 {
 	return nsums <= 4 && domain <= (double)kMaxGroupParts * (double)(1u << part_shift(nsums));
@@ -757,12 +774,8 @@ int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L)
 		const int64_t rows = std::max<int64_t>(P.rows, 1);
 		P.gregion_rows = ((rows + grid - 1) / grid + 1023) / 1024 * 1024;
 		grid = (int)((rows + P.gregion_rows - 1) / P.gregion_rows);
-		// slices small enough for ~256 partitions (a workgroup each), no
-		// larger than the LDS allows
-		P.gshift = part_shift(P.nsums);
-		while (P.gshift > 6 && (P.domain >> (P.gshift - 1)) < 256)
-			P.gshift--;
-		P.gparts = (int)((P.domain + (1ull << P.gshift) - 1) >> P.gshift);
+		P.gstage = part_stage(P);
+		part_partitions(P, 256);
 	}
 	L.grid = grid;
 	// dense LDS group-by: workgroups leave partials for merge_kernel, or add

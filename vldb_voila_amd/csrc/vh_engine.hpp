@@ -95,6 +95,11 @@ const char* last_error();
 int type_from_minmax(double mn, double mx);
 int type_pipeline(const vh_pipeline_desc* d, Typing& ty);
 int lower(const vh_pipeline_desc* d, const vh_config* cfg, Lowered& L);
+// GB_PART: tuples of a region (Plan::gregion_rows) staged in LDS
+int part_stage(const Plan& P);
+// GB_PART: slice size (gshift) and partitions (gparts), at least `want`
+// partitions where slices of 64 groups allow
+void part_partitions(Plan& P, i64 want);
 // single: a query of its own (generated probe kernel per cfg.kernel,
 // partitioned builds); false for the pipelines of a batch
 int query_create(Query* q, bool single = true);

@@ -291,10 +291,12 @@ struct Lds {
 	u32* keys;
 	// GB_PART: this region's tuple space (rows [rbase, rbase + rcap)) and
 	// its index in the [region][partition] tables; cnt = per-partition
-	// counts, keys = per-partition cursors, hi = the region's tuple count
+	// counts, keys = per-partition cursors, hi = the region's tuple count,
+	// stage = its first Plan::gstage tuples
 	i64 rbase;
 	u32 rcap;
 	u32 region;
+	char* stage;
 };
 
 // One slot per active lane of the wave from a counter in memory, with one
@@ -450,12 +452,12 @@ __device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 
 // GB_PART tuple records (Plan::grec bytes): sum s's argument at
 // SumArg::toff, in 4 bytes (SumArg::narrow, sign-extended when the argument
 // is signed and read back) or 8.
-__device__ __forceinline__ void part_put(const Plan& P, VH_GLOBAL char* rec, int s, u64 v)
+__device__ __forceinline__ void part_put(const Plan& P, char* rec, int s, u64 v)
 {
 	if (P.sums[s].narrow)
-		*(VH_GLOBAL u32*)(rec + P.sums[s].toff) = (u32)v;
+		*(u32*)(rec + P.sums[s].toff) = (u32)v;
 	else
-		*(VH_GLOBAL u64*)(rec + P.sums[s].toff) = v;
+		*(u64*)(rec + P.sums[s].toff) = v;
 }
 
 // Sum s's argument out of a tuple record held as words in registers (the
@@ -478,24 +480,27 @@ __device__ __forceinline__ u64 part_field(const Plan& P, const u64 (&wd)[N], int
 
 // GB_PART, one tuple per live row: a slot in the region's tuple space (one
 // atomic per wave) and the group index there, then each sum's argument; the
-// partition's count in LDS.  Rows beyond the space (JoinN fan-out) add into
-// the global arrays (t = null).
-__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, VH_GLOBAL char*& t)
+// partition's count in LDS.  The slot is in the LDS stage for the region's
+// first gstage tuples, in gtup0 after them (t is a generic pointer to
+// either).  Rows beyond the space (JoinN fan-out) add into the global
+// arrays (t = null).
+__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, char*& t)
 {
 	t = nullptr;
 	if (!live)
 		return;
 	const u32 pos = wave_append((u32*)L.hi);
 	if (pos < L.rcap) {
-		t = (VH_GLOBAL char*)P.gtup0 + (u64)(L.rbase + pos) * (u64)P.grec;
-		*(VH_GLOBAL u32*)t = (u32)g;
+		t = pos < (u32)P.gstage ? L.stage + (u64)pos * (u64)P.grec
+			: (char*)P.gtup0 + (u64)(L.rbase + pos) * (u64)P.grec;
+		*(u32*)t = (u32)g;
 		atomicAdd(&L.cnt[g >> P.gshift], 1u);
 	} else {
 		gatomic_add(&P.out_cnt[g], 1ull);
 	}
 }
 
-__device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL char* t, int s, u64 g, u64 v)
+__device__ __forceinline__ void part_tuple_value(const Plan& P, char* t, int s, u64 g, u64 v)
 {
 	if (t) {
 		part_put(P, t, s, v);
@@ -509,7 +514,7 @@ __device__ __forceinline__ void part_tuple_value(const Plan& P, VH_GLOBAL char* 
 __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, const Chunk& C, const u64 g[kU],
 	const bool ok[kU])
 {
-	VH_GLOBAL char* t[kU];
+	char* t[kU];
 	bool live[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
@@ -533,9 +538,9 @@ __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, cons
 	}
 }
 
-// End of a GB_PART region: order its tuples by partition (gtup0 -> gtup1,
-// the same rows' positions; records are moved as grec / 8 words) and record
-// each partition's run.
+// End of a GB_PART region: order its tuples by partition (LDS stage, then
+// gtup0 -> gtup1, the same rows' positions; records are moved as grec / 8
+// words) and record each partition's run.
 __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 {
 	__syncthreads();
@@ -551,10 +556,24 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 	__syncthreads();
 	const int W = P.grec / 8;
 	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
+	const u32 staged = n < (u32)P.gstage ? n : (u32)P.gstage;
+	for (u32 i = threadIdx.x; i < staged; i += blockDim.x) {
+		const u64* src = (const u64*)(L.stage + (u64)i * (u64)P.grec);
+		u64 x[kMaxW];
+#pragma unroll
+		for (int w = 0; w < kMaxW; w++)
+			x[w] = w < W ? src[w] : 0ull;
+		const u32 pos = atomicAdd(&L.keys[(u32)x[0] >> P.gshift], 1u);
+		VH_GLOBAL u64* dst = (VH_GLOBAL u64*)P.gtup1 + (u64)(L.rbase + pos) * W;
+#pragma unroll
+		for (int w = 0; w < kMaxW; w++)
+			if (w < W)
+				dst[w] = x[w];
+	}
 	// every word of a thread's V tuples is loaded before any is stored, so
 	// the copy costs one memory round trip per V tuples
 	constexpr int V = 4;
-	for (u32 i0 = 0; i0 < n; i0 += V * blockDim.x) {
+	for (u32 i0 = staged; i0 < n; i0 += V * blockDim.x) {
 		u64 x[V][kMaxW];
 		bool has[V];
 #pragma unroll
@@ -988,6 +1007,7 @@ __device__ __forceinline__ Lds lds_layout(const Plan& P, u64* lds)
 		L.cnt = (u32*)lds;
 		L.keys = L.cnt + P.gparts;
 		L.hi = (int*)(L.keys + P.gparts);   // [0] tuple count, [1] scan total
+		L.stage = (char*)lds + (((u64)P.gparts * 8 + 16 + 15) & ~15ull);
 	}
 	return L;
 }

@@ -549,7 +549,7 @@ size_t groupby_lds_bytes(const Plan& P)
 	else if (P.mode == GB_LDSHASH)
 		b = (size_t)P.nsums * kLdsHashSlots * 12 + kLdsHashSlots * 8 + 16;
 	else if (P.mode == GB_PART)
-		b = (size_t)P.gparts * 8 + 16;
+		b = (((size_t)P.gparts * 8 + 16 + 15) & ~(size_t)15) + (size_t)P.gstage * (size_t)P.grec;
 	return (b + 15) & ~(size_t)15;
 }
 

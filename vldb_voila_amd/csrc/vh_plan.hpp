@@ -133,19 +133,21 @@ enum GroupMode { GB_REG = 0, GB_LDS = 1, GB_GLOBAL = 2, GB_HASH = 3, GB_LDSHASH 
 
 // GB_PART: partitioned dense group-by for domains too large for LDS.  The
 // probe rows are cut into regions of gregion_rows rows (a workgroup's slab,
-// or a batched work item).  A region writes one tuple per qualifying row
-// into gtup0 at its rows' own positions: a record of grec bytes (a multiple
-// of 8) holding the group index as a u32 at byte 0 and each sum's argument
-// at SumArg::toff (4 bytes when SumArg::narrow, else 8, 8-aligned), counting
-// them per partition (g >> gshift) in LDS; at its end it reorders them by
-// partition into gtup1 and records, per partition,
-// the run's end (gend, inclusive prefix of the counts) in a [region][partition]
-// table.
+// or a batched work item).  A region keeps one tuple per qualifying row: a
+// record of grec bytes (a multiple of 8) holding the group index as a u32 at
+// byte 0 and each sum's argument at SumArg::toff (4 bytes when
+// SumArg::narrow, else 8, 8-aligned), counted per partition (g >> gshift) in
+// LDS.  The region's first gstage tuples are staged in LDS, the rest go to
+// gtup0 at their rows' own positions; at its end the region stores them
+// ordered by partition into gtup1 (its rows' positions) and records, per
+// partition, the run's end (gend, inclusive prefix of the counts) in a
+// [region][partition] table.
 // part_agg_kernel then aggregates each partition's runs in an LDS slice and
 // adds the slice into out_*.  A row with more than one match (JoinN) that
 // finds its region's tuple space full adds straight into out_* with atomics.
 static constexpr int kMaxGroupParts = 4096;
 static constexpr int kMaxGroupRegions = 2048;
+static constexpr int kPartStageBytes = 32 * 1024;   // LDS staging of a region's tuples
 
 // GB_LDSHASH: per-block LDS open-addressing table over the dense group index,
 // kLdsHashSlots entries, at most kLdsHashProbes probes; a tuple that finds no
@@ -185,7 +187,7 @@ struct Plan {
 	u64* gtup0;
 	u64* gtup1;
 	int grec;            // bytes of a tuple record
-	int gpad;
+	int gstage;          // tuples of a region staged in LDS
 	u32* gend;
 	i64 gregion_rows;
 	int gshift;
