@@ -90,8 +90,9 @@ struct vh_batch {
 // Rows of a probe work item: 16 K by default, 8 K for a partitioned
 // group-by, whose item stages its first 32 KiB of tuples in LDS and sends
 // the rest through memory (measured on the 256-pipeline batch, staged
-// tuples: 4.39 ms at 8 K rows, 4.63 at 16 K, 5.18 at 4 K; without the
-// stage: 4.88 at 16 K)
+// tuples: 4.22 ms at 8 K rows, 4.26 at 4 K; 4.42 and 4.58 with 48 KiB
+// staged, which leaves fewer workgroups per CU; without the stage: 4.88 at
+// 16 K)
 static int tile_rows_of(const vh_config& c, bool part = false)
 {
 	int shift = c.tile_shift;
@@ -330,6 +331,14 @@ int vh_batch_new(vh_batch** out, int32_t n, const vh_pipeline_desc* descs, const
 		k = e;
 	}
 	items.insert(items.end(), probe_items.begin(), probe_items.end());
+	// aggregation items, longest first (rows per partition of their
+	// pipeline): the workgroups that take the last items then wait less
+	// for the longest ones
+	std::stable_sort(agg_items.begin(), agg_items.end(), [&](const BatchItem& x, const BatchItem& y) {
+		const Plan& a = plans[x.plan];
+		const Plan& c = plans[y.plan];
+		return (double)a.rows / a.gparts > (double)c.rows / c.gparts;
+	});
 	b->launches.push_back({kBatchAgg, (u32)items.size(), (u32)agg_items.size(), b->agg_lds, nullptr});
 	items.insert(items.end(), agg_items.begin(), agg_items.end());
 	for (const BatchLaunch& l : b->launches)

@@ -621,12 +621,26 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 	const int NP = P.gparts;
 	const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
 	// region r's run of partition p: [end[r][p - 1], end[r][p]) of the
-	// region (two neighbouring words, one line read per region)
-	for (int r = threadIdx.x; r < R; r += blockDim.x) {
-		const u32* e = P.gend + (u64)r * NP;
-		const u32 b = p ? e[p - 1] : 0u;
-		pre[r] = e[p] - b;
-		base[r] = b;
+	// region (two neighbouring words, one line read per region); a thread's
+	// eight regions are loaded before any is stored
+	constexpr int RV = 8;
+	for (int r0 = threadIdx.x; r0 < R; r0 += RV * blockDim.x) {
+		u32 lo_e[RV], hi_e[RV];
+#pragma unroll
+		for (int k = 0; k < RV; k++) {
+			const int r = r0 + k * (int)blockDim.x;
+			const VH_GLOBAL u32* e = (const VH_GLOBAL u32*)P.gend + (u64)(r < R ? r : 0) * NP;
+			lo_e[k] = (r < R && p) ? e[p - 1] : 0u;
+			hi_e[k] = r < R ? e[p] : 0u;
+		}
+#pragma unroll
+		for (int k = 0; k < RV; k++) {
+			const int r = r0 + k * (int)blockDim.x;
+			if (r < R) {
+				pre[r] = hi_e[k] - lo_e[k];
+				base[r] = lo_e[k];
+			}
+		}
 	}
 	__syncthreads();
 	const u32 total = block_exclusive_scan(pre, R, total_slot);
