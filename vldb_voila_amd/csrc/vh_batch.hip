@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void batch_agg_kernel(const Plan* __restrict__
 		const u64 slice_words = (((u64)1 << P.gshift) * ((u64)P.nsums * 12 + 4) + 7) / 8;
 		const int R = (int)((P.rows + P.gregion_rows - 1) / P.gregion_rows);
 		u32* pre = (u32*)(lds + slice_words);
-		part_agg(P, (int)I.r0, lds, pre, pre + R, pre + 2 * R);
+		part_agg<4>(P, (int)I.r0, lds, pre, pre + R, pre + 2 * R);
 		__syncthreads();
 	}
 }

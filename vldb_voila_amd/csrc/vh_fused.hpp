@@ -602,8 +602,10 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 
 // GB_PART, one workgroup per partition p: aggregate the partition's runs of
 // every region in an LDS slice of 2^gshift groups, then add the slice into
-// out_* (this workgroup is the only writer of these groups).
-__device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pre, u32* base, u32* total_slot)
+// out_* (this workgroup is the only writer of these groups).  A thread
+// loads V tuples of at most KW words before adding any.
+template <int V, int KW>
+__device__ __forceinline__ void part_agg_t(const Plan& P, int p, u64* lds, u32* pre, u32* base, u32* total_slot)
 {
 	const u64 S = 1ull << P.gshift;
 	const u64 g0 = (u64)p * S;
@@ -650,14 +652,10 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 		base[r] = (u32)((i64)r * P.gregion_rows) + base[r] - pre[r];
 	__syncthreads();
 	// every word of a thread's V tuples is loaded before the LDS adds
-	constexpr int V = 4;
-	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
 	const int W = P.grec / 8;
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
-		u64 g[V];
-		u64 x[V][VH_MAX_AGGRS];
 		bool has[V];
-		u64 wd[V][kMaxW];
+		u64 wd[V][KW];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			const u32 i = i0 + v * blockDim.x + threadIdx.x;
@@ -672,28 +670,21 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 			}
 			const VH_GLOBAL u64* t = (const VH_GLOBAL u64*)P.gtup1 + (u64)(has[v] ? base[a] + i : 0u) * W;
 #pragma unroll
-			for (int w = 0; w < kMaxW; w++)
+			for (int w = 0; w < KW; w++)
 				wd[v][w] = (has[v] && w < W) ? t[w] : 0ull;
-		}
-#pragma unroll
-		for (int v = 0; v < V; v++) {
-			g[v] = (u32)wd[v][0];
-#pragma unroll
-			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++)
-				x[v][s2] = s2 < ns ? part_field(P, wd[v], s2) : 0ull;
 		}
 #pragma unroll
 		for (int v = 0; v < V; v++) {
 			if (!has[v])
 				continue;
-			const u64 li = g[v] - g0;
+			const u64 li = (u32)wd[v][0] - g0;
 			atomicAdd(&cnt[li], 1u);
 #pragma unroll
 			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
 				if (s2 < ns) {
 					const SumArg& A = P.sums[s2];
-					add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], x[v][s2], P.wide && A.wide,
-						A.arg_signed);
+					add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], part_field(P, wd[v], s2),
+						P.wide && A.wide, A.arg_signed);
 				}
 			}
 		}
@@ -754,6 +745,17 @@ __device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pr
 			}
 		}
 	}
+}
+
+// part_agg_t with the tuple width of the plan: records of up to 16 bytes
+// (two words) keep twice the tuples in flight in the same registers.
+template <int V>
+__device__ __forceinline__ void part_agg(const Plan& P, int p, u64* lds, u32* pre, u32* base, u32* total_slot)
+{
+	if (P.grec <= 16)
+		part_agg_t<2 * V, 2>(P, p, lds, pre, base, total_slot);
+	else
+		part_agg_t<V, (4 + 8 * VH_MAX_AGGRS + 7) / 8>(P, p, lds, pre, base, total_slot);
 }
 
 // Group index (dense modes) or packed key (hash mode) of the chunk's live

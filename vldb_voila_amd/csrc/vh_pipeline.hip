@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kPartBlockAgg) void part_agg_kernel(const Plan* __r
 	__shared__ u32 pre[kMaxGroupRegions];
 	__shared__ u32 base[kMaxGroupRegions];
 	__shared__ u32 total;
-	part_agg(*Pp, blockIdx.x, slice, pre, base, &total);
+	part_agg<4>(*Pp, blockIdx.x, slice, pre, base, &total);
 }
 
 __global__ __launch_bounds__(kBlock) void build_kernel(const BuildPlan* __restrict__ Bp)
