@@ -449,15 +449,32 @@ __device__ __forceinline__ void add_sum(const Plan& P, const Lds& L, int s, u64 
 	}
 }
 
-// GB_PART tuple records (Plan::grec bytes): sum s's argument at
-// SumArg::toff, in 4 bytes (SumArg::narrow, sign-extended when the argument
-// is signed and read back) or 8.
-__device__ __forceinline__ void part_put(const Plan& P, char* rec, int s, u64 v)
+// GB_PART tuple records (Plan::grec bytes): the group index (u32) at byte
+// 0, sum s's argument at SumArg::toff, in 4 bytes (SumArg::narrow,
+// sign-extended when the argument is signed and read back) or 8.  Tuple t
+// of a region (its slot) is in the LDS stage while t < gstage, else in
+// gtup0 at the region's row t; each store goes through the pointer of its
+// own address space (a generic pointer to either would compile to FLAT
+// stores, which the compiler can only wait for together with every load in
+// flight).
+#define VH_LDS __attribute__((address_space(3)))
+static constexpr u32 kPartNoSlot = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void part_store(const Plan& P, const Lds& L, u32 t, int off, u64 v, bool wide)
 {
-	if (P.sums[s].narrow)
-		*(u32*)(rec + P.sums[s].toff) = (u32)v;
-	else
-		*(u64*)(rec + P.sums[s].toff) = v;
+	if (t < (u32)P.gstage) {
+		VH_LDS char* r = (VH_LDS char*)L.stage + t * (u32)P.grec + (u32)off;
+		if (wide)
+			*(VH_LDS u64*)r = v;
+		else
+			*(VH_LDS u32*)r = (u32)v;
+	} else {
+		VH_GLOBAL char* r = (VH_GLOBAL char*)P.gtup0 + (u64)(L.rbase + t) * (u64)P.grec + off;
+		if (wide)
+			*(VH_GLOBAL u64*)r = v;
+		else
+			*(VH_GLOBAL u32*)r = (u32)v;
+	}
 }
 
 // Sum s's argument out of a tuple record held as words in registers (the
@@ -478,32 +495,29 @@ __device__ __forceinline__ u64 part_field(const Plan& P, const u64 (&wd)[N], int
 	return word;
 }
 
-// GB_PART, one tuple per live row: a slot in the region's tuple space (one
-// atomic per wave) and the group index there, then each sum's argument; the
-// partition's count in LDS.  The slot is in the LDS stage for the region's
-// first gstage tuples, in gtup0 after them (t is a generic pointer to
-// either).  Rows beyond the space (JoinN fan-out) add into the global
-// arrays (t = null).
-__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, char*& t)
+// GB_PART, one tuple per live row: a slot t in the region's tuple space
+// (one atomic per wave) and the group index there, then each sum's
+// argument; the partition's count in LDS.  Rows beyond the space (JoinN
+// fan-out) add into the global arrays (t = kPartNoSlot).
+__device__ __forceinline__ void part_tuple_slot(const Plan& P, const Lds& L, u64 g, bool live, u32& t)
 {
-	t = nullptr;
+	t = kPartNoSlot;
 	if (!live)
 		return;
 	const u32 pos = wave_append((u32*)L.hi);
 	if (pos < L.rcap) {
-		t = pos < (u32)P.gstage ? L.stage + (u64)pos * (u64)P.grec
-			: (char*)P.gtup0 + (u64)(L.rbase + pos) * (u64)P.grec;
-		*(u32*)t = (u32)g;
+		t = pos;
+		part_store(P, L, t, 0, g, false);
 		atomicAdd(&L.cnt[g >> P.gshift], 1u);
 	} else {
 		gatomic_add(&P.out_cnt[g], 1ull);
 	}
 }
 
-__device__ __forceinline__ void part_tuple_value(const Plan& P, char* t, int s, u64 g, u64 v)
+__device__ __forceinline__ void part_tuple_value(const Plan& P, const Lds& L, u32 t, int s, u64 g, u64 v)
 {
-	if (t) {
-		part_put(P, t, s, v);
+	if (t != kPartNoSlot) {
+		part_store(P, L, t, P.sums[s].toff, v, !P.sums[s].narrow);
 	} else {
 		const SumArg& S = P.sums[s];
 		add_global(&P.out_lo[(u64)s * P.domain + g], &P.out_hi[(u64)s * P.domain + g], v, P.wide && S.res128,
@@ -514,7 +528,7 @@ __device__ __forceinline__ void part_tuple_value(const Plan& P, char* t, int s, 
 __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, const Chunk& C, const u64 g[kU],
 	const bool ok[kU])
 {
-	char* t[kU];
+	u32 t[kU];
 	bool live[kU];
 #pragma unroll
 	for (int u = 0; u < kU; u++) {
@@ -533,7 +547,7 @@ __device__ __forceinline__ void part_consume_u(const Plan& P, const Lds& L, cons
 #pragma unroll
 			for (int u = 0; u < kU; u++)
 				if (live[u])
-					part_tuple_value(P, t[u], s, g[u], v[u]);
+					part_tuple_value(P, L, t[u], s, g[u], v[u]);
 		}
 	}
 }
@@ -546,7 +560,7 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 	__syncthreads();
 	const u32 n = (u32)L.hi[0] < L.rcap ? (u32)L.hi[0] : L.rcap;
 	const int NP = P.gparts;
-	u32* end = P.gend + (u64)L.region * NP;
+	VH_GLOBAL u32* end = (VH_GLOBAL u32*)P.gend + (u64)L.region * NP;
 	for (int p = threadIdx.x; p < NP; p += blockDim.x)
 		L.keys[p] = L.cnt[p];
 	__syncthreads();
@@ -558,7 +572,7 @@ __device__ __forceinline__ void part_finish(const Plan& P, const Lds& L)
 	constexpr int kMaxW = (4 + 8 * VH_MAX_AGGRS + 7) / 8;
 	const u32 staged = n < (u32)P.gstage ? n : (u32)P.gstage;
 	for (u32 i = threadIdx.x; i < staged; i += blockDim.x) {
-		const u64* src = (const u64*)(L.stage + (u64)i * (u64)P.grec);
+		const VH_LDS u64* src = (const VH_LDS u64*)((VH_LDS char*)L.stage + i * (u32)P.grec);
 		u64 x[kMaxW];
 #pragma unroll
 		for (int w = 0; w < kMaxW; w++)

@@ -371,9 +371,9 @@ struct Gen {
 			s += "\t\t\tif (!ok || (u64)g >= " + dom + ") { gatomic_or(" + err + ", 8u); return; }\n";
 		if (mode == GB_PART) {
 			// one tuple: a slot in the region's tuple space, then each sum's argument
-			s += "\t\t\tchar* t;\n\t\t\tpart_tuple_slot(P, L, g, true, t);\n";
+			s += "\t\t\tu32 t;\n\t\t\tpart_tuple_slot(P, L, g, true, t);\n";
 			for (int si = 0; si < P.nsums; si++) {
-				snprintf(b, sizeof b, "\t\t\tpart_tuple_value(P, t, %d, g, %s);\n", si,
+				snprintf(b, sizeof b, "\t\t\tpart_tuple_value(P, L, t, %d, g, %s);\n", si,
 					sum_expr(si, tail_join, tail_pay).c_str());
 				s += b;
 			}
