@@ -74,7 +74,8 @@ def pipelines():
 
 @pytest.mark.parametrize("name,desc", list(pipelines()))
 def test_generated_source_compiles(name, desc):
-    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH):
+    for mode in (abi.VH_GROUP_AUTO, abi.VH_GROUP_LDS, abi.VH_GROUP_GLOBAL, abi.VH_GROUP_HASH, abi.VH_GROUP_LDSHASH,
+                 abi.VH_GROUP_PART):
         for layout in (abi.VH_LAYOUT_AUTO, abi.VH_LAYOUT_HASH):
             for kernel in (abi.VH_KERNEL_JIT, abi.VH_KERNEL_JIT_LAZY, abi.VH_KERNEL_JIT_PIPE):
                 src = source(desc, group_mode=mode, join_layout=layout, kernel=kernel)

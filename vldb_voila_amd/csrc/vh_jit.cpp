@@ -921,9 +921,8 @@ struct Gen {
 					note_role(T.b, std::string(b) + ".b", false, P.njoins + 1);
 			}
 		// The vector main loop needs every probe column aligned to a pair of
-		// its values, an even U and a grid-strided row mapping (not GB_PART's
-		// slab per workgroup); otherwise every row takes the scalar path.
-		vec = U % 2 == 0 && mode != GB_PART;
+		// its values and an even U; otherwise every row takes the scalar path.
+		vec = U % 2 == 0;
 		if (radix)
 			pipe = false;   // the last join's lookups are in the join kernel
 		int row_bytes = 0;
@@ -969,11 +968,11 @@ struct Gen {
 					(double)P.rows * host_width(c.type) < 2147483000.0;   // buffer offsets are 32-bit
 		}
 
-		if (radix) {
-			radix_loops(mode);
-		} else if (mode == GB_PART) {
-			// rows of this workgroup: one contiguous slab (a batch item), the
-			// region its tuples are kept for
+		// rows of this workgroup only (local): a batch item, or GB_PART's
+		// slab, the region its tuples are kept for; else grid-strided
+		bool local = batch;
+		std::string lrb = "I.r0", lre = "I.r1";
+		if (!radix && mode == GB_PART) {
 			if (batch) {
 				line("\tconst i64 first = I.r0;");
 				line("\tconst i64 end = I.r1;");
@@ -985,20 +984,25 @@ struct Gen {
 			}
 			line("\tL.rbase = first;");
 			line("\tL.rcap = end > first ? (u32)(end - first) : 0u;");
-			scalar_loop(mode, "first", "end", "tile");
+			local = true;
+			lrb = "first";
+			lre = "end";
+		}
+		if (radix) {
+			radix_loops(mode);
 		} else if (!vec) {
-			if (batch)
-				scalar_loop(mode, "I.r0", "I.r1", "tile");
+			if (local)
+				scalar_loop(mode, lrb, lre, "tile");
 			else
 				scalar_loop(mode, "(i64)blockIdx.x * tile", "rows", "(i64)gridDim.x * tile");
 		} else {
 			// full chunks of 64U rows of [rb, re), one per wave at a time, then
-			// the remaining rows (fewer than 64U) on the scalar path.  A batch
-			// item's rows go to the waves of this workgroup (rb is even: items
-			// start at multiples of 1024 rows)
+			// the remaining rows (fewer than 64U) on the scalar path.  A
+			// workgroup's own rows go to its waves (rb is even: items and
+			// GB_PART slabs start at multiples of 1024 rows)
 			line("\tconst int lane = threadIdx.x & 63;");
-			if (batch) {
-				line("\tconst i64 rb = I.r0, re = I.r1;");
+			if (local) {
+				line("\tconst i64 rb = " + lrb + ", re = " + lre + ";");
 				line("\tconst i64 wv = (i64)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));");
 				line("\tconst i64 nwv = (i64)(blockDim.x >> 6);");
 			} else {
@@ -1023,7 +1027,7 @@ struct Gen {
 			vec_chunk(mode, false);
 			line("\t}");
 			line("\t}");
-			if (batch)
+			if (local)
 				scalar_loop(mode, "rb + nfull * (64 * U)", "re", "tile");
 			else
 				scalar_loop(mode, "rb + nfull * (64 * U) + (i64)blockIdx.x * tile", "re", "(i64)gridDim.x * tile");
