@@ -224,18 +224,95 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(i64* colv, u64 rows, 
 	}
 }
 
-// aggr_sum into a 128-bit column (lo, hi interleaved) / aggr_count.
+// aggr_sum into a 128-bit column (lo, hi interleaved) / aggr_count.  The
+// lanes of a wave that share a row (few groups: most of them) first add
+// their values in registers, one row at a time for up to kAggrRounds rows,
+// and one lane adds the exact 128-bit sum into the workgroup's LDS table
+// (kAggrSlots rows, slot = row mod kAggrSlots); lanes left over (a wave
+// over many groups) add their own values there.  A row whose slot holds
+// another row adds into the column with atomics; at the end the workgroup
+// adds each slot into the column.
+static constexpr int kAggrRounds = 16;
+static constexpr int kAggrSlots = 64;
+
+struct AggrSlots {
+	u64 key[kAggrSlots];   // row + 1, 0 = free
+	u64 lo[kAggrSlots];
+	u64 hi[kAggrSlots];    // two's complement high word
+};
+
+__device__ __forceinline__ void aggr_add(AggrSlots& S, u64* col2, u64 b, u64 lo, i64 hi)
+{
+	const int s = (int)(b & (kAggrSlots - 1));
+	u64 k = S.key[s];
+	if (k == 0) {
+		k = atomicCAS(&S.key[s], 0ull, b);
+		if (k == 0)
+			k = b;
+	}
+	if (k != b) {
+		atomic_add_i128(&col2[2 * (b - 1)], (i64*)&col2[2 * (b - 1) + 1], lo, hi);
+		return;
+	}
+	const u64 old = atomicAdd(&S.lo[s], lo);
+	const u64 h = (u64)hi + ((old + lo < old) ? 1ull : 0ull);
+	if (h)
+		atomicAdd(&S.hi[s], h);
+}
+
 __global__ __launch_bounds__(kThreads) void aggr_kernel(u64* col2, u64 rows, const int32_t* sel, i64 inum,
 	const u64* bucket, int vt, const void* val)
 {
-	for (i64 k = (i64)blockIdx.x * kThreads + threadIdx.x; k < inum; k += (i64)gridDim.x * kThreads) {
-		const i64 i = pos_of(sel, k);
-		const u64 b = bucket[i];
-		if (!b || b > rows)
-			continue;
-		const i64 v = val ? load1(val, vt, i) : 1;
-		const bool sgn = val ? (vt == VH_I8 || vt == VH_I16 || vt == VH_I32 || vt == VH_I64) : false;
-		atomic_add_i128(&col2[2 * (b - 1)], (i64*)&col2[2 * (b - 1) + 1], (u64)v, (sgn && v < 0) ? -1 : 0);
+	__shared__ AggrSlots S;
+	for (int s = threadIdx.x; s < kAggrSlots; s += kThreads) {
+		S.key[s] = 0;
+		S.lo[s] = 0;
+		S.hi[s] = 0;
+	}
+	__syncthreads();
+	const bool sgn = val ? (vt == VH_I8 || vt == VH_I16 || vt == VH_I32 || vt == VH_I64) : false;
+	const int lane = threadIdx.x & (kWave - 1);
+	// a workgroup-uniform trip count: every lane of a wave takes part in the
+	// wave-wide steps
+	for (i64 k0 = (i64)blockIdx.x * kThreads; k0 < inum; k0 += (i64)gridDim.x * kThreads) {
+		const i64 k = k0 + threadIdx.x;
+		u64 b = 0;
+		i64 v = 0;
+		if (k < inum) {
+			const i64 i = pos_of(sel, k);
+			b = bucket[i];
+			if (b && b <= rows)
+				v = val ? load1(val, vt, i) : 1;
+			else
+				b = 0;
+		}
+		const u64 lo = (u64)v;
+		const i64 hi = (sgn && v < 0) ? -1 : 0;
+		bool pending = b != 0;
+#pragma unroll 1
+		for (int round = 0; round < kAggrRounds; round++) {
+			const u64 m = __ballot(pending);
+			if (!m)
+				break;
+			const int leader = __ffsll((long long)m) - 1;
+			const u64 lb = ((u64)(u32)__shfl((int)(u32)(b >> 32), leader, kWave) << 32) |
+				(u64)(u32)__shfl((int)(u32)b, leader, kWave);
+			const bool mine = pending && b == lb;
+			u64 slo = mine ? lo : 0ull;
+			i64 shi = mine ? hi : 0;
+			wave_sum_i128(slo, shi);
+			if (lane == leader)
+				aggr_add(S, col2, lb, slo, shi);
+			pending = pending && !mine;
+		}
+		if (pending)
+			aggr_add(S, col2, b, lo, hi);
+	}
+	__syncthreads();
+	for (int s = threadIdx.x; s < kAggrSlots; s += kThreads) {
+		const u64 k = S.key[s];
+		if (k)
+			atomic_add_i128(&col2[2 * (k - 1)], (i64*)&col2[2 * (k - 1) + 1], S.lo[s], (i64)S.hi[s]);
 	}
 }
 
