@@ -325,17 +325,46 @@ __global__ __launch_bounds__(kThreads) void aggr_kernel(u64* col2, u64 rows, con
 // CAS is a conflict too.  Rows are only taken by winners, so the table stays
 // dense.  A full table sets *overflow and restores the head.
 static constexpr u64 kLocked = ~0ull;
+static constexpr int kInsertRounds = 8;
 
 __global__ __launch_bounds__(kThreads) void bucket_insert_kernel(u64* heads, u64 mask, u64* next,
 	u64* hashcol, u64 max_rows, u64* counters, u32* overflow, const int32_t* sel, i64 inum, u64* res,
 	const u64* hash)
 {
 	const u64 first_new = counters[1];
-	for (i64 k = (i64)blockIdx.x * kThreads + threadIdx.x; k < inum; k += (i64)gridDim.x * kThreads) {
-		const i64 i = pos_of(sel, k);
-		const u64 h = hash[i];
-		const u64 idx = h & mask;
-		res[i] = 0;
+	const int lane = threadIdx.x & (kWave - 1);
+	// a workgroup-uniform trip count: every lane of a wave takes part in the
+	// wave-wide steps
+	for (i64 k0 = (i64)blockIdx.x * kThreads; k0 < inum; k0 += (i64)gridDim.x * kThreads) {
+		const i64 k = k0 + threadIdx.x;
+		bool go = k < inum;
+		i64 i = 0;
+		u64 h = 0, idx = 0;
+		if (go) {
+			i = pos_of(sel, k);
+			h = hash[i];
+			idx = h & mask;
+			res[i] = 0;
+		}
+		// lanes of a wave with the same bucket head (few groups: most of
+		// them): only the first tries it; the others would find it locked or
+		// holding this call's row, a conflict either way
+		bool pending = go;
+#pragma unroll 1
+		for (int round = 0; round < kInsertRounds; round++) {
+			const u64 m = __ballot(pending);
+			if (!m)
+				break;
+			const int leader = __ffsll((long long)m) - 1;
+			const u64 li = ((u64)(u32)__shfl((int)(u32)(idx >> 32), leader, kWave) << 32) |
+				(u64)(u32)__shfl((int)(u32)idx, leader, kWave);
+			const bool same = pending && idx == li;
+			if (same && lane != leader)
+				go = false;
+			pending = pending && !same;
+		}
+		if (!go)
+			continue;
 		const u64 head = __hip_atomic_load(&heads[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (head == kLocked || (head && head - 1 >= first_new))
 			continue;
