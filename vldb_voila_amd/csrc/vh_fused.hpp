@@ -317,6 +317,30 @@ __device__ __forceinline__ u32 wave_append(u32* ctr)
 // first wave: each lane sums a contiguous segment, a wave scan of the
 // segment sums, then each lane rewrites its segment.  One barrier; every
 // thread gets the total (through *total, in LDS).
+// Rows laid out as runs (one per region or chunk): pre[r] = exclusive prefix
+// of the run lengths over n >= 1 runs (pre[0] = 0, non-decreasing).  For each
+// row ordinal i[v], the last run r[v] with pre[r[v]] <= i[v], by binary
+// lifting over a fixed step sequence.  Every step reads one clamped, always
+// valid word per row and updates the run with a select, so the V reads of a
+// step are issued together and waited for once (a data-dependent search
+// loop per row, or a read inside a branch, waits once per read).
+template <int V>
+__device__ __forceinline__ void runs_of(const u32* pre, int n, const u32 (&i)[V], int (&r)[V])
+{
+#pragma unroll
+	for (int v = 0; v < V; v++)
+		r[v] = 0;
+	for (int step = n > 1 ? 1 << (31 - __builtin_clz((unsigned)(n - 1))) : 0; step > 0; step >>= 1) {
+		u32 pm[V];
+#pragma unroll
+		for (int v = 0; v < V; v++)
+			pm[v] = pre[min(r[v] + step, n - 1)];
+#pragma unroll
+		for (int v = 0; v < V; v++)
+			r[v] += (r[v] + step < n && pm[v] <= i[v]) ? step : 0;
+	}
+}
+
 __device__ __forceinline__ u32 block_exclusive_scan(u32* a, int n, u32* total)
 {
 	if (threadIdx.x < (unsigned)kWave) {

@@ -259,22 +259,6 @@ __device__ __forceinline__ void spill_insert(const BuildPlan& B, u32 off, u64 va
 	}
 }
 
-// Where row i of partition p lies in pbuf: the last chunk b with pre[b] <= i
-// (pre = exclusive prefix of the partition's run lengths, base = the run's
-// start minus pre[b]).
-__device__ __forceinline__ u32 part_row_index(const u32* pre, const u32* base, int G, u32 i)
-{
-	int lo = 0, hi = G - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (pre[mid] <= i)
-			lo = mid;
-		else
-			hi = mid - 1;
-	}
-	return base[lo] + i;
-}
-
 // Partition `p` of a partitioned hash table: its rows go into an LDS copy of
 // its block of S = 2^pshift slots with LDS compare-and-swap and linear
 // probing that wraps within the block (the probe kernels follow the same
@@ -300,11 +284,25 @@ __device__ __forceinline__ void part_build_hash_slice(const BuildPlan& B, int p,
 	bool repeated = false;
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
 		u64 v[V];
+		u32 iv[V];
+		int rv[V];
 #pragma unroll
-		for (int u = 0; u < V; u++) {
-			const u32 i = i0 + u * blockDim.x + threadIdx.x;
-			v[u] = i < total ? gload(B.pbuf, (i64)part_row_index(pre, base, G, i)) : kEmpty;
-		}
+		for (int u = 0; u < V; u++)
+			iv[u] = i0 + u * blockDim.x + threadIdx.x;
+		runs_of<V>(pre, G, iv, rv);
+		u32 at[V];
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			at[u] = base[rv[u]];
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			at[u] = iv[u] < total ? at[u] + iv[u] : 0u;
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			v[u] = gload(B.pbuf, (i64)at[u]);
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			v[u] = iv[u] < total ? v[u] : kEmpty;
 #pragma unroll
 		for (int u = 0; u < V; u++) {
 			if (v[u] == kEmpty)
@@ -362,37 +360,49 @@ __device__ __forceinline__ void part_build_slice(const BuildPlan& B, int p, u32*
 	for (int b = threadIdx.x; b < G; b += blockDim.x)
 		base[b] = (u32)b * kPartChunk + base[b] - pre[b];
 	__syncthreads();
+	const u32 idle_key = (u32)(threadIdx.x % nkeys);
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
 		u64 v[V];
+		u32 iv[V];
+		int rv[V];
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			iv[u] = i0 + u * blockDim.x + threadIdx.x;
+		runs_of<V>(pre, G, iv, rv);
+		// every lane loads (rows past the total read pbuf[0]), so the loads
+		// are issued together
+		u32 at[V];
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			at[u] = base[rv[u]];
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			at[u] = iv[u] < total ? at[u] + iv[u] : 0u;
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			v[u] = gload(B.pbuf, (i64)at[u]);
+#pragma unroll
+		for (int u = 0; u < V; u++)
+			v[u] = iv[u] < total ? v[u] : kEmpty;
+		// every lane's claims are issued before any result is tested (a row
+		// past the total ANDs all-ones into a word of the slice chosen by its
+		// thread, which changes nothing and keeps such no-ops off one word)
+		u32* word[V];
+		u32 mask[V], seen[V];
 #pragma unroll
 		for (int u = 0; u < V; u++) {
-			const u32 i = i0 + u * blockDim.x + threadIdx.x;
-			v[u] = kEmpty;
-			if (i < total) {
-				int lo = 0, hi = G - 1;
-				while (lo < hi) {
-					const int mid = (lo + hi + 1) >> 1;
-					if (pre[mid] <= i)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				v[u] = gload(B.pbuf, (i64)base[lo] + i);
-			}
+			const bool has = v[u] != kEmpty;
+			u32 shift;
+			word[u] = dir_word(slice, w, has ? (u32)v[u] - (u32)k0 : idle_key, mask[u], shift);
+			const u32 bits = has ? ((((u32)(v[u] >> 32) << shift) & mask[u]) | ~mask[u]) : 0xFFFFFFFFu;
+			seen[u] = atomicAnd(word[u], bits);
 		}
 #pragma unroll
 		for (int u = 0; u < V; u++) {
-			if (v[u] == kEmpty)
-				continue;
-			const u32 off = (u32)v[u];
-			u32 mask, shift;
-			u32* word = dir_word(slice, w, off - (u32)k0, mask, shift);
-			const u32 bits = ((u32)(v[u] >> 32) << shift) & mask;
-			const u32 seen = atomicAnd(word, bits | ~mask);
-			if ((seen & mask) != mask) {
-				atomicOr(word, seen & mask);
+			if (v[u] != kEmpty && (seen[u] & mask[u]) != mask[u]) {
+				atomicOr(word[u], seen[u] & mask[u]);
 				*B.dups = 1u;
-				overflow_insert(B, off, v[u]);
+				overflow_insert(B, (u32)v[u], v[u]);
 			}
 		}
 	}
