@@ -694,19 +694,20 @@ __device__ __forceinline__ void part_agg_t(const Plan& P, int p, u64* lds, u32* 
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
 		bool has[V];
 		u64 wd[V][KW];
+		u32 iv[V], at[V];
+		int rv[V];
+#pragma unroll
+		for (int v = 0; v < V; v++)
+			iv[v] = i0 + v * blockDim.x + threadIdx.x;
+		runs_of<V>(pre, R, iv, rv);
+#pragma unroll
+		for (int v = 0; v < V; v++)
+			at[v] = base[rv[v]];
 #pragma unroll
 		for (int v = 0; v < V; v++) {
-			const u32 i = i0 + v * blockDim.x + threadIdx.x;
+			const u32 i = iv[v];
 			has[v] = i < total;
-			int a = 0, b = R - 1;
-			while (a < b) {
-				const int mid = (a + b + 1) >> 1;
-				if (pre[mid] <= i)
-					a = mid;
-				else
-					b = mid - 1;
-			}
-			const VH_GLOBAL u64* t = (const VH_GLOBAL u64*)P.gtup1 + (u64)(has[v] ? base[a] + i : 0u) * W;
+			const VH_GLOBAL u64* t = (const VH_GLOBAL u64*)P.gtup1 + (u64)(has[v] ? at[v] + i : 0u) * W;
 #pragma unroll
 			for (int w = 0; w < KW; w++)
 				wd[v][w] = (has[v] && w < W) ? t[w] : 0ull;
