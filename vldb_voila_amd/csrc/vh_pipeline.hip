@@ -166,13 +166,34 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 	// by build_row after the predicates
 	const bool packed = B.nkeys > 1 || B.npay > 1;
 	i64 key[kU], pv[kU];
+	if (!packed && B.key_type == VH_I32 && (!B.npay || B.pay_type == VH_I32)) {
+		// int32 key and payload (or none): the 2 kU loads in straight-line
+		// code, issued together (the typed loads below sit in a switch per
+		// column and row, and each waits for its value)
+		const int32_t* ps = B.npay ? (const int32_t*)B.pay : (const int32_t*)B.key;
+		int32_t kk[kU], pp[kU];
 #pragma unroll
-	for (int u = 0; u < kU; u++) {
-		const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
-		const i64 rr = r < B.rows ? r : 0;
-		key[u] = packed ? B.kmin : load1(B.key, B.key_type, rr);
-		pv[u] = (B.npay && !packed) ? load1(B.pay, B.pay_type, rr) : B.pmin;
-		ok[u] = r < B.rows;
+		for (int u = 0; u < kU; u++) {
+			const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
+			const i64 rr = r < B.rows ? r : 0;
+			kk[u] = gload((const int32_t*)B.key, rr);
+			pp[u] = gload(ps, rr);
+			ok[u] = r < B.rows;
+		}
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			key[u] = kk[u];
+			pv[u] = B.npay ? (i64)pp[u] : B.pmin;
+		}
+	} else {
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
+			const i64 rr = r < B.rows ? r : 0;
+			key[u] = packed ? B.kmin : load1(B.key, B.key_type, rr);
+			pv[u] = (B.npay && !packed) ? load1(B.pay, B.pay_type, rr) : B.pmin;
+			ok[u] = r < B.rows;
+		}
 	}
 #pragma unroll
 	for (int p = 0; p < VH_MAX_PREDS; p++) {
