@@ -691,6 +691,7 @@ __device__ __forceinline__ void part_agg_t(const Plan& P, int p, u64* lds, u32* 
 	__syncthreads();
 	// every word of a thread's V tuples is loaded before the LDS adds
 	const int W = P.grec / 8;
+	const u32 idle = threadIdx.x % (u32)ng;   // ng >= 1: this partition's groups
 	for (u32 i0 = 0; i0 < total; i0 += V * blockDim.x) {
 		bool has[V];
 		u64 wd[V][KW];
@@ -712,18 +713,39 @@ __device__ __forceinline__ void part_agg_t(const Plan& P, int p, u64* lds, u32* 
 			for (int w = 0; w < KW; w++)
 				wd[v][w] = (has[v] && w < W) ? t[w] : 0ull;
 		}
+		// every lane adds (0 into a slot of its own for a tuple past the
+		// total), so each sum's V atomics are issued together and a tracked
+		// sum waits once for its V old values, not once per tuple
+		u64 li[V];
 #pragma unroll
-		for (int v = 0; v < V; v++) {
-			if (!has[v])
-				continue;
-			const u64 li = (u32)wd[v][0] - g0;
-			atomicAdd(&cnt[li], 1u);
+		for (int v = 0; v < V; v++)
+			li[v] = has[v] ? (u64)((u32)wd[v][0] - (u32)g0) : (u64)idle;
 #pragma unroll
-			for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
-				if (s2 < ns) {
-					const SumArg& A = P.sums[s2];
-					add_lds_exact(&lo[(u64)s2 * S + li], &hi[(u64)s2 * S + li], part_field(P, wd[v], s2),
-						P.wide && A.wide, A.arg_signed);
+		for (int v = 0; v < V; v++)
+			atomicAdd(&cnt[li[v]], has[v] ? 1u : 0u);
+#pragma unroll
+		for (int s2 = 0; s2 < VH_MAX_AGGRS; s2++) {
+			if (s2 < ns) {
+				const SumArg& A = P.sums[s2];
+				u64 x[V];
+#pragma unroll
+				for (int v = 0; v < V; v++)
+					x[v] = has[v] ? part_field(P, wd[v], s2) : 0ull;
+				if (P.wide && A.wide) {
+					u64 o[V];
+#pragma unroll
+					for (int v = 0; v < V; v++)
+						o[v] = atomicAdd(&lo[(u64)s2 * S + li[v]], x[v]);
+#pragma unroll
+					for (int v = 0; v < V; v++) {
+						const int h = (int)((o[v] + x[v]) < o[v]) - (int)(A.arg_signed && (i64)x[v] < 0);
+						if (h)
+							atomicAdd(&hi[(u64)s2 * S + li[v]], h);
+					}
+				} else {
+#pragma unroll
+					for (int v = 0; v < V; v++)
+						atomicAdd(&lo[(u64)s2 * S + li[v]], x[v]);
 				}
 			}
 		}
