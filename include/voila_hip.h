@@ -209,7 +209,8 @@ typedef struct vh_config {
 	int32_t group_mode;    /* enum vh_group_mode                               */
 	int32_t grid;          /* probe-kernel workgroups, 0 = auto                */
 	int32_t table_shift;   /* join table size vs calc_num_buckets: <<n / >>-n  */
-	int32_t tile_shift;    /* batched work item = 16384 << n rows (n < 0: >>)  */
+	int32_t tile_shift;    /* batched work item = 16384 << n rows (n < 0: >>); */
+	                       /* 8192 << n for a partitioned group-by           */
 	int32_t join_layout;   /* enum vh_join_layout                              */
 	int32_t kernel;        /* enum vh_kernel (single queries; batches run the
 	                          generic kernel)                                  */

@@ -36,7 +36,8 @@ static const int kLayout[3] = {VH_LAYOUT_AUTO, VH_LAYOUT_HASH, VH_LAYOUT_HASH};
 static const int kPartShift[3] = {0, 13, -1};
 // launch shapes: {threads per workgroup, rows per lane (tile = threads x
 // rows), workgroups (0 = the lowering's: two 256-thread ones per CU),
-// batched work item 16384 << tile_shift rows}.  Waves per CU: 8, 16, 8, 16.
+// batched work item 16384 << tile_shift rows, 8192 << tile_shift for a
+// partitioned group-by}.  Waves per CU: 8, 16, 8, 16.
 static const int kShape[4][4] = {{256, 8, 0, 0}, {256, 2, 1024, -1}, {512, 4, 0, 0}, {1024, 4, 0, 1}};
 
 struct vh_mlp {
