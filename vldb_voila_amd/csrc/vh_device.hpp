@@ -101,6 +101,35 @@ __device__ __forceinline__ i64 load1(const void* p, int t, i64 i)
 	}
 }
 
+// load1 of N rows of one column: the type is dispatched once, so the N loads
+// sit in one block and are issued together (load1 per row puts each load in
+// a switch case of its own, and each waits for its value).
+template <typename T, int N>
+__device__ __forceinline__ void loadn_typed(const void* p, const i64 (&i)[N], i64 (&v)[N])
+{
+	T x[N];
+#pragma unroll
+	for (int u = 0; u < N; u++)
+		x[u] = gload((const T*)p, i[u]);
+#pragma unroll
+	for (int u = 0; u < N; u++)
+		v[u] = (i64)x[u];
+}
+
+template <int N>
+__device__ __forceinline__ void loadn(const void* p, int t, const i64 (&i)[N], i64 (&v)[N])
+{
+	switch (t) {
+	case VH_U8: loadn_typed<uint8_t, N>(p, i, v); break;
+	case VH_I8: loadn_typed<int8_t, N>(p, i, v); break;
+	case VH_U16: loadn_typed<uint16_t, N>(p, i, v); break;
+	case VH_I16: loadn_typed<int16_t, N>(p, i, v); break;
+	case VH_U32: loadn_typed<uint32_t, N>(p, i, v); break;
+	case VH_I32: loadn_typed<int32_t, N>(p, i, v); break;
+	default: loadn_typed<i64, N>(p, i, v); break;
+	}
+}
+
 // Four consecutive elements [i, i+4) with one or two 16-byte loads
 // (i % 4 == 0 and a 16-byte aligned column, checked on the host).
 __device__ __forceinline__ void load4(const void* p, int t, i64 i, i64 v[4])

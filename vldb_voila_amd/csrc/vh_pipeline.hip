@@ -186,25 +186,32 @@ __device__ __forceinline__ void part_scatter_chunk(const BuildPlan& B, int chunk
 			pv[u] = B.npay ? (i64)pp[u] : B.pmin;
 		}
 	} else {
+		i64 rr[kU];
 #pragma unroll
 		for (int u = 0; u < kU; u++) {
 			const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
-			const i64 rr = r < B.rows ? r : 0;
-			key[u] = packed ? B.kmin : load1(B.key, B.key_type, rr);
-			pv[u] = (B.npay && !packed) ? load1(B.pay, B.pay_type, rr) : B.pmin;
+			rr[u] = r < B.rows ? r : 0;
 			ok[u] = r < B.rows;
+			key[u] = B.kmin;
+			pv[u] = B.pmin;
+		}
+		if (!packed) {
+			loadn<kU>(B.key, B.key_type, rr, key);
+			if (B.npay)
+				loadn<kU>(B.pay, B.pay_type, rr, pv);
 		}
 	}
 #pragma unroll
 	for (int p = 0; p < VH_MAX_PREDS; p++) {
 		if (p < B.npreds) {
 			const Pred& Q = B.preds[p];
-			i64 x[kU];
+			i64 x[kU], rr[kU];
 #pragma unroll
 			for (int u = 0; u < kU; u++) {
 				const i64 r = r0 + (i64)u * blockDim.x + threadIdx.x;
-				x[u] = load1(Q.r.ptr, Q.r.type, r < B.rows ? r : 0);
+				rr[u] = r < B.rows ? r : 0;
 			}
+			loadn<kU>(Q.r.ptr, Q.r.type, rr, x);
 #pragma unroll
 			for (int u = 0; u < kU; u++)
 				ok[u] = ok[u] && compare(Q.cmp, Q.r.type, x[u], Q.value);
